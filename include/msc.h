@@ -1,0 +1,200 @@
+/*
+ * msc.h - C ABI of the MI355X meteor-trail FDTD time-stepping core.
+ *
+ * The reference solver (source/meteor-fork.cpp, built by source/makefile:6-9)
+ * is one monolithic main() that reads four binary files from the working
+ * directory, runs the time loop (meteor-fork.cpp:683-1667) and writes five
+ * binary files plus log.txt.  This header splits that process into the entry
+ * points a host program binds: read the run directory, create a solver (the
+ * set-up part of main), advance it, and fetch the same payloads the reference
+ * writes.  The `meteor` executable built from this library keeps the
+ * reference's process contract (no argv, files in the cwd).
+ *
+ * Conventions: every function returns 0 on success and a negative MSC_E*
+ * code on failure; msc_last_error() returns a thread-local message for the
+ * last failure on the calling thread.  No C++ exception crosses this ABI.
+ * Host buffers are owned by the caller; device memory is owned by the
+ * solver.  A solver handle is used by one host thread at a time; distinct
+ * handles may run concurrently (each owns its own HIP stream).
+ *
+ * Array layout: every 3-D array is C row-major [nx][ny][nz] (z fastest),
+ * the reference's own layout (meteor-fork.cpp:248-283).
+ */
+#ifndef MSC_H
+#define MSC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSC_MAX_PROBES 64
+#define MSC_MAX_NTF 256
+
+/* status codes */
+#define MSC_OK 0
+#define MSC_EINVAL -1   /* bad argument or inconsistent parameters */
+#define MSC_EIO -2      /* missing or truncated file */
+#define MSC_ENOMEM -3   /* device or host allocation failed */
+#define MSC_EHIP -4     /* HIP runtime error */
+#define MSC_ESTATE -5   /* call not valid in the handle's state */
+#define MSC_EBUSY -6    /* server: no capacity right now */
+
+/* field component ids for msc_solver_get_field */
+enum {
+  MSC_EX = 0, MSC_EY, MSC_EZ, MSC_HX, MSC_HY, MSC_HZ, MSC_JX, MSC_JY, MSC_JZ
+};
+
+/* source types: meteor-fork.cpp:160-179; 4 is an extension */
+enum {
+  MSC_SRC_SIN = 1, MSC_SRC_MODGAUSS = 2, MSC_SRC_RICKER = 3, MSC_SRC_GAUSSDERIV = 4
+};
+
+/*
+ * Run parameters.  The first block is inputs.dat in file order
+ * (reader meteor-fork.cpp:84-115, writer setup/meteorrun.m:45-69), then
+ * B0.dat (meteor-fork.cpp:221-226).  The last block is this build's
+ * optional extension record (absent from the reference; zero/default values
+ * reproduce the reference exactly).
+ */
+typedef struct msc_params {
+  int32_t dopml;
+  int32_t savefields;
+  double f0;
+  double fsig;
+  int32_t sourcetype;
+  int32_t sourcepol;
+  double gridfactor;
+  double dt;
+  int32_t tsteps;
+  int32_t nx, ny, nz;
+  double metsize[2];
+  double wpmax;
+  double numax;
+  double metloc[3];
+  double metangle;
+  int32_t mettype;
+  int32_t numfiles;
+  int32_t nprobes;
+  int32_t probex[MSC_MAX_PROBES];
+  int32_t probey[MSC_MAX_PROBES];
+  int32_t probez[MSC_MAX_PROBES];
+  int32_t nntf;
+  double fntf[MSC_MAX_NTF];
+  int32_t doplate;
+  double platesize;
+  int32_t doB0;        /* fork only; meteorrun.m never writes it (EOF -> 0) */
+  double B0[3];        /* tesla, B0.dat */
+  /* ---- extension record ("MSCX" block after doB0 in inputs.dat) ---- */
+  int32_t pmllen;      /* CPML thickness in cells; 0 -> 10 (meteor-fork.cpp:374) */
+  int32_t media3d;     /* 1: wp.dat/nu.dat hold [nx][ny][nz], no z-taper */
+  double nu_scale_height; /* >0: nu *= exp(-(z - metloc[2]) / H) */
+  double src_tau;      /* sourcetype 4 width (s); 0 -> 1/(pi f0) */
+} msc_params;
+
+/* Fill `p` with the defaults of setup/meteordefaults.m (plus doB0=0). */
+void msc_params_default(msc_params* p);
+
+/* Validate a parameter set (sizes, cadence, probe indices, ...). */
+int msc_params_validate(const msc_params* p);
+
+/* Thread-local message for the last failing call on this thread. */
+const char* msc_last_error(void);
+
+/* Library version string. */
+const char* msc_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Run-directory I/O (the reference's file contract, Appendix layouts) */
+/* ------------------------------------------------------------------ */
+
+/*
+ * Read inputs.dat, B0.dat, wp.dat and nu.dat from `dir`
+ * (replaces meteor-fork.cpp:49-117, 215-226, 336-347).  On success *wp and
+ * *nu point to malloc'd arrays of nx*ny doubles ([nx][ny], row-major, the
+ * byte order the reference's fread gives) or nx*ny*nz when p->media3d;
+ * release them with msc_free.  A missing trailing doB0 means 0 (the
+ * reference leaves it indeterminate).
+ */
+int msc_read_run_dir(const char* dir, msc_params* p, double** wp, double** nu);
+
+/* Write inputs.dat (with doB0 and, when any extension field differs from its
+ * default, the extension record), B0.dat, wp.dat and nu.dat. */
+int msc_write_run_dir(const char* dir, const msc_params* p, const double* wp, const double* nu);
+
+void msc_free(void* ptr);
+
+/* ------------------------------------------------------------------ */
+/* Single solve                                                        */
+/* ------------------------------------------------------------------ */
+
+typedef struct msc_solver msc_solver;
+
+/*
+ * Create a solver on the current HIP device: host FP64 precompute of the
+ * media, CPML tables, source table and incident line (meteor-fork.cpp:
+ * 119-646), device state allocated and zeroed.  `precision` is 32 (fields,
+ * J and CPML in FP32; plasma coefficients FP64; NTFF sums FP64) or 64.
+ * `wp`/`nu` as returned by msc_read_run_dir.
+ */
+int msc_solver_create(const msc_params* p, const double* wp, const double* nu,
+                      int precision, msc_solver** out);
+
+/* Enqueue `nsteps` time steps (meteor-fork.cpp:683-1667) on the solver's
+ * stream.  Asynchronous: returns before the GPU finishes. */
+int msc_solver_step(msc_solver* s, int nsteps);
+
+/* Wait for all enqueued work. */
+int msc_solver_sync(msc_solver* s);
+
+/* Steps completed so far (enqueued). */
+int msc_solver_time(const msc_solver* s);
+
+/* Copy one field component to host as FP64, [nx][ny][nz].  The J
+ * components are those the reference holds after the same number of steps. */
+int msc_solver_get_field(msc_solver* s, int comp, double* out);
+
+/* Number of doubles in the NTF payload: Eydft[2M], Ezdft[2M], then the 24
+ * face arrays of output_NTF.dat in file order (meteor-fork.cpp:1677-1714). */
+size_t msc_solver_ntff_count(const msc_solver* s);
+int msc_solver_get_ntff(msc_solver* s, double* out, size_t n);
+
+/* Probe histories (Probe.dat body after the index header): 6 arrays
+ * [tsteps][nprobes] (Ex,Ey,Ez,Hx,Hy,Hz), meteor-fork.cpp:1719-1729. */
+int msc_solver_get_probes(msc_solver* s, double* out, size_t n);
+
+/* Current E and H frames in output_E.dat / output_H.dat layout
+ * (meteor-fork.cpp:1640-1660): slices z, y, x (2 components each) then the
+ * two incident lines.  Each frame holds 2(nx*ny + nx*nz + ny*nz) + 2nx doubles. */
+size_t msc_solver_frame_count(const msc_solver* s);
+int msc_solver_get_frames(msc_solver* s, double* eframe, double* hframe);
+
+/* max |Ex| over the grid (meteor-fork.cpp:1621-1630). */
+int msc_solver_max_abs_ex(msc_solver* s, double* out);
+
+/* Device bytes held by the solver. */
+size_t msc_solver_device_bytes(const msc_solver* s);
+
+/* Average duration (ms) of the fused step kernel over the last
+ * msc_solver_step call, measured with HIP events on the solver's stream;
+ * 0 when timing is off.  msc_solver_set_timing(s, 1) turns it on. */
+int msc_solver_set_timing(msc_solver* s, int on);
+double msc_solver_kernel_ms(const msc_solver* s);
+
+void msc_solver_destroy(msc_solver* s);
+
+/*
+ * Whole drop-in run in `dir`: read the files, solve, write output_E.dat,
+ * output_H.dat, output_J.dat (when savefields), output_NTF.dat, Probe.dat
+ * and log.txt with the reference's layouts.  This is main() of
+ * meteor-fork.cpp:42-1742.  `precision` as in msc_solver_create.
+ */
+int msc_run_dir(const char* dir, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSC_H */

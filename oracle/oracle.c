@@ -1,0 +1,945 @@
+/*
+ * oracle.c - FP64 CPU restatement of source/meteor-fork.cpp (the solver the
+ * reference makefile builds, source/makefile:6-9).
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).  Each block cites the reference
+ * lines it restates.  Arithmetic is written in the reference's evaluation
+ * order (left-to-right, same divisions, glibc exp/sin/cos/pow) so results
+ * are bit-identical to the reference on well-defined inputs.  Deliberate
+ * differences, all outside bit-parity cases:
+ *   - grid coordinates use one loop per axis (meteor-fork.cpp:137-141 runs
+ *     all three to Nx, leaving z[k>=Nx] undefined);
+ *   - the source DFT accumulators start at zero (never zeroed at :586-587);
+ *   - Lee-Kalluri at nu==0 or |B|==0 uses the finite limits instead of the
+ *     reference's division by zero (:1547-1550);
+ *   - the dead |E| / |S| pass (:1488-1518) is not computed: nothing reads it;
+ *   - extension parameters (pmllen, 3-D media, nu scale height, source 4).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define PI 3.141592653589793
+#define QE 1.602177e-19
+#define E0 8.85419e-12
+#define U0 1.2566e-6
+#define ME 9.10939e-31
+#define CLIGHT 3.0000e8
+
+struct orc {
+  msc_params p;
+  int nx, ny, nz, L, M, T, t;
+  double dx, dy, dz, dt;
+  double c1h, c2h, c1e0, c2e0, c1es, c2es;
+  double *x, *y, *z;
+  double *srcA, *srcB;
+  /* fields [nx][ny][nz] */
+  double *ex, *ey, *ez, *hx, *hy, *hz, *jx, *jy, *jz;
+  double *wpe, *nue;
+  /* incident line */
+  double *eyi, *ezi, *hyi, *hzi;
+  double *peyxI, *pezxI, *phyxI, *phzxI;
+  /* cpml tables, 2L each (all axes share them: dx == dy == dz) */
+  double *be, *ae, *bh, *ah;
+  /* psi slabs */
+  double *p_hxy, *p_hxz, *p_hyx, *p_hyz, *p_hzx, *p_hzy;
+  double *p_exy, *p_exz, *p_eyx, *p_eyz, *p_ezx, *p_ezy;
+  /* ntff */
+  int ntfb;
+  double *eydft, *ezdft;
+  double *fb[8], *tb[8], *lr[8]; /* file order within each group */
+  /* probes [T][nprobes] x 6 */
+  double *probes;
+  /* slices */
+  double *eslz, *esly, *eslx, *hslz, *hsly, *hslx;
+  /* Lee-Kalluri gyro vector */
+  double wce[3], wce0;
+  int bx1, bx2, by1, by2, bz1, bz2;
+  int ssx, ssy, ssz;
+};
+
+#define IDX(o, i, j, k) ((((size_t)(i)) * (o)->ny + (size_t)(j)) * (o)->nz + (size_t)(k))
+
+static double* zalloc(size_t n) { return (double*)calloc(n ? n : 1, sizeof(double)); }
+
+/* ---- set-up --------------------------------------------------------- */
+
+/* Source waveforms, meteor-fork.cpp:153-179 (+ extension type 4). */
+static void make_sources(orc* o) {
+  const msc_params* p = &o->p;
+  double f0 = p->f0, fsig = p->fsig, dt = o->dt;
+  double rdelayA = 1.6 * sqrt(6) / PI / f0;
+  double rdelayB = (1.6 + 1 / sqrt(3)) * sqrt(6) / PI / f0;
+  double tau = p->src_tau > 0 ? p->src_tau : 1.0 / (PI * f0);
+  for (int t = 0; t < o->T; t++) {
+    double a, b;
+    if (p->sourcetype == 2) {
+      a = sin(2 * PI * f0 * (t - 3 * fsig) * dt) * exp(-(t - 3 * fsig) * (t - 3 * fsig) / (2 * fsig * fsig));
+      b = cos(2 * PI * f0 * (t - 3 * fsig) * dt) * exp(-(t - 3 * fsig) * (t - 3 * fsig) / (2 * fsig * fsig));
+    } else if (p->sourcetype == 3) {
+      a = (1 - 2 * PI * PI * f0 * f0 * (t * dt - rdelayA) * (t * dt - rdelayA)) *
+          exp(-PI * PI * f0 * f0 * (t * dt - rdelayA) * (t * dt - rdelayA));
+      b = (1 - 2 * PI * PI * f0 * f0 * (t * dt - rdelayB) * (t * dt - rdelayB)) *
+          exp(-PI * PI * f0 * f0 * (t * dt - rdelayB) * (t * dt - rdelayB));
+    } else if (p->sourcetype == 4) {
+      /* extension: unit-peak Gaussian derivative, centred at 4 tau */
+      double u = (t * dt - 4 * tau) / tau, v = (t * dt - 5 * tau) / tau;
+      a = -u * exp(0.5 - 0.5 * u * u);
+      b = -v * exp(0.5 - 0.5 * v * v);
+    } else {
+      a = sin(2 * PI * f0 * t * dt);
+      b = cos(2 * PI * f0 * t * dt);
+    }
+    o->srcA[t] = a;
+    o->srcB[t] = b;
+  }
+}
+
+/* CPML tables, meteor-fork.cpp:444-561 (kappa_max = 1, alpha_max = 0). */
+static void make_cpml(orc* o) {
+  int L = o->L;
+  double pmlm = 4;
+  double simax = 1.5 * (double)(pmlm + 1) / (150 * PI * o->dx);
+  double kamax = 1, almax = 0;
+  for (int n = 0; n < L; n++) {
+    double mf = (double)n;
+    double s = simax * pow(((mf + 0.5) / L), pmlm);
+    double sm = simax * pow(((mf + 1) / L), pmlm);
+    double k = 1 + (kamax - 1) * pow(((mf + 0.5) / L), pmlm);
+    double km = 1 + (kamax - 1) * pow(((mf + 1) / L), pmlm);
+    double a = almax * pow(((L - mf - 0.5) / L), pmlm);
+    double am = almax * pow(((L - mf - 1) / L), pmlm);
+    int m = n + L;
+    o->be[m] = exp(-((s / k) + a) * o->dt / E0);
+    o->ae[m] = s / (s * k + pow(k, 2) * a) * (o->be[m] - 1);
+    o->bh[m] = exp(-((sm / km) + am) * o->dt / E0);
+    o->ah[m] = sm / (sm * km + pow(km, 2) * am) * (o->bh[m] - 1);
+  }
+  for (int m = 0; m < L; m++) {
+    o->be[m] = o->be[2 * L - 1 - m];
+    o->ae[m] = o->ae[2 * L - 1 - m];
+    o->bh[m] = o->bh[2 * L - 1 - m];
+    o->ah[m] = o->ah[2 * L - 1 - m];
+  }
+}
+
+void orc_lk_matrices(double nu, double dt, const double w[3], double A[9], double K[9]) {
+  /* meteor-fork.cpp:1546-1573, evaluated in the same order */
+  double wx = w[0], wy = w[1], wz = w[2];
+  double w0 = sqrt(wx * wx + wy * wy + wz * wz);
+  if (nu > 0 && w0 > 0) {
+    double Ee1 = exp(-nu * dt);
+    double Ee2 = 1 / (w0 * w0 + nu * nu);
+    double Se1 = sin(w0 * dt) / w0;
+    double Ce1 = (1 - cos(w0 * dt)) / (w0 * w0);
+    double Ce2 = (1 - Ee1) / nu - Ee1 * nu * Ce1 - Ee1 * Se1;
+    double Ce3 = nu * (1 - Ee1 * cos(w0 * dt)) + Ee1 * w0 * sin(w0 * dt);
+    double Ce4 = 1 - Ee1 * cos(w0 * dt) - Ee1 * nu * Se1;
+    double c = cos(w0 * dt);
+    A[0] = Ee1 * (Ce1 * wx * wx + c);
+    A[1] = Ee1 * (Ce1 * wx * wy - Se1 * wz);
+    A[2] = Ee1 * (Ce1 * wx * wz + Se1 * wy);
+    A[3] = Ee1 * (Ce1 * wy * wx + Se1 * wz);
+    A[4] = Ee1 * (Ce1 * wy * wy + c);
+    A[5] = Ee1 * (Ce1 * wy * wz - Se1 * wx);
+    A[6] = Ee1 * (Ce1 * wz * wx - Se1 * wy);
+    A[7] = Ee1 * (Ce1 * wz * wy + Se1 * wx);
+    A[8] = Ee1 * (Ce1 * wz * wz + c);
+    K[0] = Ee2 * (Ce2 * wx * wx + Ce3);
+    K[1] = Ee2 * (Ce2 * wx * wy - Ce4 * wz);
+    K[2] = Ee2 * (Ce2 * wx * wz + Ce4 * wy);
+    K[3] = Ee2 * (Ce2 * wy * wx + Ce4 * wz);
+    K[4] = Ee2 * (Ce2 * wy * wy + Ce3);
+    K[5] = Ee2 * (Ce2 * wy * wz - Ce4 * wx);
+    K[6] = Ee2 * (Ce2 * wz * wx - Ce4 * wy);
+    K[7] = Ee2 * (Ce2 * wz * wy + Ce4 * wx);
+    K[8] = Ee2 * (Ce2 * wz * wz + Ce3);
+    return;
+  }
+  /* Finite limits of the same integrals: A = exp(M dt), K = int_0^dt exp(M s) ds,
+   * M = -nu I + [w x].  Not reachable in the reference without NaN. */
+  double a = exp(-nu * dt);
+  double kI, kX, kP, cth, sth_w, omc_w2;
+  if (w0 > 0) {
+    cth = cos(w0 * dt);
+    sth_w = sin(w0 * dt) / w0;
+    omc_w2 = (1 - cth) / (w0 * w0);
+    /* nu == 0 */
+    kI = sth_w;
+    kX = omc_w2;
+    kP = (dt - sth_w) / (w0 * w0);
+  } else {
+    cth = 1;
+    sth_w = dt;
+    omc_w2 = 0.5 * dt * dt;
+    kI = nu > 0 ? -expm1(-nu * dt) / nu : dt;
+    kX = 0;
+    kP = 0;
+  }
+  double ww[3] = {wx, wy, wz};
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) {
+      double P = ww[r] * ww[c];
+      double I = (r == c) ? 1.0 : 0.0;
+      /* [w x] entries: X[r][c] = -eps_{r c l} w_l */
+      double X = 0;
+      if (r == 0 && c == 1) X = -wz;
+      if (r == 0 && c == 2) X = wy;
+      if (r == 1 && c == 0) X = wz;
+      if (r == 1 && c == 2) X = -wx;
+      if (r == 2 && c == 0) X = -wy;
+      if (r == 2 && c == 1) X = wx;
+      A[3 * r + c] = a * (omc_w2 * P + cth * I + sth_w * X);
+      K[3 * r + c] = kP * P + kI * I + kX * X;
+    }
+}
+
+/* Inputs the reference would crash on or leave undefined. */
+static const char* orc_check(const msc_params* p) {
+  int L = p->pmllen > 0 ? p->pmllen : 10;
+  if (p->nx < 2 * L + 22 || p->ny < 2 * L + 22 || p->nz < 2 * L + 22) return "grid too small for CPML + TF/SF box";
+  if (p->tsteps < 1) return "tsteps < 1";
+  if (p->nntf < 1 || p->nntf > MSC_MAX_NTF) return "nntf out of range";
+  if (p->nprobes < 0 || p->nprobes > MSC_MAX_PROBES) return "nprobes out of range";
+  for (int n = 0; n < p->nprobes; n++)
+    if (p->probex[n] < 1 || p->probex[n] >= p->nx || p->probey[n] < 1 || p->probey[n] >= p->ny ||
+        p->probez[n] < 1 || p->probez[n] >= p->nz)
+      return "probe index out of range";
+  if (!(p->dt > 0) || !(p->f0 > 0) || !(p->gridfactor > 0)) return "dt, f0 and gridfactor must be > 0";
+  return NULL;
+}
+
+orc* orc_create(const msc_params* p, const double* wp, const double* nu, char* err, int errlen) {
+  const char* why = orc_check(p);
+  if (why) {
+    if (err) snprintf(err, errlen, "%s", why);
+    return NULL;
+  }
+  orc* o = (orc*)calloc(1, sizeof(orc));
+  o->p = *p;
+  o->nx = p->nx;
+  o->ny = p->ny;
+  o->nz = p->nz;
+  o->L = p->pmllen > 0 ? p->pmllen : 10;
+  o->M = p->nntf;
+  o->T = p->tsteps;
+  o->dt = p->dt;
+  /* meteor-fork.cpp:119-135 */
+  double lambda = CLIGHT / p->f0;
+  o->dx = lambda / p->gridfactor;
+  o->dy = lambda / p->gridfactor;
+  o->dz = lambda / p->gridfactor;
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  o->x = zalloc(nx);
+  o->y = zalloc(ny);
+  o->z = zalloc(nz);
+  for (int i = 0; i < nx; i++) o->x[i] = -(nx - 1) / 2 * o->dx + o->dx * i;
+  for (int j = 0; j < ny; j++) o->y[j] = -(ny - 1) / 2 * o->dy + o->dy * j;
+  for (int k = 0; k < nz; k++) o->z[k] = -(nz - 1) / 2 * o->dz + o->dz * k;
+  o->ssx = (nx + 1) / 2;
+  o->ssy = (ny + 1) / 2;
+  o->ssz = (nz + 1) / 2;
+  o->srcA = zalloc(o->T);
+  o->srcB = zalloc(o->T);
+  make_sources(o);
+  /* meteor-fork.cpp:181-213 */
+  double sigm = 0, sig = 0, dt = o->dt;
+  o->c1h = (2 * U0 - sigm * dt) / (2 * U0 + sigm * dt);
+  o->c2h = 2 * dt / (2 * U0 + sigm * dt);
+  o->c1e0 = (2 * E0 - sig * dt) / (2 * E0 + sig * dt);
+  o->c2e0 = 2 * dt / (2 * E0 + sig * dt);
+  o->c1es = (2 * E0 - 1e7 * dt) / (2 * E0 + 1e7 * dt);
+  o->c2es = 2 * dt / (2 * E0 + 1e7 * dt);
+  /* meteor-fork.cpp:228-232 */
+  o->wce[0] = -QE * p->B0[0] / ME;
+  o->wce[1] = -QE * p->B0[1] / ME;
+  o->wce[2] = -QE * p->B0[2] / ME;
+  o->wce0 = sqrt(o->wce[0] * o->wce[0] + o->wce[1] * o->wce[1] + o->wce[2] * o->wce[2]);
+  size_t N = (size_t)nx * ny * nz;
+  o->ex = zalloc(N); o->ey = zalloc(N); o->ez = zalloc(N);
+  o->hx = zalloc(N); o->hy = zalloc(N); o->hz = zalloc(N);
+  o->jx = zalloc(N); o->jy = zalloc(N); o->jz = zalloc(N);
+  o->wpe = zalloc(N); o->nue = zalloc(N);
+  o->eyi = zalloc(nx); o->ezi = zalloc(nx); o->hyi = zalloc(nx); o->hzi = zalloc(nx);
+  /* meteor-fork.cpp:336-366: extrude the 2-D maps along z with a Gaussian taper */
+  if (!p->doplate) {
+    double msig0 = p->metsize[0];
+    for (int i = 0; i < nx; i++)
+      for (int j = 0; j < ny; j++)
+        for (int k = 0; k < nz; k++) {
+          size_t c = IDX(o, i, j, k);
+          if (p->media3d) {
+            o->wpe[c] = wp[c];
+            o->nue[c] = nu[c];
+          } else {
+            double g = exp(-pow((o->z[k] - p->metloc[2]), 2) / (msig0 * msig0));
+            o->wpe[c] = wp[(size_t)i * ny + j] * g;
+            o->nue[c] = nu[(size_t)i * ny + j] * g;
+          }
+          if (p->nu_scale_height > 0) o->nue[c] *= exp(-(o->z[k] - p->metloc[2]) / p->nu_scale_height);
+        }
+  }
+  /* CPML, meteor-fork.cpp:369-561 */
+  int L = o->L;
+  o->be = zalloc(2 * L); o->ae = zalloc(2 * L); o->bh = zalloc(2 * L); o->ah = zalloc(2 * L);
+  if (p->dopml) make_cpml(o);
+  o->p_hxy = zalloc((size_t)nx * 2 * L * nz);
+  o->p_hxz = zalloc((size_t)nx * ny * 2 * L);
+  o->p_hyx = zalloc((size_t)2 * L * ny * nz);
+  o->p_hyz = zalloc((size_t)nx * ny * 2 * L);
+  o->p_hzx = zalloc((size_t)2 * L * ny * nz);
+  o->p_hzy = zalloc((size_t)nx * 2 * L * nz);
+  o->p_exy = zalloc((size_t)nx * 2 * L * nz);
+  o->p_exz = zalloc((size_t)nx * ny * 2 * L);
+  o->p_eyx = zalloc((size_t)2 * L * ny * nz);
+  o->p_eyz = zalloc((size_t)nx * ny * 2 * L);
+  o->p_ezx = zalloc((size_t)2 * L * ny * nz);
+  o->p_ezy = zalloc((size_t)nx * 2 * L * nz);
+  o->peyxI = zalloc(2 * L); o->pezxI = zalloc(2 * L); o->phyxI = zalloc(2 * L); o->phzxI = zalloc(2 * L);
+  /* meteor-fork.cpp:570-583 */
+  o->bx1 = L + 10; o->bx2 = nx - L - 10;
+  o->by1 = L + 10; o->by2 = ny - L - 10;
+  o->bz1 = L + 10; o->bz2 = nz - L - 10;
+  o->ntfb = L + 5;
+  o->eydft = zalloc(2 * o->M);
+  o->ezdft = zalloc(2 * o->M);
+  for (int q = 0; q < 8; q++) {
+    o->fb[q] = zalloc((size_t)ny * nz * 2 * o->M);
+    o->tb[q] = zalloc((size_t)nx * nz * 2 * o->M);
+    o->lr[q] = zalloc((size_t)nx * ny * 2 * o->M);
+  }
+  o->probes = zalloc((size_t)6 * o->T * (p->nprobes > 0 ? p->nprobes : 1));
+  o->eslz = zalloc((size_t)2 * nx * ny); o->esly = zalloc((size_t)2 * nx * nz); o->eslx = zalloc((size_t)2 * ny * nz);
+  o->hslz = zalloc((size_t)2 * nx * ny); o->hsly = zalloc((size_t)2 * nx * nz); o->hslx = zalloc((size_t)2 * ny * nz);
+  return o;
+}
+
+/* ---- one time step: meteor-fork.cpp:683-1667 ------------------------ */
+
+#define F(a, i, j, k) (a)[IDX(o, i, j, k)]
+/* psi slab element helpers */
+#define PXY(a, i, j, k) (a)[(((size_t)(i)) * 2 * L + (size_t)(j)) * nz + (size_t)(k)]
+#define PXZ(a, i, j, k) (a)[(((size_t)(i)) * ny + (size_t)(j)) * 2 * L + (size_t)(k)]
+#define PYX(a, i, j, k) (a)[(((size_t)(i)) * ny + (size_t)(j)) * nz + (size_t)(k)]
+
+static void psi_h(orc* o) {
+  /* meteor-fork.cpp:686-791 */
+  int nx = o->nx, ny = o->ny, nz = o->nz, L = o->L;
+  int xs = nx - 2 * L - 1, ys = ny - 2 * L - 1, zs = nz - 2 * L - 1;
+  double dx = o->dx, dy = o->dy, dz = o->dz;
+  const double *bh = o->bh, *ah = o->ah;
+  for (int i = 0; i < nx; i++) {
+    for (int j = 0; j < L; j++)
+      for (int k = 0; k < nz - 1; k++)
+        PXY(o->p_hxy, i, j, k) = bh[j] * PXY(o->p_hxy, i, j, k) + ah[j] * (F(o->ez, i, j + 1, k) - F(o->ez, i, j, k)) / dy;
+    for (int j = ny - L - 1; j < ny - 1; j++)
+      for (int k = 0; k < nz - 1; k++)
+        PXY(o->p_hxy, i, j - ys, k) = bh[j - ys] * PXY(o->p_hxy, i, j - ys, k) + ah[j - ys] * (F(o->ez, i, j + 1, k) - F(o->ez, i, j, k)) / dy;
+  }
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny - 1; j++) {
+      for (int k = 0; k < L; k++)
+        PXZ(o->p_hxz, i, j, k) = bh[k] * PXZ(o->p_hxz, i, j, k) + ah[k] * (F(o->ey, i, j, k + 1) - F(o->ey, i, j, k)) / dz;
+      for (int k = nz - L - 1; k < nz - 1; k++)
+        PXZ(o->p_hxz, i, j, k - zs) = bh[k - zs] * PXZ(o->p_hxz, i, j, k - zs) + ah[k - zs] * (F(o->ey, i, j, k + 1) - F(o->ey, i, j, k)) / dz;
+    }
+  for (int i = 0; i < L; i++)
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz - 1; k++)
+        PYX(o->p_hyx, i, j, k) = bh[i] * PYX(o->p_hyx, i, j, k) + ah[i] * (F(o->ez, i + 1, j, k) - F(o->ez, i, j, k)) / dx;
+  for (int i = nx - L - 1; i < nx - 1; i++) {
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz - 1; k++)
+        PYX(o->p_hyx, i - xs, j, k) = bh[i - xs] * PYX(o->p_hyx, i - xs, j, k) + ah[i - xs] * (F(o->ez, i + 1, j, k) - F(o->ez, i, j, k)) / dx;
+    o->phyxI[i - xs] = bh[i - xs] * o->phyxI[i - xs] + ah[i - xs] * (o->ezi[i + 1] - o->ezi[i]) / dx;
+  }
+  for (int i = 0; i < nx - 1; i++)
+    for (int j = 0; j < ny; j++) {
+      for (int k = 0; k < L; k++)
+        PXZ(o->p_hyz, i, j, k) = bh[k] * PXZ(o->p_hyz, i, j, k) + ah[k] * (F(o->ex, i, j, k + 1) - F(o->ex, i, j, k)) / dz;
+      for (int k = nz - L - 1; k < nz - 1; k++)
+        PXZ(o->p_hyz, i, j, k - zs) = bh[k - zs] * PXZ(o->p_hyz, i, j, k - zs) + ah[k - zs] * (F(o->ex, i, j, k + 1) - F(o->ex, i, j, k)) / dz;
+    }
+  for (int i = 0; i < L; i++)
+    for (int j = 0; j < ny - 1; j++)
+      for (int k = 0; k < nz; k++)
+        PYX(o->p_hzx, i, j, k) = bh[i] * PYX(o->p_hzx, i, j, k) + ah[i] * (F(o->ey, i + 1, j, k) - F(o->ey, i, j, k)) / dx;
+  for (int i = nx - L - 1; i < nx - 1; i++) {
+    for (int j = 0; j < ny - 1; j++)
+      for (int k = 0; k < nz; k++)
+        PYX(o->p_hzx, i - xs, j, k) = bh[i - xs] * PYX(o->p_hzx, i - xs, j, k) + ah[i - xs] * (F(o->ey, i + 1, j, k) - F(o->ey, i, j, k)) / dx;
+    o->phzxI[i - xs] = bh[i - xs] * o->phzxI[i - xs] + ah[i - xs] * (o->eyi[i + 1] - o->eyi[i]) / dx;
+  }
+  for (int i = 0; i < nx - 1; i++) {
+    for (int j = 0; j < L; j++)
+      for (int k = 0; k < nz; k++)
+        PXY(o->p_hzy, i, j, k) = bh[j] * PXY(o->p_hzy, i, j, k) + ah[j] * (F(o->ex, i, j + 1, k) - F(o->ex, i, j, k)) / dy;
+    for (int j = ny - L - 1; j < ny - 1; j++)
+      for (int k = 0; k < nz; k++)
+        PXY(o->p_hzy, i, j - ys, k) = bh[j - ys] * PXY(o->p_hzy, i, j - ys, k) + ah[j - ys] * (F(o->ex, i, j + 1, k) - F(o->ex, i, j, k)) / dy;
+  }
+}
+
+static void update_h(orc* o) {
+  int nx = o->nx, ny = o->ny, nz = o->nz, L = o->L;
+  int xs = nx - 2 * L - 1, ys = ny - 2 * L - 1, zs = nz - 2 * L - 1;
+  double dx = o->dx, dy = o->dy, dz = o->dz, c1h = o->c1h, c2h = o->c2h;
+  int dopml = o->p.dopml;
+  int bx1 = o->bx1, bx2 = o->bx2, by1 = o->by1, by2 = o->by2, bz1 = o->bz1, bz2 = o->bz2;
+  /* Hx, meteor-fork.cpp:795-850 */
+#pragma omp parallel for
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny - 1; j++)
+      for (int k = 0; k < nz - 1; k++)
+        F(o->hx, i, j, k) = c1h * F(o->hx, i, j, k) - c2h * ((F(o->ez, i, j + 1, k) - F(o->ez, i, j, k)) / dy - (F(o->ey, i, j, k + 1) - F(o->ey, i, j, k)) / dz);
+  for (int i = bx1; i < bx2; i++)
+    for (int k = bz1; k < bz2; k++) {
+      F(o->hx, i, by1 - 1, k) = F(o->hx, i, by1 - 1, k) + c2h * o->ezi[i] / dy;
+      F(o->hx, i, by2 - 1, k) = F(o->hx, i, by2 - 1, k) - c2h * o->ezi[i] / dy;
+    }
+  for (int i = bx1; i < bx2; i++)
+    for (int j = by1; j < by2; j++) {
+      F(o->hx, i, j, bz1 - 1) = F(o->hx, i, j, bz1 - 1) - c2h * o->eyi[i] / dz;
+      F(o->hx, i, j, bz2 - 1) = F(o->hx, i, j, bz2 - 1) + c2h * o->eyi[i] / dz;
+    }
+  if (dopml) {
+    for (int i = 0; i < nx; i++) {
+      for (int j = 0; j < L; j++)
+        for (int k = 0; k < nz - 1; k++) F(o->hx, i, j, k) = F(o->hx, i, j, k) - c2h * PXY(o->p_hxy, i, j, k);
+      for (int j = ny - L - 1; j < ny - 1; j++)
+        for (int k = 0; k < nz - 1; k++) F(o->hx, i, j, k) = F(o->hx, i, j, k) - c2h * PXY(o->p_hxy, i, j - ys, k);
+    }
+    for (int i = 0; i < nx; i++)
+      for (int j = 0; j < ny - 1; j++) {
+        for (int k = 0; k < L; k++) F(o->hx, i, j, k) = F(o->hx, i, j, k) + c2h * PXZ(o->p_hxz, i, j, k);
+        for (int k = nz - L - 1; k < nz - 1; k++) F(o->hx, i, j, k) = F(o->hx, i, j, k) + c2h * PXZ(o->p_hxz, i, j, k - zs);
+      }
+  }
+  /* Hy, meteor-fork.cpp:853-910 */
+#pragma omp parallel for
+  for (int i = 0; i < nx - 1; i++) {
+    for (int j = 0; j < ny; j++)
+      for (int k = 0; k < nz - 1; k++)
+        F(o->hy, i, j, k) = c1h * F(o->hy, i, j, k) - c2h * ((F(o->ex, i, j, k + 1) - F(o->ex, i, j, k)) / dz - (F(o->ez, i + 1, j, k) - F(o->ez, i, j, k)) / dx);
+    o->hyi[i] = c1h * o->hyi[i] + c2h * (o->ezi[i + 1] - o->ezi[i]) / dx;
+  }
+  for (int j = by1; j < by2; j++)
+    for (int k = bz1; k < bz2; k++) {
+      F(o->hy, bx1 - 1, j, k) = F(o->hy, bx1 - 1, j, k) - c2h * o->ezi[bx1] / dx;
+      F(o->hy, bx2 - 1, j, k) = F(o->hy, bx2 - 1, j, k) + c2h * o->ezi[bx2 - 1] / dx;
+    }
+  if (dopml) {
+    for (int i = 0; i < L; i++)
+      for (int j = 0; j < ny; j++)
+        for (int k = 0; k < nz - 1; k++) F(o->hy, i, j, k) = F(o->hy, i, j, k) + c2h * PYX(o->p_hyx, i, j, k);
+    for (int i = nx - L - 1; i < nx - 1; i++) {
+      for (int j = 0; j < ny; j++)
+        for (int k = 0; k < nz - 1; k++) F(o->hy, i, j, k) = F(o->hy, i, j, k) + c2h * PYX(o->p_hyx, i - xs, j, k);
+      o->hyi[i] = o->hyi[i] + c2h * o->phyxI[i - xs];
+    }
+    for (int i = 0; i < nx - 1; i++)
+      for (int j = 0; j < ny; j++) {
+        for (int k = 0; k < L; k++) F(o->hy, i, j, k) = F(o->hy, i, j, k) - c2h * PXZ(o->p_hyz, i, j, k);
+        for (int k = nz - L - 1; k < nz - 1; k++) F(o->hy, i, j, k) = F(o->hy, i, j, k) - c2h * PXZ(o->p_hyz, i, j, k - zs);
+      }
+  }
+  /* Hz, meteor-fork.cpp:912-968 */
+#pragma omp parallel for
+  for (int i = 0; i < nx - 1; i++) {
+    for (int j = 0; j < ny - 1; j++)
+      for (int k = 0; k < nz; k++)
+        F(o->hz, i, j, k) = c1h * F(o->hz, i, j, k) - c2h * ((F(o->ey, i + 1, j, k) - F(o->ey, i, j, k)) / dx - (F(o->ex, i, j + 1, k) - F(o->ex, i, j, k)) / dy);
+    o->hzi[i] = c1h * o->hzi[i] - c2h * (o->eyi[i + 1] - o->eyi[i]) / dx;
+  }
+  for (int j = by1; j < by2; j++)
+    for (int k = bz1; k < bz2; k++) {
+      F(o->hz, bx1 - 1, j, k) = F(o->hz, bx1 - 1, j, k) + c2h * o->eyi[bx1] / dx;
+      F(o->hz, bx2 - 1, j, k) = F(o->hz, bx2 - 1, j, k) - c2h * o->eyi[bx2 - 1] / dx;
+    }
+  if (dopml) {
+    for (int i = 0; i < L; i++)
+      for (int j = 0; j < ny - 1; j++)
+        for (int k = 0; k < nz; k++) F(o->hz, i, j, k) = F(o->hz, i, j, k) - c2h * PYX(o->p_hzx, i, j, k);
+    for (int i = nx - L - 1; i < nx - 1; i++) {
+      for (int j = 0; j < ny - 1; j++)
+        for (int k = 0; k < nz; k++) F(o->hz, i, j, k) = F(o->hz, i, j, k) - c2h * PYX(o->p_hzx, i - xs, j, k);
+      o->hzi[i] = o->hzi[i] - c2h * o->phzxI[i - xs];
+    }
+    for (int i = 0; i < nx - 1; i++) {
+      for (int j = 0; j < L; j++)
+        for (int k = 0; k < nz; k++) F(o->hz, i, j, k) = F(o->hz, i, j, k) + c2h * PXY(o->p_hzy, i, j, k);
+      for (int j = ny - L - 1; j < ny - 1; j++)
+        for (int k = 0; k < nz; k++) F(o->hz, i, j, k) = F(o->hz, i, j, k) + c2h * PXY(o->p_hzy, i, j - ys, k);
+    }
+  }
+}
+
+/* Running DFT of one sample into acc[2m] (sin) and acc[2m+1] (cos). */
+static inline void dft_acc(double* acc, double v, const double* sn, const double* cs, int M) {
+  for (int m = 0; m < M; m++) {
+    acc[2 * m] += v * sn[m];
+    acc[2 * m + 1] += v * cs[m];
+  }
+}
+
+static void ntff_h(orc* o, const double* sn, const double* cs) {
+  /* meteor-fork.cpp:971-1023; face arrays in file order:
+   * fb: My_f Mz_f Jy_f Jz_f My_b Mz_b Jy_b Jz_b
+   * tb: Mx_t Mz_t Jx_t Jz_t Mx_bo Mz_bo Jx_bo Jz_bo
+   * lr: Mx_l My_l Jx_l Jy_l Mx_r My_r Jx_r Jy_r */
+  int nx = o->nx, ny = o->ny, nz = o->nz, M = o->M, b = o->ntfb;
+  for (int j = 0; j < ny; j++)
+    for (int k = 0; k < nz; k++) {
+      size_t e = ((size_t)j * nz + k) * 2 * M;
+      dft_acc(o->fb[2] + e, 0.5 * (F(o->hz, b, j, k) + F(o->hz, b - 1, j, k)), sn, cs, M);
+      dft_acc(o->fb[3] + e, -0.5 * (F(o->hy, b, j, k) + F(o->hy, b - 1, j, k)), sn, cs, M);
+      dft_acc(o->fb[6] + e, -0.5 * (F(o->hz, nx - b, j, k) + F(o->hz, nx - b - 1, j, k)), sn, cs, M);
+      dft_acc(o->fb[7] + e, 0.5 * (F(o->hy, nx - b, j, k) + F(o->hy, nx - b - 1, j, k)), sn, cs, M);
+    }
+  for (int i = 0; i < nx; i++)
+    for (int k = 0; k < nz; k++) {
+      size_t e = ((size_t)i * nz + k) * 2 * M;
+      dft_acc(o->tb[2] + e, 0.5 * (F(o->hz, i, ny - b, k) + F(o->hz, i, ny - b - 1, k)), sn, cs, M);
+      dft_acc(o->tb[3] + e, -0.5 * (F(o->hx, i, ny - b, k) + F(o->hx, i, ny - b - 1, k)), sn, cs, M);
+      dft_acc(o->tb[6] + e, -0.5 * (F(o->hz, i, b, k) + F(o->hz, i, b - 1, k)), sn, cs, M);
+      dft_acc(o->tb[7] + e, 0.5 * (F(o->hx, i, b, k) + F(o->hx, i, b - 1, k)), sn, cs, M);
+    }
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) {
+      size_t e = ((size_t)i * ny + j) * 2 * M;
+      dft_acc(o->lr[2] + e, 0.5 * (F(o->hy, i, j, b) + F(o->hy, i, j, b - 1)), sn, cs, M);
+      dft_acc(o->lr[3] + e, -0.5 * (F(o->hx, i, j, b) + F(o->hx, i, j, b - 1)), sn, cs, M);
+      dft_acc(o->lr[6] + e, -0.5 * (F(o->hy, i, j, nz - b) + F(o->hy, i, j, nz - b - 1)), sn, cs, M);
+      dft_acc(o->lr[7] + e, 0.5 * (F(o->hx, i, j, nz - b) + F(o->hx, i, j, nz - b - 1)), sn, cs, M);
+    }
+}
+
+static void ntff_e(orc* o, const double* sn, const double* cs) {
+  /* meteor-fork.cpp:1417-1470 */
+  int nx = o->nx, ny = o->ny, nz = o->nz, M = o->M, b = o->ntfb;
+  for (int j = 0; j < ny; j++)
+    for (int k = 0; k < nz; k++) {
+      size_t e = ((size_t)j * nz + k) * 2 * M;
+      dft_acc(o->fb[0] + e, -F(o->ez, b, j, k), sn, cs, M);
+      dft_acc(o->fb[1] + e, F(o->ey, b, j, k), sn, cs, M);
+      dft_acc(o->fb[4] + e, F(o->ez, nx - b, j, k), sn, cs, M);
+      dft_acc(o->fb[5] + e, -F(o->ey, nx - b, j, k), sn, cs, M);
+    }
+  for (int i = 0; i < nx; i++)
+    for (int k = 0; k < nz; k++) {
+      size_t e = ((size_t)i * nz + k) * 2 * M;
+      dft_acc(o->tb[0] + e, -F(o->ez, i, ny - b, k), sn, cs, M);
+      dft_acc(o->tb[1] + e, F(o->ex, i, ny - b, k), sn, cs, M);
+      dft_acc(o->tb[4] + e, F(o->ez, i, b, k), sn, cs, M);
+      dft_acc(o->tb[5] + e, -F(o->ex, i, b, k), sn, cs, M);
+    }
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) {
+      size_t e = ((size_t)i * ny + j) * 2 * M;
+      dft_acc(o->lr[0] + e, -F(o->ey, i, j, b), sn, cs, M);
+      dft_acc(o->lr[1] + e, F(o->ex, i, j, b), sn, cs, M);
+      dft_acc(o->lr[4] + e, F(o->ey, i, j, nz - b), sn, cs, M);
+      dft_acc(o->lr[5] + e, -F(o->ex, i, j, nz - b), sn, cs, M);
+    }
+}
+
+static void h_slices(orc* o) {
+  /* meteor-fork.cpp:1025-1045 */
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny - 1; j++) {
+      o->hslz[(size_t)i * ny + j] = F(o->hy, i, j, o->ssz);
+      o->hslz[(size_t)nx * ny + (size_t)i * ny + j] = F(o->hz, i, j, o->ssz);
+    }
+  for (int i = 0; i < nx; i++)
+    for (int k = 0; k < nz - 1; k++) {
+      o->hsly[(size_t)i * nz + k] = F(o->hy, i, o->ssy, k);
+      o->hsly[(size_t)nx * nz + (size_t)i * nz + k] = F(o->hz, i, o->ssy, k);
+    }
+  for (int j = 0; j < ny - 1; j++)
+    for (int k = 0; k < nz - 1; k++) {
+      o->hslx[(size_t)j * nz + k] = F(o->hy, o->ssx, j, k);
+      o->hslx[(size_t)ny * nz + (size_t)j * nz + k] = F(o->hz, o->ssx, j, k);
+    }
+}
+
+static void e_slices(orc* o) {
+  /* meteor-fork.cpp:1353-1371 (before the Ez CPML correction) */
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) {
+      o->eslz[(size_t)i * ny + j] = F(o->ey, i, j, o->ssz);
+      o->eslz[(size_t)nx * ny + (size_t)i * ny + j] = F(o->ez, i, j, o->ssz);
+    }
+  for (int i = 0; i < nx; i++)
+    for (int k = 0; k < nz; k++) {
+      o->esly[(size_t)i * nz + k] = F(o->ey, i, o->ssy, k);
+      o->esly[(size_t)nx * nz + (size_t)i * nz + k] = F(o->ez, i, o->ssy, k);
+    }
+  for (int j = 0; j < ny; j++)
+    for (int k = 0; k < nz; k++) {
+      o->eslx[(size_t)j * nz + k] = F(o->ey, o->ssx, j, k);
+      o->eslx[(size_t)ny * nz + (size_t)j * nz + k] = F(o->ez, o->ssx, j, k);
+    }
+}
+
+static void psi_e(orc* o) {
+  /* meteor-fork.cpp:1049-1156 */
+  int nx = o->nx, ny = o->ny, nz = o->nz, L = o->L;
+  int xs = nx - 2 * L - 1, ys = ny - 2 * L - 1, zs = nz - 2 * L - 1;
+  double dx = o->dx, dy = o->dy, dz = o->dz;
+  const double *be = o->be, *ae = o->ae;
+  for (int i = 0; i < nx - 1; i++) {
+    for (int j = 1; j < L + 1; j++)
+      for (int k = 1; k < nz - 1; k++)
+        PXY(o->p_exy, i, j - 1, k) = be[j - 1] * PXY(o->p_exy, i, j - 1, k) + ae[j - 1] * (F(o->hz, i, j, k) - F(o->hz, i, j - 1, k)) / dy;
+    for (int j = ny - L - 1; j < ny - 1; j++)
+      for (int k = 1; k < nz - 1; k++)
+        PXY(o->p_exy, i, j - ys, k) = be[j - ys] * PXY(o->p_exy, i, j - ys, k) + ae[j - ys] * (F(o->hz, i, j, k) - F(o->hz, i, j - 1, k)) / dy;
+  }
+  for (int i = 0; i < nx; i++)
+    for (int j = 1; j < ny - 1; j++) {
+      for (int k = 1; k < L + 1; k++)
+        PXZ(o->p_exz, i, j, k - 1) = be[k - 1] * PXZ(o->p_exz, i, j, k - 1) + ae[k - 1] * (F(o->hy, i, j, k) - F(o->hy, i, j, k - 1)) / dz;
+      for (int k = nz - L - 1; k < nz - 1; k++)
+        PXZ(o->p_exz, i, j, k - zs) = be[k - zs] * PXZ(o->p_exz, i, j, k - zs) + ae[k - zs] * (F(o->hy, i, j, k) - F(o->hy, i, j, k - 1)) / dz;
+    }
+  for (int i = 1; i < L + 1; i++)
+    for (int j = 0; j < ny - 1; j++)
+      for (int k = 1; k < nz - 1; k++)
+        PYX(o->p_eyx, i - 1, j, k) = be[i - 1] * PYX(o->p_eyx, i - 1, j, k) + ae[i - 1] * (F(o->hz, i, j, k) - F(o->hz, i - 1, j, k)) / dx;
+  for (int i = nx - L - 1; i < nx - 1; i++) {
+    for (int j = 0; j < ny - 1; j++)
+      for (int k = 1; k < nz - 1; k++)
+        PYX(o->p_eyx, i - xs, j, k) = be[i - xs] * PYX(o->p_eyx, i - xs, j, k) + ae[i - xs] * (F(o->hz, i, j, k) - F(o->hz, i - 1, j, k)) / dx;
+    o->peyxI[i - xs] = be[i - xs] * o->peyxI[i - xs] + ae[i - xs] * (o->hzi[i] - o->hzi[i - 1]) / dx;
+  }
+  for (int i = 1; i < nx - 1; i++)
+    for (int j = 0; j < ny - 1; j++) {
+      for (int k = 1; k < L + 1; k++)
+        PXZ(o->p_eyz, i, j, k - 1) = be[k - 1] * PXZ(o->p_eyz, i, j, k - 1) + ae[k - 1] * (F(o->hx, i, j, k) - F(o->hx, i, j, k - 1)) / dz;
+      for (int k = nz - L - 1; k < nz - 1; k++)
+        PXZ(o->p_eyz, i, j, k - zs) = be[k - zs] * PXZ(o->p_eyz, i, j, k - zs) + ae[k - zs] * (F(o->hx, i, j, k) - F(o->hx, i, j, k - 1)) / dz;
+    }
+  for (int i = 1; i < L + 1; i++)
+    for (int j = 1; j < ny - 1; j++)
+      for (int k = 0; k < nz - 1; k++)
+        PYX(o->p_ezx, i - 1, j, k) = be[i - 1] * PYX(o->p_ezx, i - 1, j, k) + ae[i - 1] * (F(o->hy, i, j, k) - F(o->hy, i - 1, j, k)) / dx;
+  for (int i = nx - L - 1; i < nx - 1; i++) {
+    for (int j = 1; j < ny - 1; j++)
+      for (int k = 0; k < nz - 1; k++)
+        PYX(o->p_ezx, i - xs, j, k) = be[i - xs] * PYX(o->p_ezx, i - xs, j, k) + ae[i - xs] * (F(o->hy, i, j, k) - F(o->hy, i - 1, j, k)) / dx;
+    o->pezxI[i - xs] = be[i - xs] * o->pezxI[i - xs] + ae[i - xs] * (o->hyi[i] - o->hyi[i - 1]) / dx;
+  }
+  for (int i = 1; i < nx - 1; i++) {
+    for (int j = 1; j < L + 1; j++)
+      for (int k = 0; k < nz - 1; k++)
+        PXY(o->p_ezy, i, j - 1, k) = be[j - 1] * PXY(o->p_ezy, i, j - 1, k) + ae[j - 1] * (F(o->hx, i, j, k) - F(o->hx, i, j - 1, k)) / dy;
+    for (int j = ny - L - 1; j < ny - 1; j++)
+      for (int k = 0; k < nz - 1; k++)
+        PXY(o->p_ezy, i, j - ys, k) = be[j - ys] * PXY(o->p_ezy, i, j - ys, k) + ae[j - ys] * (F(o->hx, i, j, k) - F(o->hx, i, j - 1, k)) / dy;
+  }
+}
+
+static inline int in_sphere(const orc* o, int i, int j, int k) {
+  /* meteor-fork.cpp:203-209 */
+  double dist = sqrt(o->x[i] * o->x[i] + o->y[j] * o->y[j] + o->z[k] * o->z[k]);
+  return dist < o->p.platesize;
+}
+
+static void update_e(orc* o) {
+  int nx = o->nx, ny = o->ny, nz = o->nz, L = o->L;
+  int xs = nx - 2 * L - 1, ys = ny - 2 * L - 1, zs = nz - 2 * L - 1;
+  double dx = o->dx, dy = o->dy, dz = o->dz, c1e0 = o->c1e0, c2e0 = o->c2e0;
+  int dopml = o->p.dopml, sphere = (o->p.doplate == 2);
+  int bx1 = o->bx1, bx2 = o->bx2, by1 = o->by1, by2 = o->by2, bz1 = o->bz1, bz2 = o->bz2;
+  /* Ex, meteor-fork.cpp:1160-1216 */
+#pragma omp parallel for
+  for (int i = 0; i < nx - 1; i++)
+    for (int j = 1; j < ny - 1; j++)
+      for (int k = 1; k < nz - 1; k++) {
+        double c1 = c1e0, c2 = c2e0;
+        if (sphere && in_sphere(o, i, j, k)) { c1 = o->c1es; c2 = o->c2es; }
+        F(o->ex, i, j, k) = c1 * F(o->ex, i, j, k) + c2 * ((F(o->hz, i, j, k) - F(o->hz, i, j - 1, k)) / dy - (F(o->hy, i, j, k) - F(o->hy, i, j, k - 1)) / dz) - c2 * F(o->jx, i, j, k);
+      }
+  for (int i = bx1; i < bx2 - 1; i++)
+    for (int j = by1; j < by2; j++) {
+      F(o->ex, i, j, bz1) = F(o->ex, i, j, bz1) + c2e0 * o->hyi[i] / dz;
+      F(o->ex, i, j, bz2) = F(o->ex, i, j, bz2) - c2e0 * o->hyi[i] / dz;
+    }
+  for (int i = bx1; i < bx2 - 1; i++)
+    for (int k = bz1; k < bz2; k++) {
+      F(o->ex, i, by1, k) = F(o->ex, i, by1, k) - c2e0 * o->hzi[i] / dy;
+      F(o->ex, i, by2, k) = F(o->ex, i, by2, k) + c2e0 * o->hzi[i] / dy;
+    }
+  if (dopml) {
+    for (int i = 0; i < nx - 1; i++) {
+      for (int j = 1; j < L + 1; j++)
+        for (int k = 1; k < nz - 1; k++) F(o->ex, i, j, k) = F(o->ex, i, j, k) + c2e0 * PXY(o->p_exy, i, j - 1, k);
+      for (int j = ny - L - 1; j < ny - 1; j++)
+        for (int k = 1; k < nz - 1; k++) F(o->ex, i, j, k) = F(o->ex, i, j, k) + c2e0 * PXY(o->p_exy, i, j - ys, k);
+    }
+    for (int i = 0; i < nx - 1; i++)
+      for (int j = 1; j < ny - 1; j++) {
+        for (int k = 1; k < L + 1; k++) F(o->ex, i, j, k) = F(o->ex, i, j, k) - c2e0 * PXZ(o->p_exz, i, j, k - 1);
+        for (int k = nz - L - 1; k < nz - 1; k++) F(o->ex, i, j, k) = F(o->ex, i, j, k) - c2e0 * PXZ(o->p_exz, i, j, k - zs);
+      }
+  }
+  /* Ey, meteor-fork.cpp:1220-1275 */
+#pragma omp parallel for
+  for (int i = 1; i < nx - 1; i++) {
+    for (int j = 0; j < ny; j++)
+      for (int k = 1; k < nz; k++) {
+        double c1 = c1e0, c2 = c2e0;
+        if (sphere && in_sphere(o, i, j, k)) { c1 = o->c1es; c2 = o->c2es; }
+        F(o->ey, i, j, k) = c1 * F(o->ey, i, j, k) + c2 * ((F(o->hx, i, j, k) - F(o->hx, i, j, k - 1)) / dz - (F(o->hz, i, j, k) - F(o->hz, i - 1, j, k)) / dx) - c2 * F(o->jy, i, j, k);
+      }
+    o->eyi[i] = c1e0 * o->eyi[i] - c2e0 * (o->hzi[i] - o->hzi[i - 1]) / dx;
+  }
+  for (int j = by1; j < by2; j++)
+    for (int k = bz1; k < bz2; k++) {
+      F(o->ey, bx1, j, k) = F(o->ey, bx1, j, k) + c2e0 * o->hzi[bx1 - 1] / dx;
+      F(o->ey, bx2 - 1, j, k) = F(o->ey, bx2 - 1, j, k) - c2e0 * o->hzi[bx2 - 1] / dx;
+    }
+  if (dopml) {
+    for (int i = 1; i < L + 1; i++)
+      for (int j = 0; j < ny; j++)
+        for (int k = 1; k < nz; k++) F(o->ey, i, j, k) = F(o->ey, i, j, k) - c2e0 * PYX(o->p_eyx, i - 1, j, k);
+    for (int i = nx - L - 1; i < nx - 1; i++) {
+      for (int j = 0; j < ny; j++)
+        for (int k = 1; k < nz; k++) F(o->ey, i, j, k) = F(o->ey, i, j, k) - c2e0 * PYX(o->p_eyx, i - xs, j, k);
+      o->eyi[i] = o->eyi[i] - c2e0 * o->peyxI[i - xs];
+    }
+    for (int i = 1; i < nx - 1; i++)
+      for (int j = 0; j < ny; j++) {
+        for (int k = 1; k < L + 1; k++) F(o->ey, i, j, k) = F(o->ey, i, j, k) + c2e0 * PXZ(o->p_eyz, i, j, k - 1);
+        for (int k = nz - L - 1; k < nz - 1; k++) F(o->ey, i, j, k) = F(o->ey, i, j, k) + c2e0 * PXZ(o->p_eyz, i, j, k - zs);
+      }
+  }
+  /* Ez, meteor-fork.cpp:1278-1302 (its CPML correction comes after the slices) */
+#pragma omp parallel for
+  for (int i = 1; i < nx - 1; i++) {
+    for (int j = 1; j < ny; j++)
+      for (int k = 0; k < nz; k++) {
+        double c1 = c1e0, c2 = c2e0;
+        if (sphere && in_sphere(o, i, j, k)) { c1 = o->c1es; c2 = o->c2es; }
+        F(o->ez, i, j, k) = c1 * F(o->ez, i, j, k) + c2 * ((F(o->hy, i, j, k) - F(o->hy, i - 1, j, k)) / dx - (F(o->hx, i, j, k) - F(o->hx, i, j - 1, k)) / dy) - c2 * F(o->jz, i, j, k);
+      }
+    o->ezi[i] = c1e0 * o->ezi[i] + c2e0 * (o->hyi[i] - o->hyi[i - 1]) / dx;
+  }
+  for (int j = by1; j < by2; j++)
+    for (int k = bz1; k < bz2; k++) {
+      F(o->ez, bx1, j, k) = F(o->ez, bx1, j, k) - c2e0 * o->hyi[bx1 - 1] / dx;
+      F(o->ez, bx2 - 1, j, k) = F(o->ez, bx2 - 1, j, k) + c2e0 * o->hyi[bx2 - 1] / dx;
+    }
+}
+
+static void hard_source_and_plate(orc* o, int t) {
+  /* meteor-fork.cpp:1308-1351 */
+  double a = o->srcA[t], b = o->srcB[t];
+  switch (o->p.sourcepol) {
+    case 2: o->ezi[0] = 0; o->eyi[0] = a; break;
+    case 3: o->ezi[0] = a; o->eyi[0] = a; break;
+    case 4: o->ezi[0] = a; o->eyi[0] = b; break;
+    case 5: o->ezi[0] = b; o->eyi[0] = a; break;
+    default: o->ezi[0] = a; o->eyi[0] = 0; break;
+  }
+  if (o->p.doplate == 1) {
+    int lo = (int)(o->ssy - o->p.platesize / 2 / o->dy);
+    int up = (int)(o->ssy + o->p.platesize / 2 / o->dy);
+    for (int j = lo; j < up - 1; j++)
+      for (int k = lo; k < up; k++) F(o->ey, o->ssx, j, k) = 0;
+    for (int j = lo; j < up; j++)
+      for (int k = lo; k < up - 1; k++) F(o->ez, o->ssx, j, k) = 0;
+  }
+}
+
+static void pml_ez(orc* o) {
+  /* meteor-fork.cpp:1373-1405 */
+  int nx = o->nx, ny = o->ny, nz = o->nz, L = o->L;
+  int xs = nx - 2 * L - 1, ys = ny - 2 * L - 1;
+  double c2e0 = o->c2e0;
+  for (int i = 1; i < L + 1; i++)
+    for (int j = 1; j < ny; j++)
+      for (int k = 0; k < nz; k++) F(o->ez, i, j, k) = F(o->ez, i, j, k) + c2e0 * PYX(o->p_ezx, i - 1, j, k);
+  for (int i = nx - L - 1; i < nx - 1; i++) {
+    for (int j = 1; j < ny; j++)
+      for (int k = 0; k < nz; k++) F(o->ez, i, j, k) = F(o->ez, i, j, k) + c2e0 * PYX(o->p_ezx, i - xs, j, k);
+    o->ezi[i] = o->ezi[i] + c2e0 * o->pezxI[i - xs];
+  }
+  for (int i = 1; i < nx - 1; i++) {
+    for (int j = 1; j < L + 1; j++)
+      for (int k = 0; k < nz - 1; k++) F(o->ez, i, j, k) = F(o->ez, i, j, k) - c2e0 * PXY(o->p_ezy, i, j - 1, k);
+    for (int j = ny - L - 1; j < ny - 1; j++)
+      for (int k = 0; k < nz - 1; k++) F(o->ez, i, j, k) = F(o->ez, i, j, k) - c2e0 * PXY(o->p_ezy, i, j - ys, k);
+  }
+}
+
+static void probes(orc* o, int t) {
+  /* meteor-fork.cpp:1474-1486 */
+  int np = o->p.nprobes;
+  size_t T = o->T;
+  for (int n = 0; n < np; n++) {
+    int ix = o->p.probex[n], iy = o->p.probey[n], iz = o->p.probez[n];
+    size_t e = (size_t)t * np + n;
+    o->probes[0 * T * np + e] = 0.5 * (F(o->ex, ix, iy, iz) + F(o->ex, ix - 1, iy, iz));
+    o->probes[1 * T * np + e] = 0.5 * (F(o->ey, ix, iy, iz) + F(o->ey, ix, iy - 1, iz));
+    o->probes[2 * T * np + e] = 0.5 * (F(o->ez, ix, iy, iz) + F(o->ez, ix, iy, iz - 1));
+    o->probes[3 * T * np + e] = 0.25 * (F(o->hx, ix, iy, iz) + F(o->hx, ix, iy - 1, iz) + F(o->hx, ix, iy, iz - 1) + F(o->hx, ix, iy - 1, iz - 1));
+    o->probes[4 * T * np + e] = 0.25 * (F(o->hy, ix, iy, iz) + F(o->hy, ix - 1, iy, iz) + F(o->hy, ix, iy, iz - 1) + F(o->hy, ix - 1, iy, iz - 1));
+    o->probes[5 * T * np + e] = 0.25 * (F(o->hz, ix, iy, iz) + F(o->hz, ix, iy - 1, iz) + F(o->hz, ix - 1, iy, iz) + F(o->hz, ix - 1, iy - 1, iz));
+  }
+}
+
+static void update_j(orc* o) {
+  int nx = o->nx, ny = o->ny, nz = o->nz;
+  double dt = o->dt;
+  if (!o->p.doB0) {
+    /* Young's direct integration, meteor-fork.cpp:1523-1537 */
+#pragma omp parallel for
+    for (int i = 1; i < nx - 1; i++)
+      for (int j = 1; j < ny - 1; j++)
+        for (int k = 1; k < nz - 1; k++) {
+          size_t c = IDX(o, i, j, k);
+          double nu = o->nue[c], wp = o->wpe[c];
+          o->jx[c] = ((2 - nu * dt) / (2 + nu * dt)) * o->jx[c] + 2 * ((E0 * wp * wp * dt) / (2 + nu * dt)) * o->ex[c];
+          o->jy[c] = ((2 - nu * dt) / (2 + nu * dt)) * o->jy[c] + 2 * ((E0 * wp * wp * dt) / (2 + nu * dt)) * o->ey[c];
+          o->jz[c] = ((2 - nu * dt) / (2 + nu * dt)) * o->jz[c] + 2 * ((E0 * wp * wp * dt) / (2 + nu * dt)) * o->ez[c];
+        }
+    return;
+  }
+  /* Lee & Kalluri magnetized update, meteor-fork.cpp:1539-1597 */
+#pragma omp parallel for
+  for (int i = 1; i < nx - 1; i++)
+    for (int j = 1; j < ny - 1; j++)
+      for (int k = 1; k < nz - 1; k++) {
+        size_t c = IDX(o, i, j, k);
+        double A[9], K[9], Em[3];
+        double wp = o->wpe[c];
+        orc_lk_matrices(o->nue[c], dt, o->wce, A, K);
+        Em[0] = (o->ex[c] + F(o->ex, i - 1, j, k)) / 2;
+        Em[1] = (o->ey[c] + F(o->ey, i, j - 1, k)) / 2;
+        Em[2] = (o->ez[c] + F(o->ez, i, j, k - 1)) / 2;
+        double jx0 = A[0] * o->jx[c] + A[1] * o->jy[c] + A[2] * o->jz[c] + E0 * wp * wp * (K[0] * Em[0] + K[1] * Em[1] + K[2] * Em[2]);
+        double jy0 = A[3] * o->jx[c] + A[4] * o->jy[c] + A[5] * o->jz[c] + E0 * wp * wp * (K[3] * Em[0] + K[4] * Em[1] + K[5] * Em[2]);
+        o->jz[c] = A[6] * o->jx[c] + A[7] * o->jy[c] + A[8] * o->jz[c] + E0 * wp * wp * (K[6] * Em[0] + K[7] * Em[1] + K[8] * Em[2]);
+        o->jx[c] = jx0;
+        o->jy[c] = jy0;
+      }
+}
+
+void orc_step(orc* o, int nsteps) {
+  int M = o->M;
+  double* sn = zalloc(M);
+  double* cs = zalloc(M);
+  for (int s = 0; s < nsteps && o->t < o->T; s++) {
+    int t = o->t;
+    for (int m = 0; m < M; m++) {
+      /* meteor-fork.cpp:977 / :1410: both use t*dt ((t+1/2) is integer) */
+      sn[m] = sin(2 * PI * o->p.fntf[m] * t * o->dt);
+      cs[m] = cos(2 * PI * o->p.fntf[m] * t * o->dt);
+    }
+    if (o->p.dopml) psi_h(o);
+    update_h(o);
+    ntff_h(o, sn, cs);
+    h_slices(o);
+    if (o->p.dopml) psi_e(o);
+    update_e(o);
+    hard_source_and_plate(o, t);
+    e_slices(o);
+    if (o->p.dopml) pml_ez(o);
+    for (int m = 0; m < M; m++) {
+      /* meteor-fork.cpp:1409-1414 */
+      o->ezdft[2 * m] += o->ezi[0] * sn[m];
+      o->ezdft[2 * m + 1] += o->ezi[0] * cs[m];
+      o->eydft[2 * m] += o->eyi[0] * sn[m];
+      o->eydft[2 * m + 1] += o->eyi[0] * cs[m];
+    }
+    ntff_e(o, sn, cs);
+    probes(o, t);
+    update_j(o);
+    o->t++;
+  }
+  free(sn);
+  free(cs);
+}
+
+int orc_time(const orc* o) { return o->t; }
+
+void orc_get_field(const orc* o, int comp, double* out) {
+  const double* src[9] = {o->ex, o->ey, o->ez, o->hx, o->hy, o->hz, o->jx, o->jy, o->jz};
+  if (comp < 0 || comp > 8) return;
+  memcpy(out, src[comp], sizeof(double) * (size_t)o->nx * o->ny * o->nz);
+}
+
+size_t orc_ntff_count(const orc* o) {
+  size_t M2 = 2 * (size_t)o->M;
+  return 2 * M2 + 8 * M2 * ((size_t)o->ny * o->nz + (size_t)o->nx * o->nz + (size_t)o->nx * o->ny);
+}
+
+void orc_get_ntff(const orc* o, double* out) {
+  size_t M2 = 2 * (size_t)o->M;
+  memcpy(out, o->eydft, sizeof(double) * M2);
+  out += M2;
+  memcpy(out, o->ezdft, sizeof(double) * M2);
+  out += M2;
+  size_t nfb = (size_t)o->ny * o->nz * M2, ntb = (size_t)o->nx * o->nz * M2, nlr = (size_t)o->nx * o->ny * M2;
+  for (int q = 0; q < 8; q++, out += nfb) memcpy(out, o->fb[q], sizeof(double) * nfb);
+  for (int q = 0; q < 8; q++, out += ntb) memcpy(out, o->tb[q], sizeof(double) * ntb);
+  for (int q = 0; q < 8; q++, out += nlr) memcpy(out, o->lr[q], sizeof(double) * nlr);
+}
+
+size_t orc_probe_count(const orc* o) { return (size_t)6 * o->T * o->p.nprobes; }
+
+void orc_get_probes(const orc* o, double* out) { memcpy(out, o->probes, sizeof(double) * orc_probe_count(o)); }
+
+size_t orc_frame_count(const orc* o) {
+  return 2 * ((size_t)o->nx * o->ny + (size_t)o->nx * o->nz + (size_t)o->ny * o->nz) + 2 * (size_t)o->nx;
+}
+
+void orc_get_frames(const orc* o, double* e, double* h) {
+  /* frame layout: meteor-fork.cpp:1640-1660 */
+  size_t a = 2 * (size_t)o->nx * o->ny, b = 2 * (size_t)o->nx * o->nz, c = 2 * (size_t)o->ny * o->nz;
+  if (e) {
+    memcpy(e, o->eslz, a * 8); memcpy(e + a, o->esly, b * 8); memcpy(e + a + b, o->eslx, c * 8);
+    memcpy(e + a + b + c, o->eyi, o->nx * 8); memcpy(e + a + b + c + o->nx, o->ezi, o->nx * 8);
+  }
+  if (h) {
+    memcpy(h, o->hslz, a * 8); memcpy(h + a, o->hsly, b * 8); memcpy(h + a + b, o->hslx, c * 8);
+    memcpy(h + a + b + c, o->hyi, o->nx * 8); memcpy(h + a + b + c + o->nx, o->hzi, o->nx * 8);
+  }
+}
+
+double orc_max_abs_ex(const orc* o) {
+  /* meteor-fork.cpp:1621-1630 */
+  double m = 0;
+  size_t N = (size_t)o->nx * o->ny * o->nz;
+  for (size_t c = 0; c < N; c++)
+    if (fabs(o->ex[c]) > m) m = fabs(o->ex[c]);
+  return m;
+}
+
+void orc_cpml_tables(const orc* o, double* be, double* ae, double* bh, double* ah) {
+  size_t n = 2 * (size_t)o->L * sizeof(double);
+  memcpy(be, o->be, n); memcpy(ae, o->ae, n); memcpy(bh, o->bh, n); memcpy(ah, o->ah, n);
+}
+
+void orc_source_tables(const orc* o, double* a, double* b) {
+  memcpy(a, o->srcA, sizeof(double) * o->T);
+  memcpy(b, o->srcB, sizeof(double) * o->T);
+}
+
+void orc_destroy(orc* o) {
+  if (!o) return;
+  double* arrs[] = {o->x, o->y, o->z, o->srcA, o->srcB, o->ex, o->ey, o->ez, o->hx, o->hy, o->hz, o->jx, o->jy, o->jz,
+                    o->wpe, o->nue, o->eyi, o->ezi, o->hyi, o->hzi, o->peyxI, o->pezxI, o->phyxI, o->phzxI,
+                    o->be, o->ae, o->bh, o->ah, o->p_hxy, o->p_hxz, o->p_hyx, o->p_hyz, o->p_hzx, o->p_hzy,
+                    o->p_exy, o->p_exz, o->p_eyx, o->p_eyz, o->p_ezx, o->p_ezy, o->eydft, o->ezdft, o->probes,
+                    o->eslz, o->esly, o->eslx, o->hslz, o->hsly, o->hslx};
+  for (size_t n = 0; n < sizeof(arrs) / sizeof(arrs[0]); n++) free(arrs[n]);
+  for (int q = 0; q < 8; q++) { free(o->fb[q]); free(o->tb[q]); free(o->lr[q]); }
+  free(o);
+}

@@ -1,0 +1,231 @@
+"""Pins the FP64 oracle (oracle/oracle.c) before it is trusted as the checker.
+
+1. Golden fixtures written by the reference solver compiled from its own
+   source (tests/golden/make_golden.py): NTF face arrays, probes, slices,
+   incident lines and the log's max|Ex| must agree bit for bit.
+2. Closed forms: Lee-Kalluri A and K against a matrix exponential (Van Loan
+   block form), CPML tables, source waveforms, the source DFT of a tone.
+3. Invariants: vacuum TF/SF leakage at round-off; DI = L&K with B -> 0 and
+   exponential collision decay.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+import cases
+import summary
+from meteorscattering_amd import runio
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "meteor-fork")
+
+
+def load_fixture(name):
+    f = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    p = json.loads(str(f["params"]))
+    return f, p
+
+
+def ntf_split(payload, p):
+    nx, ny, nz, M2 = p["nx"], p["ny"], p["nz"], 2 * p["nntf"]
+    out = {"Eydft": payload[:M2], "Ezdft": payload[M2:2 * M2]}
+    off = 2 * M2
+    for fn, (a, c) in zip(runio.ntf_face_names(), [(ny, nz)] * 8 + [(nx, nz)] * 8 + [(nx, ny)] * 8):
+        out[fn] = payload[off:off + a * c * M2].reshape(a, c, M2)
+        off += a * c * M2
+    assert off == payload.size
+    return out
+
+
+@pytest.mark.parametrize("name", cases.CASES)
+def test_oracle_matches_reference_fixture(oracle_lib, name):
+    f, p = load_fixture(name)
+    r = oracle_lib.run_like_reference(p, f["wp"], f["nu"])
+    nx, ny, nz = p["nx"], p["ny"], p["nz"]
+    # probes: bit-exact
+    probes = r["probes"].reshape(6, p["tsteps"], p["nprobes"])
+    assert np.array_equal(probes, f["probes"])
+    # NTF faces: bit-exact in the carved regions, projections exact elsewhere
+    faces = ntf_split(r["ntff"], p)
+    b = int(f["ntf_header"][1])
+    for fn in runio.ntf_face_names():
+        a = faces[fn]
+        assert np.array_equal(a[b:a.shape[0] - b + 1, b:a.shape[1] - b + 1], f["carve_" + fn]), fn
+        assert summary.compare_projection(fn, a, f["proj_" + fn], 0) == 0.0, fn
+    # frames
+    assert len(r["eframes"]) == int(f["nframes"][0])
+    for kind, frames in (("E", r["eframes"]), ("H", r["hframes"])):
+        for n, fr in enumerate(frames):
+            parts = runio.split_frame(fr, nx, ny, nz)
+            for key in ("slicez", "slicey", "slicex"):
+                assert summary.compare_projection(f"{kind}{n}_{key}", parts[key], f[f"proj_{kind}{n}_{key}"], 0) == 0.0
+            assert np.array_equal(np.concatenate([parts["inc1"], parts["inc2"]]), f[f"{kind}{n}_inc"])
+    # J frames are all zeros in the reference (slices never filled)
+    assert float(f["jframes_absmax"]) == 0.0
+    # log.txt max|Ex| lines (6 significant digits)
+    np.testing.assert_allclose(r["emax"], f["emax_log"], rtol=1e-5, atol=1e-12)
+
+
+def test_source_dft_matches_reference_where_initialised(oracle_lib):
+    """The reference never zeroes Ezdft/Eydft (meteor-fork.cpp:586-587); in
+    this fixture the stack held zeros, so the sums must agree exactly."""
+    f, p = load_fixture("lk_nopml_lhcp")
+    r = oracle_lib.run_like_reference(p, f["wp"], f["nu"])
+    faces = ntf_split(r["ntff"], p)
+    assert np.array_equal(faces["Eydft"], f["Eydft"])
+    assert np.array_equal(faces["Ezdft"], f["Ezdft"])
+
+
+@pytest.mark.skipif(not os.path.exists(REF_BIN), reason="reference not built here (oracle/_ref)")
+def test_oracle_matches_live_reference_random(oracle_lib):
+    """A case not in the fixtures, run live against the reference binary."""
+    p, wp, nu = cases.case("lk_ricker_ez")
+    rng = np.random.default_rng(7)
+    p.update(nx=42, ny=42, nz=42, tsteps=90, sourcepol=4, nntf=2, fntf=[48e6, 52e6], numfiles=3,
+             probex=[21, 20], probey=[21, 22], probez=[21, 19])
+    wp, nu = cases.trail_maps(p, n0=8e12, r0=1.2, nu0=3e5)
+    wp = wp * rng.uniform(0.9, 1.1, size=wp.shape)
+    nu = nu * rng.uniform(0.5, 2.0, size=nu.shape)
+    with tempfile.TemporaryDirectory() as d:
+        runio.write_run_dir(d, p, wp, nu)
+        subprocess.run(["bash", "-c", f"ulimit -s unlimited && cd '{d}' && OMP_NUM_THREADS=2 '{REF_BIN}'"],
+                       check=True, capture_output=True, timeout=300)
+        ntf = runio.read_ntf(os.path.join(d, "output_NTF.dat"), 42, 42, 42)
+        probe = runio.read_probe(os.path.join(d, "Probe.dat"), p["tsteps"])
+    r = oracle_lib.run_like_reference(p, wp, nu)
+    faces = ntf_split(r["ntff"], p)
+    for fn in runio.ntf_face_names():
+        assert np.array_equal(faces[fn], ntf[fn]), fn
+    assert np.array_equal(r["probes"].reshape(6, p["tsteps"], 2), probe["data"])
+
+
+def _lk_expm(nu, dt, w):
+    Wx = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+    Mx = -nu * np.eye(3) + Wx
+    big = np.zeros((6, 6))
+    big[:3, :3] = Mx
+    big[:3, 3:] = np.eye(3)
+    E = scipy.linalg.expm(big * dt)  # Van Loan: [[e^{M dt}, int_0^dt e^{M s} ds], [0, I]]
+    return E[:3, :3], E[:3, 3:]
+
+
+@pytest.mark.parametrize("nu,B", [(1e5, (4.33e-5, 0, 2.5e-5)), (1e7, (0, 3e-5, 4e-5)),
+                                  (3e3, (1e-5, -2e-5, 5e-5)), (0.0, (4.33e-5, 0, 2.5e-5)),
+                                  (2e6, (0, 0, 0)), (0.0, (0, 0, 0))])
+def test_lee_kalluri_matrices_closed_form(oracle_lib, nu, B):
+    dt = 9.5263e-10
+    w = -cases.QE * np.asarray(B) / cases.ME
+    A, K = oracle_lib.lk_matrices(nu, dt, w)
+    Ae, Ke = _lk_expm(nu, dt, w)
+    np.testing.assert_allclose(A, Ae, rtol=0, atol=1e-12)
+    # The reference's closed form (kept for bit parity) cancels in
+    # (1 - e^{-nu dt})/nu at small nu dt: ~5e-12 relative at nu dt = 3e-6.
+    np.testing.assert_allclose(K, Ke, rtol=0, atol=2e-11 * dt)
+    # invariants: A e^{nu dt} is a rotation about w
+    R = A * math.exp(nu * dt)
+    np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-12)
+    assert abs(np.linalg.det(R) - 1) < 1e-12
+    np.testing.assert_allclose(A @ w, math.exp(-nu * dt) * w, atol=1e-9 * (np.linalg.norm(w) + 1))
+
+
+def test_cpml_tables_closed_form(oracle_lib):
+    p, wp, nu = cases.case("lk_ricker_ez")
+    o = oracle_lib.Oracle(p, wp, nu)
+    be, ae, bh, ah = o.cpml_tables()
+    L = 10
+    dx = 0.5
+    smax = 1.5 * 5 / (150 * math.pi * dx)
+    n = np.arange(L)
+    se = smax * ((n + 0.5) / L) ** 4
+    sh = smax * ((n + 1.0) / L) ** 4
+    np.testing.assert_allclose(be[L:], np.exp(-se * p["dt"] / 8.85419e-12), rtol=1e-14)
+    np.testing.assert_allclose(bh[L:], np.exp(-sh * p["dt"] / 8.85419e-12), rtol=1e-14)
+    np.testing.assert_allclose(be[:L], be[L:][::-1], rtol=0)
+    np.testing.assert_allclose(ae, be - 1, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(ah, bh - 1, rtol=1e-15, atol=0)
+    assert np.all((be > 0) & (be <= 1))
+
+
+def test_source_waveforms_closed_form(oracle_lib):
+    p, wp, nu = cases.case("lk_ricker_ez")
+    for st in (1, 2, 3, 4):
+        p.update(sourcetype=st, tsteps=400)
+        o = oracle_lib.Oracle(p, wp, nu)
+        a, b = o.source_tables()
+        t = np.arange(400) * p["dt"]
+        f0 = p["f0"]
+        if st == 1:
+            np.testing.assert_allclose(a, np.sin(2 * np.pi * f0 * t), atol=1e-12)
+            np.testing.assert_allclose(b, np.cos(2 * np.pi * f0 * t), atol=1e-12)
+        elif st == 2:
+            s = p["fsig"]
+            tt = np.arange(400) - 3 * s
+            np.testing.assert_allclose(a, np.sin(2 * np.pi * f0 * tt * p["dt"]) * np.exp(-tt ** 2 / (2 * s * s)), atol=1e-12)
+        elif st == 3:
+            d = 1.6 * math.sqrt(6) / math.pi / f0
+            u = (math.pi * f0 * (t - d)) ** 2
+            np.testing.assert_allclose(a, (1 - 2 * u) * np.exp(-u), atol=1e-12)
+            assert abs(a.max() - 1) < 5e-3  # Ricker peak 1 at t = delay (sampled)
+        else:
+            tau = 1 / (math.pi * f0)
+            u = (t - 4 * tau) / tau
+            np.testing.assert_allclose(a, -u * np.exp(0.5 - 0.5 * u * u), atol=1e-12)
+            assert abs(np.abs(a).max() - 1) < 5e-3
+
+
+def test_source_dft_of_tone(oracle_lib):
+    """Ezdft for a sinusoidal Ez source equals the closed-form DFT sum."""
+    p, wp, nu = cases.case("lk_ricker_ez")
+    p.update(sourcetype=1, sourcepol=1, tsteps=60, nntf=3, fntf=[40e6, 50e6, 63e6], numfiles=1)
+    o = oracle_lib.Oracle(p, wp * 0, nu)
+    o.step(60)
+    pay = o.ntff()
+    M2 = 6
+    ez = pay[M2:2 * M2]
+    ey = pay[:M2]
+    t = np.arange(60) * p["dt"]
+    src = np.sin(2 * np.pi * p["f0"] * t)
+    for m, f in enumerate(p["fntf"]):
+        assert abs(ez[2 * m] - np.sum(src * np.sin(2 * np.pi * f * t))) < 1e-12
+        assert abs(ez[2 * m + 1] - np.sum(src * np.cos(2 * np.pi * f * t))) < 1e-12
+    assert np.all(ey == 0)
+
+
+@pytest.mark.parametrize("pol", [1, 2, 4])
+def test_vacuum_tfsf_leakage_roundoff(oracle_lib, pol):
+    """No plasma, no scatterer: the scattered-field region stays at round-off."""
+    p, wp, nu = cases.case("lk_ricker_ez")
+    p.update(sourcepol=pol, tsteps=150, doB0=0)
+    o = oracle_lib.Oracle(p, wp * 0, nu)
+    L, n = 10, p["nx"]
+    b1, b2 = L + 10, n - L - 10
+    worst_in = worst_out = 0.0
+    for t in range(150):
+        o.step(1)
+        if t % 10:
+            continue
+        for c in (0, 1, 2):
+            f = o.field(c)
+            worst_in = max(worst_in, np.abs(f[b1 + 1:b2 - 1, b1 + 1:b2 - 1, b1 + 1:b2 - 1]).max())
+            m = np.ones(f.shape, bool)
+            m[b1 - 2:b2 + 2, b1 - 2:b2 + 2, b1 - 2:b2 + 2] = False
+            worst_out = max(worst_out, np.abs(f[m]).max())
+    assert worst_in > 0.2
+    assert worst_out < 1e-14 * worst_in
+
+
+def test_lk_reduces_to_collisional_decay_without_field(oracle_lib):
+    """With E = 0 and B = 0, J decays as exp(-nu dt) per step (A = e^{-nu dt} I)."""
+    w = np.zeros(3)
+    A, K = oracle_lib.lk_matrices(2e6, 1e-9, w)
+    np.testing.assert_allclose(A, math.exp(-2e-3) * np.eye(3), rtol=1e-15)
+    np.testing.assert_allclose(K, (1 - math.exp(-2e-3)) / 2e6 * np.eye(3), rtol=1e-14)
