@@ -1,0 +1,139 @@
+"""Benchmark of the fused FDTD step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config2|config1] [--no-cpu]
+
+A step is one time step of the whole grid: the fused Yee H + plasma J + Yee
+E kernel (with CPML, TF/SF), the NTFF running DFT and the probe/cadence
+kernels.  The workload is the 400x400x800 FP32 single solve (128M cells,
+12-cell CPML, Lee-Kalluri magnetized plasma, trail tilted 30 deg, nu(z)
+with a 7 km scale height), synthetic media from a seed, all inputs resident
+in HBM before the timed region.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gcell-updates/s (% HBM peak); RCS solves/hr + p50/p99 vs serial GPU solves"
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, public spec
+ALGO_BYTES_FP32 = 80           # E,H,J read+write (72 B) + eps0 wp^2, nu (8 B) per cell-update
+ALGO_BYTES_FP64 = 160
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(p, wp, nu, steps: int) -> dict:
+    """The FP64 oracle (C restatement of the reference, OpenMP) on the host
+    cores, on the same grid, timed over `steps` steps (set-up excluded)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = pyoracle.Oracle(p, wp, nu)
+    o.step(1)  # first-touch of every array
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    ncell = p["nx"] * p["ny"] * p["nz"]
+    return {"value": round(ncell * steps / dt / 1e9, 5), "unit": "Gcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} steps of the same {p['nx']}x{p['ny']}x{p['nz']} grid, FP64, after 1 warm step; "
+                      f"{dt:.1f} s"}
+
+
+def traffic_from_profile(workload: str):
+    path = os.path.join(ROOT, "profiles", f"fdtd_step_{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="config2")
+    ap.add_argument("--precision", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: only one GPU is used in this build (asked for {args.gpus}); "
+              "the solve is single-GPU and multi-GPU sharding is design-only.", file=sys.stderr)
+        sys.exit(2)
+
+    from meteorscattering_amd import configs
+    from meteorscattering_amd.solver import Solver
+
+    t0 = time.perf_counter()
+    if args.workload == "config2":
+        p, wp, nu = configs.config2(seed=0)
+        wname = "400x400x800_fp32_lk_cpml12"
+    elif args.workload == "config1":
+        p, wp, nu = configs.config1()
+        wname = "96x96x192_fp32_lk_cpml10"
+    else:
+        raise SystemExit(f"unknown workload {args.workload}")
+    if args.precision == 64:
+        wname = wname.replace("fp32", "fp64")
+    total = args.warmup + args.steps
+    if total > p["tsteps"]:
+        p["tsteps"] = total
+        p["numfiles"] = min(p["numfiles"], total)
+    ncell = p["nx"] * p["ny"] * p["nz"]
+    log(f"workload {wname}: {ncell} cells, media built in {time.perf_counter() - t0:.1f} s")
+
+    s = Solver(p, wp, nu, precision=args.precision)
+    log(f"solver resident: {s.device_bytes / 2**30:.2f} GiB on device")
+    s.step(args.warmup)
+    s.sync()
+    s.set_timing(True)
+    t1 = time.perf_counter()
+    s.step(args.steps)
+    s.sync()
+    el = time.perf_counter() - t1
+    kms = s.kernel_ms
+    s.close()
+    ms_per_step = el * 1e3 / args.steps
+    value = ncell * args.steps / el / 1e9
+    algo = (ALGO_BYTES_FP32 if args.precision == 32 else ALGO_BYTES_FP64) * ncell
+    achieved = algo / (kms * 1e-3) / 1e9 if kms > 0 else None
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, fused kernel {kms:.3f} ms, "
+        f"{value:.2f} Gcell/s, kernel {achieved:.0f} GB/s algorithmic")
+    out = {
+        "metric": METRIC, "value": round(value, 4), "unit": "Gcell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.precision == 32 else "f64", "data": "synthetic",
+        "config": {"workload": f"fused Yee+Lee-Kalluri plasma step, {wname}", "nx": p["nx"], "ny": p["ny"],
+                   "nz": p["nz"], "pmllen": p["pmllen"], "ntff_freqs": p["nntf"],
+                   "media": "Gaussian trail tilted 30 deg, nu(z) 7 km scale height, seed 0"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+                     "traffic": traffic_from_profile(wname),
+                     "kernel": "fdtd_step", "kernel_ms": round(kms, 4),
+                     "algorithmic_bytes_per_launch": algo},
+    }
+    if not args.no_cpu:
+        t2 = time.perf_counter()
+        out["cpu_baseline"] = cpu_baseline(p, wp, nu, args.cpu_steps)
+        log(f"cpu baseline: {out['cpu_baseline']} ({time.perf_counter() - t2:.1f} s incl. set-up)")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -165,11 +165,15 @@ int msc_solver_get_ntff(msc_solver* s, double* out, size_t n);
  * [tsteps][nprobes] (Ex,Ey,Ez,Hx,Hy,Hz), meteor-fork.cpp:1719-1729. */
 int msc_solver_get_probes(msc_solver* s, double* out, size_t n);
 
-/* Current E and H frames in output_E.dat / output_H.dat layout
- * (meteor-fork.cpp:1640-1660): slices z, y, x (2 components each) then the
- * two incident lines.  Each frame holds 2(nx*ny + nx*nz + ny*nz) + 2nx doubles. */
+/* Output frames, captured on the device every tsteps/numfiles steps
+ * (meteor-fork.cpp:1616-1660).  msc_solver_frame_count gives the doubles in
+ * one E or H frame: slices z, y, x (2 components each) then the two
+ * incident lines, 2(nx*ny + nx*nz + ny*nz) + 2nx.  msc_solver_nframes gives
+ * the frames captured so far; msc_solver_get_frame copies frame f (E, H and
+ * the max|Ex| the reference logs at that step; any pointer may be NULL). */
 size_t msc_solver_frame_count(const msc_solver* s);
-int msc_solver_get_frames(msc_solver* s, double* eframe, double* hframe);
+int msc_solver_nframes(const msc_solver* s);
+int msc_solver_get_frame(msc_solver* s, int f, double* eframe, double* hframe, double* emax);
 
 /* max |Ex| over the grid (meteor-fork.cpp:1621-1630). */
 int msc_solver_max_abs_ex(msc_solver* s, double* out);

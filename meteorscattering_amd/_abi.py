@@ -135,7 +135,8 @@ def _declare(lib: C.CDLL) -> None:
         "msc_solver_get_ntff": (C.c_int, [vp, dp, C.c_size_t]),
         "msc_solver_get_probes": (C.c_int, [vp, dp, C.c_size_t]),
         "msc_solver_frame_count": (C.c_size_t, [vp]),
-        "msc_solver_get_frames": (C.c_int, [vp, dp, dp]),
+        "msc_solver_nframes": (C.c_int, [vp]),
+        "msc_solver_get_frame": (C.c_int, [vp, C.c_int, dp, dp, dp]),
         "msc_solver_max_abs_ex": (C.c_int, [vp, dp]),
         "msc_solver_device_bytes": (C.c_size_t, [vp]),
         "msc_solver_set_timing": (C.c_int, [vp, C.c_int]),

@@ -1,0 +1,80 @@
+// Device-side data structures shared by the HIP kernels and the host solver.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace msc {
+
+// Planes of z-tiles: each wavefront covers 64 consecutive k, lane 0 being
+// the k0-1 halo, so a tile owns 63 cells along z.
+constexpr int kLanes = 64;
+constexpr int kOwnZ = kLanes - 1;
+
+// Everything one fused time step needs.  Passed by value as the kernel
+// argument; it does not change from step to step (the step index lives in
+// device memory at `tdev`, and the ping-pong buffers are selected by its
+// parity), so the per-step launch sequence can be captured in a hipGraph.
+template <typename T>
+struct StepArgs {
+  int nx, ny, nz, L;
+  int xs, ys, zs;                       // nx-2L-1, ny-2L-1, nz-2L-1
+  int bx1, bx2, by1, by2, bz1, bz2;     // TF/SF box
+  int ssx, ssy, ssz;                    // slice planes
+  int dopml, doplate, doB0;
+  int plate_lo, plate_up;
+  int chunk, tiles_y, tiles_z, nchunks; // launch decomposition
+  int cadence;                          // tsteps/numfiles
+  long long sy, sx;                     // strides: sy = nz, sx = ny*nz
+  T c2h, c2e0;                          // dt/mu0, dt/eps0 (sigma = 0)
+  T inv_d;                              // 1/dx (dx == dy == dz)
+  T c1s, c2s;                           // sphere (sigma = 1e7) coefficients
+  double x0, y0, z0, dd, rsph;          // coordinates for the sphere test
+  double dt;
+  // Lee-Kalluri constants: R = Ce1 w w^T + cos I + Se1 [w x]
+  double wx, wy, wz, w0, cth, sth, Se1, Ce1;
+  double R[9];
+  // device pointers
+  T* e[2][3];        // ping-pong E: [parity][x,y,z]
+  T* h[2][3];        // ping-pong H
+  T* j[3];           // J, updated in place (deferred one step)
+  const T* wpe2;     // eps0 * wpe^2 per cell
+  const T* nue;      // collision frequency per cell
+  T* psih[2][6];     // ping-pong psi for H: hxy hxz hyx hyz hzx hzy
+  T* psie[6];        // psi for E (owner-only, in place): exy exz eyx eyz ezx ezy
+  const T* bh; const T* ah; const T* be; const T* ae;  // 2L each
+  const T* inc;      // [tsteps][4][nx]: c2h Ezinc/dx, c2h Eyinc/dx, c2e0 Hyinc/dx, c2e0 Hzinc/dx
+  T* eslice;         // staging of E slices: [2][nx][ny] [2][nx][nz] [2][ny][nz]
+  const int* tdev;   // step index
+};
+
+// Per-step accumulation of the near-to-far-field surface DFT.
+template <typename T>
+struct NtffArgs {
+  int nx, ny, nz, M, b;
+  long long sy, sx;
+  T* e[2][3];
+  T* h[2][3];
+  double* acc;              // 24 arrays, each [M][2][n1*n2]
+  long long off[24];        // offsets of the 24 arrays in acc
+  const double* sincos;     // [tsteps][M][2]
+  const int* tdev;
+};
+
+template <typename T>
+struct AuxArgs {
+  int nx, ny, nz, nprobes, tsteps, cadence, nframes;
+  long long sy, sx;
+  T* e[2][3];
+  T* h[2][3];
+  const int* px; const int* py; const int* pz;
+  double* probes;           // [6][tsteps][nprobes]
+  int ssx, ssy, ssz;
+  const T* eslice;          // staging written by the step kernel
+  double* eframes;          // [nframes][2(nx ny + nx nz + ny nz)]
+  double* hframes;
+  unsigned long long* emax; // [nframes] bit patterns of max|Ex| (as double)
+  int* tdev;
+};
+
+}  // namespace msc

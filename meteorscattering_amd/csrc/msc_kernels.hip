@@ -1,0 +1,674 @@
+// HIP kernels (gfx950) of the meteor-trail FDTD time step.
+//
+// Reference time loop: source/meteor-fork.cpp:683-1667.  One launch of
+// fdtd_step advances the whole grid by one step and fuses, per cell:
+//   - the CPML psi_H updates and H corrections  (:686-791, :824-968)
+//   - the H Yee update and TF/SF H corrections  (:795-934)
+//   - the plasma current J, deferred by one step (:1523-1597): J^n is formed
+//     from J^(n-1) and the node-averaged E^n at the start of step n instead
+//     of at the end of step n-1, which is the same arithmetic on the same
+//     inputs but lets E^(n+1) consume J^n in the same pass
+//   - the CPML psi_E updates and E corrections  (:1049-1156, :1190-1216,
+//     :1245-1275, :1373-1405)
+//   - the E Yee update, TF/SF E corrections, plate and sphere (:1160-1351)
+//   - the E slice capture before the Ez CPML correction (:1353-1371)
+// so the interior streams E, H, J and the two media scalars once per step.
+//
+// E and H are ping-pong buffered: each workgroup recomputes the H values of
+// its one-cell low halo (row j0-1, column k0-1) from read-only step-n data,
+// which removes any ordering between workgroups.
+#include <hip/hip_runtime.h>
+
+#include "msc_kernels.h"
+
+namespace msc {
+
+// Spread consecutive logical tiles over one XCD: hardware dispatch sends
+// workgroup b to XCD b % 8, so logical tile ids are laid out XCD-major and
+// neighbouring tiles share that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+  constexpr int X = 8;
+  const int per = n / X, rem = n % X;
+  const int xcd = b % X, idx = b / X;
+  return (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + idx;
+}
+
+// CPML slab index of an H-grid position p (meteor-fork.cpp:690-700):
+// [0,L) low slab, [n-L-1, n-1) high slab mapped to [L, 2L).
+__device__ __forceinline__ int slab_h(int p, int n, int L) {
+  if (p < L) return p;
+  if (p >= n - L - 1 && p < n - 1) return p - (n - 2 * L - 1);
+  return -1;
+}
+// E-grid slab index (meteor-fork.cpp:1053-1064): [1, L+1) -> p-1.
+__device__ __forceinline__ int slab_e(int p, int n, int L) {
+  if (p >= 1 && p < L + 1) return p - 1;
+  if (p >= n - L - 1 && p < n - 1) return p - (n - 2 * L - 1);
+  return -1;
+}
+
+// Lee & Kalluri (doB0=1, meteor-fork.cpp:1546-1595) or Young DI (doB0=0,
+// :1525-1536) current update in FP64 for one cell.  The per-cell
+// coefficients follow the reference's closed form; nu == 0 and |B| == 0 use
+// the finite limits of the same integrals (the reference divides by zero).
+template <typename T>
+__device__ __forceinline__ void current_update(const StepArgs<T>& a, double nu, double w,
+                                               double& jx, double& jy, double& jz,
+                                               double ex, double ey, double ez) {
+  if (!a.doB0) {
+    const double g = 2 + nu * a.dt;
+    const double f = (2 - nu * a.dt) / g;
+    const double s = 2 * ((w * a.dt) / g);
+    jx = f * jx + s * ex;
+    jy = f * jy + s * ey;
+    jz = f * jz + s * ez;
+    return;
+  }
+  double Ee1, kP, kI, kX;
+  if (nu > 0 && a.w0 > 0) {
+    Ee1 = exp(-nu * a.dt);
+    const double Ee2 = 1 / (a.w0 * a.w0 + nu * nu);
+    const double Ce2 = (1 - Ee1) / nu - Ee1 * nu * a.Ce1 - Ee1 * a.Se1;
+    const double Ce3 = nu * (1 - Ee1 * a.cth) + Ee1 * a.w0 * a.sth;
+    const double Ce4 = 1 - Ee1 * a.cth - Ee1 * nu * a.Se1;
+    kP = Ee2 * Ce2;
+    kI = Ee2 * Ce3;
+    kX = Ee2 * Ce4;
+  } else if (a.w0 > 0) {
+    Ee1 = 1;
+    kI = a.Se1;
+    kX = a.Ce1;
+    kP = (a.dt - a.Se1) / (a.w0 * a.w0);
+  } else {
+    Ee1 = exp(-nu * a.dt);
+    kI = nu > 0 ? -expm1(-nu * a.dt) / nu : a.dt;
+    kX = 0;
+    kP = 0;
+  }
+  // A J = Ee1 R J with R uniform; K E = kP w (w.E) + kI E + kX (w x E)
+  const double rx = a.R[0] * jx + a.R[1] * jy + a.R[2] * jz;
+  const double ry = a.R[3] * jx + a.R[4] * jy + a.R[5] * jz;
+  const double rz = a.R[6] * jx + a.R[7] * jy + a.R[8] * jz;
+  const double wE = a.wx * ex + a.wy * ey + a.wz * ez;
+  const double cx = a.wy * ez - a.wz * ey;
+  const double cy = a.wz * ex - a.wx * ez;
+  const double cz = a.wx * ey - a.wy * ex;
+  jx = Ee1 * rx + w * (kP * a.wx * wE + kI * ex + kX * cx);
+  jy = Ee1 * ry + w * (kP * a.wy * wE + kI * ey + kX * cy);
+  jz = Ee1 * rz + w * (kP * a.wz * wE + kI * ez + kX * cz);
+}
+
+template <typename T>
+__device__ __forceinline__ bool in_sphere(const StepArgs<T>& a, int i, int j, int k) {
+  // meteor-fork.cpp:203-209, coordinates as :137-141 (one loop per axis)
+  const double x = __dadd_rn(a.x0, __dmul_rn(a.dd, (double)i));
+  const double y = __dadd_rn(a.y0, __dmul_rn(a.dd, (double)j));
+  const double z = __dadd_rn(a.z0, __dmul_rn(a.dd, (double)k));
+  return sqrt(__dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z))) < a.rsph;
+}
+
+template <typename T, int TY>
+__global__ void __launch_bounds__((TY + 1) * kLanes)
+fdtd_step(StepArgs<T> a) {
+  constexpr int R = TY + 1;  // rows including the j0-1 halo row
+  __shared__ T sE[2][3][R + 1][kLanes + 1];
+  __shared__ T sH[3][R][kLanes];
+
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int r = threadIdx.x / kLanes;
+  int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int tz = b % a.tiles_z;
+  b /= a.tiles_z;
+  const int ty = b % a.tiles_y;
+  const int chunk = b / a.tiles_y;
+
+  const int nx = a.nx, ny = a.ny, nz = a.nz, L = a.L;
+  const int j = ty * TY - 1 + r;
+  const int k = tz * kOwnZ - 1 + lane;
+  const bool valid = j >= 0 && j < ny && k >= 0 && k < nz;
+  const bool own = valid && r >= 1 && lane >= 1;
+  const int ib = chunk * a.chunk;
+  const int ie = min(ib + a.chunk, nx);
+  const int istart = ib > 0 ? ib - 1 : 0;
+
+  const int t = *a.tdev;
+  const int par = t & 1;
+  const T* __restrict__ Exo = a.e[par][0];
+  const T* __restrict__ Eyo = a.e[par][1];
+  const T* __restrict__ Ezo = a.e[par][2];
+  const T* __restrict__ Hxo = a.h[par][0];
+  const T* __restrict__ Hyo = a.h[par][1];
+  const T* __restrict__ Hzo = a.h[par][2];
+  T* __restrict__ Exn = a.e[par ^ 1][0];
+  T* __restrict__ Eyn = a.e[par ^ 1][1];
+  T* __restrict__ Ezn = a.e[par ^ 1][2];
+  T* __restrict__ Hxn = a.h[par ^ 1][0];
+  T* __restrict__ Hyn = a.h[par ^ 1][1];
+  T* __restrict__ Hzn = a.h[par ^ 1][2];
+  T* const* psho = a.psih[par];
+  T* const* pshn = a.psih[par ^ 1];
+  const T* __restrict__ inc = a.inc + (size_t)t * 4 * nx;
+  const bool do_slice = a.eslice != nullptr && (t % a.cadence) == 0;
+  const T c2h = a.c2h, c2e0 = a.c2e0, inv = a.inv_d;
+  const long long sx = a.sx, sy = a.sy;
+
+  const bool xrow = (r == R - 1);
+  const bool xcol = (lane == kLanes - 1);
+  const bool rowok = xrow && (j + 1) < ny && k >= 0 && k < nz;
+  const bool colok = xcol && j >= 0 && j < ny && (k + 1) < nz;
+  const long long off = (long long)j * sy + k;  // in-plane offset, valid only if `valid`
+
+  // E^n at the current plane (registers) and its +1 row / +1 column halo.
+  T ecx = 0, ecy = 0, ecz = 0, rx = 0, rz = 0, cx = 0, cy = 0;
+  {
+    const long long c = (long long)istart * sx + off;
+    if (valid) { ecx = Exo[c]; ecy = Eyo[c]; ecz = Ezo[c]; }
+    if (rowok) { rx = Exo[c + sy]; rz = Ezo[c + sy]; }
+    if (colok) { cx = Exo[c + 1]; cy = Eyo[c + 1]; }
+  }
+  T exprev = 0, hyprev = 0, hzprev = 0;
+  int buf = 0;
+
+  for (int i = istart; i < ie; ++i, buf ^= 1) {
+    const long long c = (long long)i * sx + off;
+    // ---- phase A: stage plane i of E^n, prefetch plane i+1 and H^(n-1/2)
+    sE[buf][0][r][lane] = ecx;
+    sE[buf][1][r][lane] = ecy;
+    sE[buf][2][r][lane] = ecz;
+    if (xrow) { sE[buf][0][R][lane] = rx; sE[buf][2][R][lane] = rz; }
+    if (xcol) { sE[buf][0][r][kLanes] = cx; sE[buf][1][r][kLanes] = cy; }
+    T enx = 0, eny = 0, enz = 0, nrx = 0, nrz = 0, ncx = 0, ncy = 0;
+    const bool nxt = (i + 1) < nx;
+    if (nxt) {
+      const long long cn = c + sx;
+      if (valid) { enx = Exo[cn]; eny = Eyo[cn]; enz = Ezo[cn]; }
+      if (rowok) { nrx = Exo[cn + sy]; nrz = Ezo[cn + sy]; }
+      if (colok) { ncx = Exo[cn + 1]; ncy = Eyo[cn + 1]; }
+    }
+    T hx = 0, hy = 0, hz = 0;
+    if (valid) { hx = Hxo[c]; hy = Hyo[c]; hz = Hzo[c]; }
+    __syncthreads();
+
+    // ---- phase B: H^(n+1/2) at plane i (owned cells and the low halo)
+    const bool wr = own && i >= ib;
+    if (valid) {
+      const T ezjp = sE[buf][2][r + 1][lane];
+      const T eykp = sE[buf][1][r][lane + 1];
+      const T exkp = sE[buf][0][r][lane + 1];
+      const T exjp = sE[buf][0][r + 1][lane];
+      // Hx: i in [0,nx), j in [0,ny-1), k in [0,nz-1)   (:797-806)
+      if (j < ny - 1 && k < nz - 1) {
+        const T dEz = (ezjp - ecz) * inv;
+        const T dEy = (eykp - ecy) * inv;
+        hx = hx - c2h * (dEz - dEy);
+        if (i >= a.bx1 && i < a.bx2) {  // TF/SF (:808-822)
+          if (k >= a.bz1 && k < a.bz2) {
+            if (j == a.by1 - 1) hx = hx + inc[i];
+            if (j == a.by2 - 1) hx = hx - inc[i];
+          }
+          if (j >= a.by1 && j < a.by2) {
+            if (k == a.bz1 - 1) hx = hx - inc[nx + i];
+            if (k == a.bz2 - 1) hx = hx + inc[nx + i];
+          }
+        }
+        if (a.dopml) {  // (:690-714, :824-850)
+          const int jj = slab_h(j, ny, L);
+          if (jj >= 0) {
+            const size_t q = ((size_t)i * 2 * L + jj) * nz + k;
+            const T p = a.bh[jj] * psho[0][q] + a.ah[jj] * dEz;
+            if (wr) pshn[0][q] = p;
+            hx = hx - c2h * p;
+          }
+          const int kk = slab_h(k, nz, L);
+          if (kk >= 0) {
+            const size_t q = ((size_t)i * ny + j) * 2 * L + kk;
+            const T p = a.bh[kk] * psho[1][q] + a.ah[kk] * dEy;
+            if (wr) pshn[1][q] = p;
+            hx = hx + c2h * p;
+          }
+        }
+        if (wr) Hxn[c] = hx;
+      }
+      // Hy: i in [0,nx-1), j in [0,ny), k in [0,nz-1)   (:854-865)
+      if (i < nx - 1 && k < nz - 1) {
+        const T dEx = (exkp - ecx) * inv;
+        const T dEz = (enz - ecz) * inv;
+        hy = hy - c2h * (dEx - dEz);
+        if (j >= a.by1 && j < a.by2 && k >= a.bz1 && k < a.bz2) {  // (:868-875)
+          if (i == a.bx1 - 1) hy = hy - inc[a.bx1];
+          if (i == a.bx2 - 1) hy = hy + inc[a.bx2 - 1];
+        }
+        if (a.dopml) {  // (:718-744, :877-910)
+          const int ii = slab_h(i, nx, L);
+          if (ii >= 0) {
+            const size_t q = ((size_t)ii * ny + j) * nz + k;
+            const T p = a.bh[ii] * psho[2][q] + a.ah[ii] * dEz;
+            if (wr) pshn[2][q] = p;
+            hy = hy + c2h * p;
+          }
+          const int kk = slab_h(k, nz, L);
+          if (kk >= 0) {
+            const size_t q = ((size_t)i * ny + j) * 2 * L + kk;
+            const T p = a.bh[kk] * psho[3][q] + a.ah[kk] * dEx;
+            if (wr) pshn[3][q] = p;
+            hy = hy - c2h * p;
+          }
+        }
+        if (wr) Hyn[c] = hy;
+      }
+      // Hz: i in [0,nx-1), j in [0,ny-1), k in [0,nz)   (:913-924)
+      if (i < nx - 1 && j < ny - 1) {
+        const T dEy = (eny - ecy) * inv;
+        const T dEx = (exjp - ecx) * inv;
+        hz = hz - c2h * (dEy - dEx);
+        if (j >= a.by1 && j < a.by2 && k >= a.bz1 && k < a.bz2) {  // (:927-934)
+          if (i == a.bx1 - 1) hz = hz + inc[nx + a.bx1];
+          if (i == a.bx2 - 1) hz = hz - inc[nx + a.bx2 - 1];
+        }
+        if (a.dopml) {  // (:747-791, :936-968)
+          const int ii = slab_h(i, nx, L);
+          if (ii >= 0) {
+            const size_t q = ((size_t)ii * ny + j) * nz + k;
+            const T p = a.bh[ii] * psho[4][q] + a.ah[ii] * dEy;
+            if (wr) pshn[4][q] = p;
+            hz = hz - c2h * p;
+          }
+          const int jj = slab_h(j, ny, L);
+          if (jj >= 0) {
+            const size_t q = ((size_t)i * 2 * L + jj) * nz + k;
+            const T p = a.bh[jj] * psho[5][q] + a.ah[jj] * dEx;
+            if (wr) pshn[5][q] = p;
+            hz = hz + c2h * p;
+          }
+        }
+        if (wr) Hzn[c] = hz;
+      }
+    }
+    sH[0][r][lane] = hx;
+    sH[1][r][lane] = hy;
+    sH[2][r][lane] = hz;
+    __syncthreads();
+
+    // ---- phase C: J^n (deferred) and E^(n+1) on owned cells
+    if (wr) {
+      // J^n from J^(n-1) and E^n; domain [1,N-1)^3   (:1523-1597)
+      double jxv = 0, jyv = 0, jzv = 0;
+      if (i >= 1 && i < nx - 1 && j >= 1 && j < ny - 1 && k >= 1 && k < nz - 1) {
+        jxv = a.j[0][c];
+        jyv = a.j[1][c];
+        jzv = a.j[2][c];
+        const double w = a.wpe2[c];
+        const double nu = a.nue[c];
+        double mx, my, mz;
+        if (a.doB0) {
+          mx = ((double)ecx + (double)exprev) / 2;
+          my = ((double)ecy + (double)sE[buf][1][r - 1][lane]) / 2;
+          mz = ((double)ecz + (double)sE[buf][2][r][lane - 1]) / 2;
+        } else {
+          mx = ecx; my = ecy; mz = ecz;
+        }
+        current_update(a, nu, w, jxv, jyv, jzv, mx, my, mz);
+        a.j[0][c] = (T)jxv;
+        a.j[1][c] = (T)jyv;
+        a.j[2][c] = (T)jzv;
+      }
+      const T jx = (T)jxv, jy = (T)jyv, jz = (T)jzv;
+      T c1 = 1, c2 = c2e0;
+      if (a.doplate == 2 && in_sphere(a, i, j, k)) { c1 = a.c1s; c2 = a.c2s; }
+      const T hzjm = sH[2][r - 1][lane];
+      const T hykm = sH[1][r][lane - 1];
+      const T hxkm = sH[0][r][lane - 1];
+      const T hxjm = sH[0][r - 1][lane];
+      // Ex: i in [0,nx-1), j in [1,ny-1), k in [1,nz-1)   (:1161-1216)
+      if (i < nx - 1 && j >= 1 && j < ny - 1 && k >= 1 && k < nz - 1) {
+        const T dHz = (hz - hzjm) * inv;
+        const T dHy = (hy - hykm) * inv;
+        T ex = c1 * ecx + c2 * (dHz - dHy) - c2 * jx;
+        if (i >= a.bx1 && i < a.bx2 - 1) {
+          if (j >= a.by1 && j < a.by2) {
+            if (k == a.bz1) ex = ex + inc[2 * nx + i];
+            if (k == a.bz2) ex = ex - inc[2 * nx + i];
+          }
+          if (k >= a.bz1 && k < a.bz2) {
+            if (j == a.by1) ex = ex - inc[3 * nx + i];
+            if (j == a.by2) ex = ex + inc[3 * nx + i];
+          }
+        }
+        if (a.dopml) {
+          const int jj = slab_e(j, ny, L);
+          if (jj >= 0) {
+            const size_t q = ((size_t)i * 2 * L + jj) * nz + k;
+            const T p = a.be[jj] * a.psie[0][q] + a.ae[jj] * dHz;
+            a.psie[0][q] = p;
+            ex = ex + c2e0 * p;
+          }
+          const int kk = slab_e(k, nz, L);
+          if (kk >= 0) {
+            const size_t q = ((size_t)i * ny + j) * 2 * L + kk;
+            const T p = a.be[kk] * a.psie[1][q] + a.ae[kk] * dHy;
+            a.psie[1][q] = p;
+            ex = ex - c2e0 * p;
+          }
+        }
+        Exn[c] = ex;
+      }
+      // Ey: i in [1,nx-1), j in [0,ny), k in [1,nz)   (:1221-1275)
+      T ey = ecy;
+      if (i >= 1 && i < nx - 1 && k >= 1) {
+        const T dHx = (hx - hxkm) * inv;
+        const T dHz = (hz - hzprev) * inv;
+        ey = c1 * ecy + c2 * (dHx - dHz) - c2 * jy;
+        if (j >= a.by1 && j < a.by2 && k >= a.bz1 && k < a.bz2) {
+          if (i == a.bx1) ey = ey + inc[3 * nx + a.bx1 - 1];
+          if (i == a.bx2 - 1) ey = ey - inc[3 * nx + a.bx2 - 1];
+        }
+        if (a.dopml) {
+          const int ii = slab_e(i, nx, L);
+          if (ii >= 0 && j < ny - 1 && k < nz - 1) {
+            const size_t q = ((size_t)ii * ny + j) * nz + k;
+            const T p = a.be[ii] * a.psie[2][q] + a.ae[ii] * dHz;
+            a.psie[2][q] = p;
+            ey = ey - c2e0 * p;
+          }
+          const int kk = slab_e(k, nz, L);
+          if (kk >= 0 && j < ny - 1) {
+            const size_t q = ((size_t)i * ny + j) * 2 * L + kk;
+            const T p = a.be[kk] * a.psie[3][q] + a.ae[kk] * dHx;
+            a.psie[3][q] = p;
+            ey = ey + c2e0 * p;
+          }
+        }
+        if (a.doplate == 1 && i == a.ssx && j >= a.plate_lo && j < a.plate_up - 1 && k >= a.plate_lo && k < a.plate_up)
+          ey = 0;  // (:1337-1343)
+        Eyn[c] = ey;
+      }
+      // Ez: i in [1,nx-1), j in [1,ny), k in [0,nz)   (:1279-1302, :1373-1405)
+      T ez = ecz, ezpre = ecz;
+      if (i >= 1 && i < nx - 1 && j >= 1) {
+        const T dHy = (hy - hyprev) * inv;
+        const T dHx = (hx - hxjm) * inv;
+        ez = c1 * ecz + c2 * (dHy - dHx) - c2 * jz;
+        if (j >= a.by1 && j < a.by2 && k >= a.bz1 && k < a.bz2) {
+          if (i == a.bx1) ez = ez - inc[2 * nx + a.bx1 - 1];
+          if (i == a.bx2 - 1) ez = ez + inc[2 * nx + a.bx2 - 1];
+        }
+        if (a.doplate == 1 && i == a.ssx && j >= a.plate_lo && j < a.plate_up && k >= a.plate_lo && k < a.plate_up - 1)
+          ez = 0;  // (:1344-1350)
+        ezpre = ez;
+        if (a.dopml) {
+          const int ii = slab_e(i, nx, L);
+          if (ii >= 0 && j < ny - 1 && k < nz - 1) {
+            const size_t q = ((size_t)ii * ny + j) * nz + k;
+            const T p = a.be[ii] * a.psie[4][q] + a.ae[ii] * dHy;
+            a.psie[4][q] = p;
+            ez = ez + c2e0 * p;
+          }
+          const int jj = slab_e(j, ny, L);
+          if (jj >= 0 && k < nz - 1) {
+            const size_t q = ((size_t)i * 2 * L + jj) * nz + k;
+            const T p = a.be[jj] * a.psie[5][q] + a.ae[jj] * dHx;
+            a.psie[5][q] = p;
+            ez = ez - c2e0 * p;
+          }
+        }
+        Ezn[c] = ez;
+      }
+      if (do_slice) {  // E slices before the Ez CPML correction (:1353-1371)
+        T* s = a.eslice;
+        const size_t zxy = (size_t)nx * ny, zxz = (size_t)nx * nz;
+        if (k == a.ssz) { s[(size_t)i * ny + j] = ey; s[zxy + (size_t)i * ny + j] = ezpre; }
+        s += 2 * zxy;
+        if (j == a.ssy) { s[(size_t)i * nz + k] = ey; s[zxz + (size_t)i * nz + k] = ezpre; }
+        s += 2 * zxz;
+        if (i == a.ssx) { s[(size_t)j * nz + k] = ey; s[(size_t)ny * nz + (size_t)j * nz + k] = ezpre; }
+      }
+    }
+    // ---- rotate the x-direction registers
+    exprev = ecx;
+    hyprev = hy;
+    hzprev = hz;
+    ecx = enx; ecy = eny; ecz = enz;
+    rx = nrx; rz = nrz; cx = ncx; cy = ncy;
+  }
+}
+
+// J^(t+1) from J^t and E^(t+1) into `out` (what the reference holds after
+// the same number of steps); used only to read J back.
+template <typename T>
+__global__ void current_catchup(StepArgs<T> a, T* ox, T* oy, T* oz) {
+  const long long n = (long long)a.nx * a.ny * a.nz;
+  const int t = *a.tdev;
+  const T* Ex = a.e[t & 1][0];
+  const T* Ey = a.e[t & 1][1];
+  const T* Ez = a.e[t & 1][2];
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+    const int k = (int)(c % a.nz);
+    const int j = (int)((c / a.nz) % a.ny);
+    const int i = (int)(c / a.sx);
+    double jx = a.j[0][c], jy = a.j[1][c], jz = a.j[2][c];
+    if (i >= 1 && i < a.nx - 1 && j >= 1 && j < a.ny - 1 && k >= 1 && k < a.nz - 1) {
+      double mx, my, mz;
+      if (a.doB0) {
+        mx = ((double)Ex[c] + (double)Ex[c - a.sx]) / 2;
+        my = ((double)Ey[c] + (double)Ey[c - a.sy]) / 2;
+        mz = ((double)Ez[c] + (double)Ez[c - 1]) / 2;
+      } else {
+        mx = Ex[c]; my = Ey[c]; mz = Ez[c];
+      }
+      current_update(a, (double)a.nue[c], (double)a.wpe2[c], jx, jy, jz, mx, my, mz);
+    }
+    ox[c] = (T)jx;
+    oy[c] = (T)jy;
+    oz[c] = (T)jz;
+  }
+}
+
+// Running DFT of the six NTFF planes (meteor-fork.cpp:971-1023 for the
+// face-averaged H, :1417-1470 for E), FP64 accumulators, both at phase t*dt.
+template <typename T>
+__global__ void __launch_bounds__(256) ntff_accumulate(NtffArgs<T> a) {
+  const int face = blockIdx.y;
+  const int nx = a.nx, ny = a.ny, nz = a.nz, b = a.b;
+  int n2;
+  long long npts;
+  if (face < 2) { n2 = nz; npts = (long long)ny * nz; }
+  else if (face < 4) { n2 = nz; npts = (long long)nx * nz; }
+  else { n2 = ny; npts = (long long)nx * ny; }
+  const long long pt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= npts) return;
+  const int u = (int)(pt / n2), v = (int)(pt % n2);
+  const int t = *a.tdev;
+  const int par = (t & 1) ^ 1;  // buffers this step wrote
+  const T* Ex = a.e[par][0];
+  const T* Ey = a.e[par][1];
+  const T* Ez = a.e[par][2];
+  const T* Hx = a.h[par][0];
+  const T* Hy = a.h[par][1];
+  const T* Hz = a.h[par][2];
+  double q0, q1, q2, q3;
+  long long c;
+  switch (face) {
+    case 0:  // front, x = b: My=-Ez, Mz=Ey, Jy=+avg Hz, Jz=-avg Hy
+      c = (long long)b * a.sx + (long long)u * a.sy + v;
+      q0 = -(double)Ez[c]; q1 = Ey[c];
+      q2 = 0.5 * ((double)Hz[c] + (double)Hz[c - a.sx]);
+      q3 = -0.5 * ((double)Hy[c] + (double)Hy[c - a.sx]);
+      break;
+    case 1:  // back, x = nx-b
+      c = (long long)(nx - b) * a.sx + (long long)u * a.sy + v;
+      q0 = Ez[c]; q1 = -(double)Ey[c];
+      q2 = -0.5 * ((double)Hz[c] + (double)Hz[c - a.sx]);
+      q3 = 0.5 * ((double)Hy[c] + (double)Hy[c - a.sx]);
+      break;
+    case 2:  // top, y = ny-b: Mx=-Ez, Mz=Ex, Jx=+avg Hz, Jz=-avg Hx
+      c = (long long)u * a.sx + (long long)(ny - b) * a.sy + v;
+      q0 = -(double)Ez[c]; q1 = Ex[c];
+      q2 = 0.5 * ((double)Hz[c] + (double)Hz[c - a.sy]);
+      q3 = -0.5 * ((double)Hx[c] + (double)Hx[c - a.sy]);
+      break;
+    case 3:  // bottom, y = b
+      c = (long long)u * a.sx + (long long)b * a.sy + v;
+      q0 = Ez[c]; q1 = -(double)Ex[c];
+      q2 = -0.5 * ((double)Hz[c] + (double)Hz[c - a.sy]);
+      q3 = 0.5 * ((double)Hx[c] + (double)Hx[c - a.sy]);
+      break;
+    case 4:  // left, z = b: Mx=-Ey, My=Ex, Jx=+avg Hy, Jy=-avg Hx
+      c = (long long)u * a.sx + (long long)v * a.sy + b;
+      q0 = -(double)Ey[c]; q1 = Ex[c];
+      q2 = 0.5 * ((double)Hy[c] + (double)Hy[c - 1]);
+      q3 = -0.5 * ((double)Hx[c] + (double)Hx[c - 1]);
+      break;
+    default:  // right, z = nz-b
+      c = (long long)u * a.sx + (long long)v * a.sy + (nz - b);
+      q0 = Ey[c]; q1 = -(double)Ex[c];
+      q2 = -0.5 * ((double)Hy[c] + (double)Hy[c - 1]);
+      q3 = 0.5 * ((double)Hx[c] + (double)Hx[c - 1]);
+      break;
+  }
+  const double* sc = a.sincos + (size_t)t * a.M * 2;
+  const double q[4] = {q0, q1, q2, q3};
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    double* acc = a.acc + a.off[4 * face + n] + pt;
+    for (int m = 0; m < a.M; ++m) {
+      const double s = sc[2 * m], co = sc[2 * m + 1];
+      acc[(size_t)(2 * m) * npts] += q[n] * s;
+      acc[(size_t)(2 * m + 1) * npts] += q[n] * co;
+    }
+  }
+}
+
+// Output cadence (t % cadence == 0): H slices (meteor-fork.cpp:1025-1045),
+// staged E slices, and max|Ex| (:1621-1630) into the frame store.
+template <typename T>
+__global__ void __launch_bounds__(256) cadence_frames(AuxArgs<T> a) {
+  const int t = *a.tdev;
+  if (t % a.cadence) return;
+  const int f = t / a.cadence;
+  if (f >= a.nframes) return;
+  const int par = (t & 1) ^ 1;
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const long long gsz = (long long)gridDim.x * blockDim.x;
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // max|Ex|
+  {
+    const T* Ex = a.e[par][0];
+    const long long n = (long long)nx * ny * nz;
+    double m = 0;
+    for (long long c = tid; c < n; c += gsz) m = fmax(m, fabs((double)Ex[c]));
+    for (int s = 32; s > 0; s >>= 1) m = fmax(m, __shfl_xor(m, s));
+    if ((threadIdx.x & 63) == 0) atomicMax(a.emax + f, (unsigned long long)__double_as_longlong(m));
+  }
+  if (!a.hframes) return;
+  const long long fsz = 2LL * ((long long)nx * ny + (long long)nx * nz + (long long)ny * nz);
+  double* hf = a.hframes + (size_t)f * fsz;
+  double* ef = a.eframes + (size_t)f * fsz;
+  const T* Hy = a.h[par][1];
+  const T* Hz = a.h[par][2];
+  for (long long q = tid; q < fsz; q += gsz) {
+    ef[q] = (double)a.eslice[q];
+    long long rem = q;
+    double v = 0;
+    const long long zxy = (long long)nx * ny, zxz = (long long)nx * nz, zyz = (long long)ny * nz;
+    if (rem < 2 * zxy) {
+      const int comp = (int)(rem / zxy);
+      const long long e = rem % zxy;
+      const int i = (int)(e / ny), jj = (int)(e % ny);
+      if (jj < ny - 1) {
+        const long long c = (long long)i * a.sx + (long long)jj * a.sy + a.ssz;
+        v = comp ? (double)Hz[c] : (double)Hy[c];
+      }
+    } else if ((rem -= 2 * zxy) < 2 * zxz) {
+      const int comp = (int)(rem / zxz);
+      const long long e = rem % zxz;
+      const int i = (int)(e / nz), kk = (int)(e % nz);
+      if (kk < nz - 1) {
+        const long long c = (long long)i * a.sx + (long long)a.ssy * a.sy + kk;
+        v = comp ? (double)Hz[c] : (double)Hy[c];
+      }
+    } else {
+      rem -= 2 * zxz;
+      const int comp = (int)(rem / zyz);
+      const long long e = rem % zyz;
+      const int jj = (int)(e / nz), kk = (int)(e % nz);
+      if (jj < ny - 1 && kk < nz - 1) {
+        const long long c = (long long)a.ssx * a.sx + (long long)jj * a.sy + kk;
+        v = comp ? (double)Hz[c] : (double)Hy[c];
+      }
+    }
+    hf[q] = v;
+  }
+}
+
+// Probes (meteor-fork.cpp:1474-1486) and the step counter.
+template <typename T>
+__global__ void __launch_bounds__(64) probes_and_tick(AuxArgs<T> a) {
+  const int t = *a.tdev;
+  const int par = (t & 1) ^ 1;
+  const T* Ex = a.e[par][0];
+  const T* Ey = a.e[par][1];
+  const T* Ez = a.e[par][2];
+  const T* Hx = a.h[par][0];
+  const T* Hy = a.h[par][1];
+  const T* Hz = a.h[par][2];
+  const long long sx = a.sx, sy = a.sy;
+  for (int n = threadIdx.x; n < a.nprobes; n += blockDim.x) {
+    const long long c = (long long)a.px[n] * sx + (long long)a.py[n] * sy + a.pz[n];
+    const size_t e = (size_t)t * a.nprobes + n, T6 = (size_t)a.tsteps * a.nprobes;
+    a.probes[0 * T6 + e] = 0.5 * ((double)Ex[c] + (double)Ex[c - sx]);
+    a.probes[1 * T6 + e] = 0.5 * ((double)Ey[c] + (double)Ey[c - sy]);
+    a.probes[2 * T6 + e] = 0.5 * ((double)Ez[c] + (double)Ez[c - 1]);
+    a.probes[3 * T6 + e] = 0.25 * ((double)Hx[c] + (double)Hx[c - sy] + (double)Hx[c - 1] + (double)Hx[c - sy - 1]);
+    a.probes[4 * T6 + e] = 0.25 * ((double)Hy[c] + (double)Hy[c - sx] + (double)Hy[c - 1] + (double)Hy[c - sx - 1]);
+    a.probes[5 * T6 + e] = 0.25 * ((double)Hz[c] + (double)Hz[c - sy] + (double)Hz[c - sx] + (double)Hz[c - sx - sy]);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *a.tdev = t + 1;
+}
+
+// ---------------------------------------------------------------- launchers
+
+template <typename T>
+hipError_t launch_step(const StepArgs<T>& a, int ty, hipStream_t s) {
+  const int nblocks = a.tiles_y * a.tiles_z * a.nchunks;
+  switch (ty) {
+    case 3: hipLaunchKernelGGL((fdtd_step<T, 3>), dim3(nblocks), dim3(4 * kLanes), 0, s, a); break;
+    case 7: hipLaunchKernelGGL((fdtd_step<T, 7>), dim3(nblocks), dim3(8 * kLanes), 0, s, a); break;
+    case 15: hipLaunchKernelGGL((fdtd_step<T, 15>), dim3(nblocks), dim3(16 * kLanes), 0, s, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ntff(const NtffArgs<T>& a, hipStream_t s) {
+  long long mx = (long long)a.ny * a.nz;
+  mx = mx > (long long)a.nx * a.nz ? mx : (long long)a.nx * a.nz;
+  mx = mx > (long long)a.nx * a.ny ? mx : (long long)a.nx * a.ny;
+  hipLaunchKernelGGL((ntff_accumulate<T>), dim3((unsigned)((mx + 255) / 256), 6), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_aux(const AuxArgs<T>& a, bool frames, hipStream_t s) {
+  if (frames) hipLaunchKernelGGL((cadence_frames<T>), dim3(1024), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((probes_and_tick<T>), dim3(1), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_catchup(const StepArgs<T>& a, T* ox, T* oy, T* oz, hipStream_t s) {
+  hipLaunchKernelGGL((current_catchup<T>), dim3(2048), dim3(256), 0, s, a, ox, oy, oz);
+  return hipGetLastError();
+}
+
+template hipError_t launch_step<float>(const StepArgs<float>&, int, hipStream_t);
+template hipError_t launch_step<double>(const StepArgs<double>&, int, hipStream_t);
+template hipError_t launch_ntff<float>(const NtffArgs<float>&, hipStream_t);
+template hipError_t launch_ntff<double>(const NtffArgs<double>&, hipStream_t);
+template hipError_t launch_aux<float>(const AuxArgs<float>&, bool, hipStream_t);
+template hipError_t launch_aux<double>(const AuxArgs<double>&, bool, hipStream_t);
+template hipError_t launch_catchup<float>(const StepArgs<float>&, float*, float*, float*, hipStream_t);
+template hipError_t launch_catchup<double>(const StepArgs<double>&, double*, double*, double*, hipStream_t);
+
+}  // namespace msc

@@ -1,0 +1,664 @@
+// Solver handle: FP64 host precompute, device state, step sequencing and
+// read-back.  The set-up mirrors meteor-fork.cpp:119-646; the per-step work
+// is the kernel sequence of msc_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "msc_common.h"
+#include "msc_kernels.h"
+
+namespace msc {
+
+template <typename T>
+hipError_t launch_step(const StepArgs<T>& a, int ty, hipStream_t s);
+template <typename T>
+hipError_t launch_ntff(const NtffArgs<T>& a, hipStream_t s);
+template <typename T>
+hipError_t launch_aux(const AuxArgs<T>& a, bool frames, hipStream_t s);
+template <typename T>
+hipError_t launch_catchup(const StepArgs<T>& a, T* ox, T* oy, T* oz, hipStream_t s);
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) return fail(MSC_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// ------------------------------------------------------------ host precompute
+
+struct HostPlan {
+  msc_params p;
+  int nx, ny, nz, L, M, T, cadence, nframes;
+  double dx, dt, x0, y0, z0;
+  double c1h, c2h, c1e0, c2e0, c1s, c2s;
+  double wce[3], w0, cth, sth, Se1, Ce1, R[9];
+  std::vector<double> be, ae, bh, ah;  // 2L
+  std::vector<double> srcA, srcB;      // T
+  std::vector<double> inc;             // [T][4][nx]
+  std::vector<double> eline, hline;    // [nframes][2][nx] incident lines at frame steps
+  std::vector<double> eydft, ezdft;    // 2M
+  std::vector<double> sincos;          // [T][M][2]
+  std::vector<double> zcoord;          // nz
+};
+
+// Source waveforms, meteor-fork.cpp:153-179 (+ extension type 4).
+static void plan_sources(HostPlan& h) {
+  const msc_params& p = h.p;
+  const double f0 = p.f0, fsig = p.fsig, dt = h.dt;
+  const double rdelayA = 1.6 * std::sqrt(6) / kPi / f0;
+  const double rdelayB = (1.6 + 1 / std::sqrt(3)) * std::sqrt(6) / kPi / f0;
+  const double tau = p.src_tau > 0 ? p.src_tau : 1.0 / (kPi * f0);
+  h.srcA.resize(h.T);
+  h.srcB.resize(h.T);
+  for (int t = 0; t < h.T; t++) {
+    double a, b;
+    if (p.sourcetype == 2) {
+      a = std::sin(2 * kPi * f0 * (t - 3 * fsig) * dt) * std::exp(-(t - 3 * fsig) * (t - 3 * fsig) / (2 * fsig * fsig));
+      b = std::cos(2 * kPi * f0 * (t - 3 * fsig) * dt) * std::exp(-(t - 3 * fsig) * (t - 3 * fsig) / (2 * fsig * fsig));
+    } else if (p.sourcetype == 3) {
+      a = (1 - 2 * kPi * kPi * f0 * f0 * (t * dt - rdelayA) * (t * dt - rdelayA)) *
+          std::exp(-kPi * kPi * f0 * f0 * (t * dt - rdelayA) * (t * dt - rdelayA));
+      b = (1 - 2 * kPi * kPi * f0 * f0 * (t * dt - rdelayB) * (t * dt - rdelayB)) *
+          std::exp(-kPi * kPi * f0 * f0 * (t * dt - rdelayB) * (t * dt - rdelayB));
+    } else if (p.sourcetype == 4) {
+      const double u = (t * dt - 4 * tau) / tau, v = (t * dt - 5 * tau) / tau;
+      a = -u * std::exp(0.5 - 0.5 * u * u);
+      b = -v * std::exp(0.5 - 0.5 * v * v);
+    } else {
+      a = std::sin(2 * kPi * f0 * t * dt);
+      b = std::cos(2 * kPi * f0 * t * dt);
+    }
+    h.srcA[t] = a;
+    h.srcB[t] = b;
+  }
+}
+
+// CPML tables, meteor-fork.cpp:444-561 (kappa 1, alpha 0; one table per grid
+// since dx == dy == dz).
+static void plan_cpml(HostPlan& h) {
+  const int L = h.L;
+  h.be.assign(2 * L, 0); h.ae.assign(2 * L, 0); h.bh.assign(2 * L, 0); h.ah.assign(2 * L, 0);
+  if (!h.p.dopml) return;
+  const double pmlm = 4;
+  const double simax = 1.5 * (double)(pmlm + 1) / (150 * kPi * h.dx);
+  const double kamax = 1, almax = 0;
+  for (int n = 0; n < L; n++) {
+    const double mf = (double)n;
+    const double s = simax * std::pow(((mf + 0.5) / L), pmlm);
+    const double sm = simax * std::pow(((mf + 1) / L), pmlm);
+    const double k = 1 + (kamax - 1) * std::pow(((mf + 0.5) / L), pmlm);
+    const double km = 1 + (kamax - 1) * std::pow(((mf + 1) / L), pmlm);
+    const double a = almax * std::pow(((L - mf - 0.5) / L), pmlm);
+    const double am = almax * std::pow(((L - mf - 1) / L), pmlm);
+    const int m = n + L;
+    h.be[m] = std::exp(-((s / k) + a) * h.dt / kE0);
+    h.ae[m] = s / (s * k + std::pow(k, 2) * a) * (h.be[m] - 1);
+    h.bh[m] = std::exp(-((sm / km) + am) * h.dt / kE0);
+    h.ah[m] = sm / (sm * km + std::pow(km, 2) * am) * (h.bh[m] - 1);
+  }
+  for (int m = 0; m < L; m++) {
+    h.be[m] = h.be[2 * L - 1 - m]; h.ae[m] = h.ae[2 * L - 1 - m];
+    h.bh[m] = h.bh[2 * L - 1 - m]; h.ah[m] = h.ah[2 * L - 1 - m];
+  }
+}
+
+// The 1-D incident line (Eyinc, Ezinc, Hyinc, Hzinc with its +x CPML and the
+// hard source at i=0) does not depend on the 3-D grid, so its whole history
+// is computed here in FP64 (meteor-fork.cpp:736,772,864,894,923,952,1101,
+// 1137,1232,1261,1290,1308-1335,1389), giving per step the TF/SF correction
+// values, the source DFT (:1407-1414) and the lines written with each frame.
+static void plan_incident(HostPlan& h) {
+  const int nx = h.nx, L = h.L, T = h.T, M = h.M;
+  const int xs = nx - 2 * L - 1;
+  const double dx = h.dx, c1h = h.c1h, c2h = h.c2h, c1e0 = h.c1e0, c2e0 = h.c2e0;
+  std::vector<double> eyi(nx, 0), ezi(nx, 0), hyi(nx, 0), hzi(nx, 0);
+  std::vector<double> phy(2 * L, 0), phz(2 * L, 0), pey(2 * L, 0), pez(2 * L, 0);
+  const bool pml = h.p.dopml != 0;
+  h.inc.assign((size_t)T * 4 * nx, 0);
+  h.eline.assign((size_t)h.nframes * 2 * nx, 0);
+  h.hline.assign((size_t)h.nframes * 2 * nx, 0);
+  h.eydft.assign(2 * M, 0);
+  h.ezdft.assign(2 * M, 0);
+  for (int t = 0; t < T; t++) {
+    double* row = &h.inc[(size_t)t * 4 * nx];
+    for (int i = 0; i < nx; i++) {
+      row[i] = c2h * ezi[i] / dx;
+      row[nx + i] = c2h * eyi[i] / dx;
+    }
+    if (pml)
+      for (int i = nx - L - 1; i < nx - 1; i++) {
+        phy[i - xs] = h.bh[i - xs] * phy[i - xs] + h.ah[i - xs] * (ezi[i + 1] - ezi[i]) / dx;
+        phz[i - xs] = h.bh[i - xs] * phz[i - xs] + h.ah[i - xs] * (eyi[i + 1] - eyi[i]) / dx;
+      }
+    for (int i = 0; i < nx - 1; i++) hyi[i] = c1h * hyi[i] + c2h * (ezi[i + 1] - ezi[i]) / dx;
+    if (pml)
+      for (int i = nx - L - 1; i < nx - 1; i++) hyi[i] = hyi[i] + c2h * phy[i - xs];
+    for (int i = 0; i < nx - 1; i++) hzi[i] = c1h * hzi[i] - c2h * (eyi[i + 1] - eyi[i]) / dx;
+    if (pml)
+      for (int i = nx - L - 1; i < nx - 1; i++) hzi[i] = hzi[i] - c2h * phz[i - xs];
+    for (int i = 0; i < nx; i++) {
+      row[2 * nx + i] = c2e0 * hyi[i] / dx;
+      row[3 * nx + i] = c2e0 * hzi[i] / dx;
+    }
+    if (pml)
+      for (int i = nx - L - 1; i < nx - 1; i++) {
+        pey[i - xs] = h.be[i - xs] * pey[i - xs] + h.ae[i - xs] * (hzi[i] - hzi[i - 1]) / dx;
+        pez[i - xs] = h.be[i - xs] * pez[i - xs] + h.ae[i - xs] * (hyi[i] - hyi[i - 1]) / dx;
+      }
+    for (int i = 1; i < nx - 1; i++) eyi[i] = c1e0 * eyi[i] - c2e0 * (hzi[i] - hzi[i - 1]) / dx;
+    if (pml)
+      for (int i = nx - L - 1; i < nx - 1; i++) eyi[i] = eyi[i] - c2e0 * pey[i - xs];
+    for (int i = 1; i < nx - 1; i++) ezi[i] = c1e0 * ezi[i] + c2e0 * (hyi[i] - hyi[i - 1]) / dx;
+    const double a = h.srcA[t], b = h.srcB[t];
+    switch (h.p.sourcepol) {  // meteor-fork.cpp:1308-1335
+      case 2: ezi[0] = 0; eyi[0] = a; break;
+      case 3: ezi[0] = a; eyi[0] = a; break;
+      case 4: ezi[0] = a; eyi[0] = b; break;
+      case 5: ezi[0] = b; eyi[0] = a; break;
+      default: ezi[0] = a; eyi[0] = 0; break;
+    }
+    if (pml)
+      for (int i = nx - L - 1; i < nx - 1; i++) ezi[i] = ezi[i] + c2e0 * pez[i - xs];
+    for (int m = 0; m < M; m++) {
+      const double sn = h.sincos[((size_t)t * M + m) * 2], cs = h.sincos[((size_t)t * M + m) * 2 + 1];
+      h.ezdft[2 * m] += ezi[0] * sn;
+      h.ezdft[2 * m + 1] += ezi[0] * cs;
+      h.eydft[2 * m] += eyi[0] * sn;
+      h.eydft[2 * m + 1] += eyi[0] * cs;
+    }
+    if (t % h.cadence == 0) {
+      const int f = t / h.cadence;
+      std::copy(eyi.begin(), eyi.end(), h.eline.begin() + (size_t)f * 2 * nx);
+      std::copy(ezi.begin(), ezi.end(), h.eline.begin() + (size_t)f * 2 * nx + nx);
+      std::copy(hyi.begin(), hyi.end(), h.hline.begin() + (size_t)f * 2 * nx);
+      std::copy(hzi.begin(), hzi.end(), h.hline.begin() + (size_t)f * 2 * nx + nx);
+    }
+  }
+}
+
+static int make_plan(const msc_params* p, HostPlan& h) {
+  int rc = msc_params_validate(p);
+  if (rc) return rc;
+  h.p = *p;
+  h.nx = p->nx; h.ny = p->ny; h.nz = p->nz;
+  h.L = pml_len(p);
+  h.M = p->nntf;
+  h.T = p->tsteps;
+  h.cadence = p->tsteps / p->numfiles;
+  h.nframes = (h.T - 1) / h.cadence + 1;
+  h.dt = p->dt;
+  const double lambda = kC / p->f0;
+  h.dx = lambda / p->gridfactor;  // meteor-fork.cpp:127-132
+  h.x0 = -(h.nx - 1) / 2 * h.dx;  // :137-141 (integer division), one loop per axis
+  h.y0 = -(h.ny - 1) / 2 * h.dx;
+  h.z0 = -(h.nz - 1) / 2 * h.dx;
+  h.zcoord.resize(h.nz);
+  for (int k = 0; k < h.nz; k++) h.zcoord[k] = -(h.nz - 1) / 2 * h.dx + h.dx * k;
+  const double sigm = 0, sig = 0, dt = h.dt;  // :181-213
+  h.c1h = (2 * kU0 - sigm * dt) / (2 * kU0 + sigm * dt);
+  h.c2h = 2 * dt / (2 * kU0 + sigm * dt);
+  h.c1e0 = (2 * kE0 - sig * dt) / (2 * kE0 + sig * dt);
+  h.c2e0 = 2 * dt / (2 * kE0 + sig * dt);
+  h.c1s = (2 * kE0 - 1e7 * dt) / (2 * kE0 + 1e7 * dt);
+  h.c2s = 2 * dt / (2 * kE0 + 1e7 * dt);
+  for (int q = 0; q < 3; q++) h.wce[q] = -kQE * p->B0[q] / kME;  // :228-232
+  h.w0 = std::sqrt(h.wce[0] * h.wce[0] + h.wce[1] * h.wce[1] + h.wce[2] * h.wce[2]);
+  if (h.w0 > 0) {
+    h.cth = std::cos(h.w0 * dt);
+    h.sth = std::sin(h.w0 * dt);
+    h.Se1 = std::sin(h.w0 * dt) / h.w0;
+    h.Ce1 = (1 - std::cos(h.w0 * dt)) / (h.w0 * h.w0);
+  } else {
+    h.cth = 1; h.sth = 0; h.Se1 = dt; h.Ce1 = 0.5 * dt * dt;
+  }
+  const double* w = h.wce;
+  // R = Ce1 w w^T + cos I + Se1 [w x]   (A = Ee1 R, meteor-fork.cpp:1554-1562)
+  h.R[0] = h.Ce1 * w[0] * w[0] + h.cth; h.R[1] = h.Ce1 * w[0] * w[1] - h.Se1 * w[2]; h.R[2] = h.Ce1 * w[0] * w[2] + h.Se1 * w[1];
+  h.R[3] = h.Ce1 * w[1] * w[0] + h.Se1 * w[2]; h.R[4] = h.Ce1 * w[1] * w[1] + h.cth; h.R[5] = h.Ce1 * w[1] * w[2] - h.Se1 * w[0];
+  h.R[6] = h.Ce1 * w[2] * w[0] - h.Se1 * w[1]; h.R[7] = h.Ce1 * w[2] * w[1] + h.Se1 * w[0]; h.R[8] = h.Ce1 * w[2] * w[2] + h.cth;
+  h.sincos.resize((size_t)h.T * h.M * 2);
+  for (int t = 0; t < h.T; t++)
+    for (int m = 0; m < h.M; m++) {
+      h.sincos[((size_t)t * h.M + m) * 2] = std::sin(2 * kPi * p->fntf[m] * t * dt);  // :977, :1410
+      h.sincos[((size_t)t * h.M + m) * 2 + 1] = std::cos(2 * kPi * p->fntf[m] * t * dt);
+    }
+  plan_sources(h);
+  plan_cpml(h);
+  plan_incident(h);
+  return MSC_OK;
+}
+
+// ---------------------------------------------------------------- solver
+
+struct SolverBase {
+  HostPlan plan;
+  hipStream_t stream = nullptr;
+  int t_host = 0;
+  int ty = 7, chunk = 32;
+  bool timing = false;
+  std::vector<hipEvent_t> ev;  // pairs around step kernels of the last batch
+  double kernel_ms = 0;
+  size_t dev_bytes = 0;
+  std::vector<void*> allocs;
+  virtual ~SolverBase() {
+    if (stream) hipStreamSynchronize(stream);
+    for (void* q : allocs) hipFree(q);
+    for (hipEvent_t e : ev) hipEventDestroy(e);
+    if (stream) hipStreamDestroy(stream);
+  }
+  template <typename U>
+  int alloc(U** out, size_t n) {
+    void* q = nullptr;
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(U);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail(MSC_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    allocs.push_back(q);
+    dev_bytes += bytes;
+    e = hipMemsetAsync(q, 0, bytes, stream);
+    if (e != hipSuccess) return fail(MSC_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+    *out = (U*)q;
+    return MSC_OK;
+  }
+  virtual int init(const double* wp, const double* nu) = 0;
+  virtual int step(int n) = 0;
+  virtual int get_field(int comp, double* out) = 0;
+  virtual int get_ntff(double* out) = 0;
+  virtual int get_probes(double* out) = 0;
+  virtual int get_frame(int f, double* e, double* h, double* emax) = 0;
+  int sync() {
+    HIPCHK(hipStreamSynchronize(stream));
+    if (timing && !ev.empty()) {
+      double tot = 0;
+      for (size_t q = 0; q + 1 < ev.size(); q += 2) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+        tot += ms;
+      }
+      kernel_ms = tot / (double)(ev.size() / 2);
+    }
+    return MSC_OK;
+  }
+  size_t frame_doubles() const {
+    return 2 * ((size_t)plan.nx * plan.ny + (size_t)plan.nx * plan.nz + (size_t)plan.ny * plan.nz);
+  }
+};
+
+template <typename T>
+struct Solver : SolverBase {
+  StepArgs<T> sa{};
+  NtffArgs<T> na{};
+  AuxArgs<T> xa{};
+  T* eslice = nullptr;
+  double* acc = nullptr;
+  size_t acc_n = 0;
+  double* probes = nullptr;
+  double* eframes = nullptr;
+  double* hframes = nullptr;
+  unsigned long long* emax = nullptr;
+  int* tdev = nullptr;
+  int* pidx = nullptr;
+
+  int init(const double* wp, const double* nu) override {
+    const HostPlan& h = plan;
+    const int nx = h.nx, ny = h.ny, nz = h.nz, L = h.L;
+    const size_t N = (size_t)nx * ny * nz;
+    int rc;
+    if (const char* s = std::getenv("MSC_TY")) ty = std::atoi(s);
+    if (const char* s = std::getenv("MSC_CHUNK")) chunk = std::max(1, std::atoi(s));
+    if (ty != 3 && ty != 7 && ty != 15) return fail(MSC_EINVAL, "MSC_TY must be 3, 7 or 15");
+    HIPCHK(hipStreamCreate(&stream));
+    for (int q = 0; q < 2; q++)
+      for (int c = 0; c < 3; c++) {
+        if ((rc = alloc(&sa.e[q][c], N))) return rc;
+        if ((rc = alloc(&sa.h[q][c], N))) return rc;
+      }
+    for (int c = 0; c < 3; c++)
+      if ((rc = alloc(&sa.j[c], N))) return rc;
+    T *w2 = nullptr, *nud = nullptr;
+    if ((rc = alloc(&w2, N)) || (rc = alloc(&nud, N))) return rc;
+    sa.wpe2 = w2;
+    sa.nue = nud;
+    // psi slabs: H ping-pong, E in place
+    const size_t sxy = (size_t)nx * 2 * L * nz, sxz = (size_t)nx * ny * 2 * L, syx = (size_t)2 * L * ny * nz;
+    const size_t psz[6] = {sxy, sxz, syx, sxz, syx, sxy};  // hxy hxz hyx hyz hzx hzy (and exy exz eyx eyz ezx ezy)
+    for (int q = 0; q < 6; q++) {
+      if (h.p.dopml) {
+        if ((rc = alloc(&sa.psih[0][q], psz[q])) || (rc = alloc(&sa.psih[1][q], psz[q])) ||
+            (rc = alloc(&sa.psie[q], psz[q])))
+          return rc;
+      } else {
+        sa.psih[0][q] = sa.psih[1][q] = sa.psie[q] = nullptr;
+      }
+    }
+    // tables
+    auto upload = [&](const std::vector<double>& src, const T** dst) -> int {
+      std::vector<T> tmp(src.begin(), src.end());
+      T* d = nullptr;
+      int r = alloc(&d, tmp.size());
+      if (r) return r;
+      HIPCHK(hipMemcpyAsync(d, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      *dst = d;
+      return MSC_OK;
+    };
+    if ((rc = upload(h.bh, &sa.bh)) || (rc = upload(h.ah, &sa.ah)) || (rc = upload(h.be, &sa.be)) ||
+        (rc = upload(h.ae, &sa.ae)) || (rc = upload(h.inc, &sa.inc)))
+      return rc;
+    // media: eps0 wpe^2 and nu per cell, computed in FP64 as meteor-fork.cpp:336-366
+    {
+      std::vector<T> pw((size_t)ny * nz), pn((size_t)ny * nz);
+      std::vector<double> g(nz), gn(nz);
+      const double msig0 = h.p.metsize[0];
+      for (int k = 0; k < nz; k++) {
+        g[k] = std::exp(-std::pow((h.zcoord[k] - h.p.metloc[2]), 2) / (msig0 * msig0));
+        gn[k] = h.p.nu_scale_height > 0 ? std::exp(-(h.zcoord[k] - h.p.metloc[2]) / h.p.nu_scale_height) : 1.0;
+      }
+      for (int i = 0; i < nx; i++) {
+        for (int j = 0; j < ny; j++)
+          for (int k = 0; k < nz; k++) {
+            double wpe = 0, nue = 0;
+            if (!h.p.doplate) {
+              if (h.p.media3d) {
+                const size_t c = ((size_t)i * ny + j) * nz + k;
+                wpe = wp[c];
+                nue = nu[c];
+              } else {
+                wpe = wp[(size_t)i * ny + j] * g[k];
+                nue = nu[(size_t)i * ny + j] * g[k];
+              }
+              if (h.p.nu_scale_height > 0) nue *= gn[k];
+            }
+            pw[(size_t)j * nz + k] = (T)(kE0 * wpe * wpe);
+            pn[(size_t)j * nz + k] = (T)nue;
+          }
+        HIPCHK(hipMemcpyAsync(w2 + (size_t)i * ny * nz, pw.data(), pw.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(nud + (size_t)i * ny * nz, pn.data(), pn.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+      }
+    }
+    if ((rc = alloc(&tdev, 1))) return rc;
+    const size_t fsz = frame_doubles();
+    if (h.p.savefields) {
+      if ((rc = alloc(&eslice, fsz)) || (rc = alloc(&eframes, fsz * h.nframes)) || (rc = alloc(&hframes, fsz * h.nframes)))
+        return rc;
+    }
+    if ((rc = alloc(&emax, h.nframes))) return rc;
+    // NTFF accumulators: 24 arrays of [2M][n1*n2] in file order
+    const size_t M2 = 2 * (size_t)h.M;
+    const size_t fb = (size_t)ny * nz, tb = (size_t)nx * nz, lr = (size_t)nx * ny;
+    size_t off = 0;
+    for (int q = 0; q < 24; q++) {
+      na.off[q] = (long long)off;
+      off += M2 * (q < 8 ? fb : q < 16 ? tb : lr);
+    }
+    acc_n = off;
+    if ((rc = alloc(&acc, acc_n))) return rc;
+    double* scd = nullptr;
+    if ((rc = alloc(&scd, h.sincos.size()))) return rc;
+    HIPCHK(hipMemcpyAsync(scd, h.sincos.data(), h.sincos.size() * 8, hipMemcpyHostToDevice, stream));
+    if ((rc = alloc(&probes, (size_t)6 * h.T * std::max(1, h.p.nprobes)))) return rc;
+    if ((rc = alloc(&pidx, 3 * (size_t)MSC_MAX_PROBES))) return rc;
+    HIPCHK(hipMemcpyAsync(pidx, h.p.probex, 4 * MSC_MAX_PROBES, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(pidx + MSC_MAX_PROBES, h.p.probey, 4 * MSC_MAX_PROBES, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(pidx + 2 * MSC_MAX_PROBES, h.p.probez, 4 * MSC_MAX_PROBES, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+
+    // step arguments
+    sa.nx = nx; sa.ny = ny; sa.nz = nz; sa.L = L;
+    sa.xs = nx - 2 * L - 1; sa.ys = ny - 2 * L - 1; sa.zs = nz - 2 * L - 1;
+    sa.bx1 = L + 10; sa.bx2 = nx - L - 10; sa.by1 = L + 10; sa.by2 = ny - L - 10; sa.bz1 = L + 10; sa.bz2 = nz - L - 10;
+    sa.ssx = (nx + 1) / 2; sa.ssy = (ny + 1) / 2; sa.ssz = (nz + 1) / 2;
+    sa.dopml = h.p.dopml; sa.doplate = h.p.doplate; sa.doB0 = h.p.doB0;
+    sa.plate_lo = (int)(sa.ssy - h.p.platesize / 2 / h.dx);  // meteor-fork.cpp:1339-1340
+    sa.plate_up = (int)(sa.ssy + h.p.platesize / 2 / h.dx);
+    sa.chunk = chunk;
+    sa.tiles_y = (ny + ty - 1) / ty;
+    sa.tiles_z = (nz + kOwnZ - 1) / kOwnZ;
+    sa.nchunks = (nx + chunk - 1) / chunk;
+    sa.cadence = h.cadence;
+    sa.sy = nz; sa.sx = (long long)ny * nz;
+    sa.c2h = (T)h.c2h; sa.c2e0 = (T)h.c2e0; sa.inv_d = (T)(1.0 / h.dx);
+    sa.c1s = (T)h.c1s; sa.c2s = (T)h.c2s;
+    sa.x0 = h.x0; sa.y0 = h.y0; sa.z0 = h.z0; sa.dd = h.dx; sa.rsph = h.p.platesize;
+    sa.dt = h.dt;
+    sa.wx = h.wce[0]; sa.wy = h.wce[1]; sa.wz = h.wce[2]; sa.w0 = h.w0;
+    sa.cth = h.cth; sa.sth = h.sth; sa.Se1 = h.Se1; sa.Ce1 = h.Ce1;
+    for (int q = 0; q < 9; q++) sa.R[q] = h.R[q];
+    sa.eslice = eslice;
+    sa.tdev = tdev;
+
+    na.nx = nx; na.ny = ny; na.nz = nz; na.M = h.M; na.b = L + 5;
+    na.sy = sa.sy; na.sx = sa.sx;
+    for (int q = 0; q < 2; q++)
+      for (int c = 0; c < 3; c++) { na.e[q][c] = sa.e[q][c]; na.h[q][c] = sa.h[q][c]; }
+    na.acc = acc; na.sincos = scd; na.tdev = tdev;
+
+    xa.nx = nx; xa.ny = ny; xa.nz = nz; xa.nprobes = h.p.nprobes; xa.tsteps = h.T;
+    xa.cadence = h.cadence; xa.nframes = h.nframes;
+    xa.sy = sa.sy; xa.sx = sa.sx;
+    for (int q = 0; q < 2; q++)
+      for (int c = 0; c < 3; c++) { xa.e[q][c] = sa.e[q][c]; xa.h[q][c] = sa.h[q][c]; }
+    xa.px = pidx; xa.py = pidx + MSC_MAX_PROBES; xa.pz = pidx + 2 * MSC_MAX_PROBES;
+    xa.probes = probes;
+    xa.ssx = sa.ssx; xa.ssy = sa.ssy; xa.ssz = sa.ssz;
+    xa.eslice = eslice; xa.eframes = eframes; xa.hframes = hframes; xa.emax = emax;
+    xa.tdev = tdev;
+    return MSC_OK;
+  }
+
+  int step(int n) override {
+    if (n < 0) return fail(MSC_EINVAL, "nsteps < 0");
+    if (t_host + n > plan.T) return fail(MSC_ESTATE, "stepping past tsteps (%d + %d > %d)", t_host, n, plan.T);
+    if (timing) {
+      for (hipEvent_t e : ev) hipEventDestroy(e);
+      ev.assign(2 * (size_t)n, nullptr);
+      for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    }
+    for (int s = 0; s < n; s++) {
+      if (timing) HIPCHK(hipEventRecord(ev[2 * s], stream));
+      HIPCHK(launch_step(sa, ty, stream));
+      if (timing) HIPCHK(hipEventRecord(ev[2 * s + 1], stream));
+      HIPCHK(launch_ntff(na, stream));
+      const bool cad = (t_host % plan.cadence) == 0;
+      HIPCHK(launch_aux(xa, cad, stream));
+      ++t_host;
+    }
+    return MSC_OK;
+  }
+
+  int get_field(int comp, double* out) override {
+    if (comp < 0 || comp > 8) return fail(MSC_EINVAL, "component %d out of range", comp);
+    const size_t N = (size_t)plan.nx * plan.ny * plan.nz;
+    std::vector<T> tmp(N);
+    const int par = t_host & 1;
+    const T* src;
+    T* scratch[3] = {nullptr, nullptr, nullptr};
+    if (comp < 3) src = sa.e[par][comp];
+    else if (comp < 6) src = sa.h[par][comp - 3];
+    else {
+      // the device J lags one step (deferred update); catch it up into scratch
+      for (int c = 0; c < 3; c++) HIPCHK(hipMalloc(&scratch[c], N * sizeof(T)));
+      hipError_t e = launch_catchup(sa, scratch[0], scratch[1], scratch[2], stream);
+      if (e != hipSuccess) {
+        for (int c = 0; c < 3; c++) hipFree(scratch[c]);
+        return fail(MSC_EHIP, "catch-up kernel: %s", hipGetErrorString(e));
+      }
+      src = scratch[comp - 6];
+    }
+    hipError_t e = hipMemcpyAsync(tmp.data(), src, N * sizeof(T), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    for (int c = 0; c < 3; c++)
+      if (scratch[c]) hipFree(scratch[c]);
+    if (e != hipSuccess) return fail(MSC_EHIP, "field read-back: %s", hipGetErrorString(e));
+    for (size_t c = 0; c < N; c++) out[c] = (double)tmp[c];
+    return MSC_OK;
+  }
+
+  int get_ntff(double* out) override {
+    const size_t M2 = 2 * (size_t)plan.M;
+    std::vector<double> a(acc_n);
+    HIPCHK(hipMemcpyAsync(a.data(), acc, acc_n * 8, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    std::memcpy(out, plan.eydft.data(), M2 * 8);
+    std::memcpy(out + M2, plan.ezdft.data(), M2 * 8);
+    double* o = out + 2 * M2;
+    const size_t fb = (size_t)plan.ny * plan.nz, tb = (size_t)plan.nx * plan.nz, lr = (size_t)plan.nx * plan.ny;
+    for (int q = 0; q < 24; q++) {
+      const size_t np = q < 8 ? fb : q < 16 ? tb : lr;
+      const double* src = a.data() + na.off[q];
+      for (size_t pt = 0; pt < np; pt++)
+        for (size_t m = 0; m < M2; m++) o[pt * M2 + m] = src[m * np + pt];
+      o += np * M2;
+    }
+    return MSC_OK;
+  }
+
+  int get_probes(double* out) override {
+    const size_t n = (size_t)6 * plan.T * plan.p.nprobes;
+    if (!n) return MSC_OK;
+    HIPCHK(hipMemcpyAsync(out, probes, n * 8, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return MSC_OK;
+  }
+
+  int get_frame(int f, double* e, double* hh, double* em) override {
+    const int done = t_host == 0 ? 0 : (t_host - 1) / plan.cadence + 1;
+    if (f < 0 || f >= done) return fail(MSC_EINVAL, "frame %d not captured (have %d)", f, done);
+    const size_t fsz = frame_doubles(), nx = plan.nx;
+    if ((e || hh) && !plan.p.savefields) return fail(MSC_ESTATE, "savefields=0: no slice frames");
+    if (e) {
+      HIPCHK(hipMemcpyAsync(e, eframes + (size_t)f * fsz, fsz * 8, hipMemcpyDeviceToHost, stream));
+    }
+    if (hh) {
+      HIPCHK(hipMemcpyAsync(hh, hframes + (size_t)f * fsz, fsz * 8, hipMemcpyDeviceToHost, stream));
+    }
+    unsigned long long bits = 0;
+    if (em) HIPCHK(hipMemcpyAsync(&bits, emax + f, 8, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    if (e) std::memcpy(e + fsz, plan.eline.data() + (size_t)f * 2 * nx, 2 * nx * 8);
+    if (hh) std::memcpy(hh + fsz, plan.hline.data() + (size_t)f * 2 * nx, 2 * nx * 8);
+    if (em) std::memcpy(em, &bits, 8);
+    return MSC_OK;
+  }
+};
+
+}  // namespace msc
+
+using namespace msc;
+
+struct msc_solver {
+  std::unique_ptr<SolverBase> impl;
+};
+
+#define SOLVER_OK(s) \
+  if (!(s) || !(s)->impl) return fail(MSC_EINVAL, "NULL solver")
+
+extern "C" {
+
+int msc_solver_create(const msc_params* p, const double* wp, const double* nu, int precision, msc_solver** out) {
+  if (!p || !wp || !nu || !out) return fail(MSC_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (precision != 32 && precision != 64) return fail(MSC_EINVAL, "precision must be 32 or 64 (got %d)", precision);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MSC_EHIP, "no HIP device visible");
+  std::unique_ptr<SolverBase> s;
+  if (precision == 32) s.reset(new Solver<float>());
+  else s.reset(new Solver<double>());
+  int rc = make_plan(p, s->plan);
+  if (rc) return rc;
+  rc = s->init(wp, nu);
+  if (rc) return rc;
+  rc = s->sync();
+  if (rc) return rc;
+  msc_solver* h = new msc_solver;
+  h->impl = std::move(s);
+  *out = h;
+  return MSC_OK;
+}
+
+int msc_solver_step(msc_solver* s, int nsteps) {
+  SOLVER_OK(s);
+  return s->impl->step(nsteps);
+}
+
+int msc_solver_sync(msc_solver* s) {
+  SOLVER_OK(s);
+  return s->impl->sync();
+}
+
+int msc_solver_time(const msc_solver* s) { return (s && s->impl) ? s->impl->t_host : -1; }
+
+int msc_solver_get_field(msc_solver* s, int comp, double* out) {
+  SOLVER_OK(s);
+  if (!out) return fail(MSC_EINVAL, "NULL output");
+  return s->impl->get_field(comp, out);
+}
+
+size_t msc_solver_ntff_count(const msc_solver* s) {
+  if (!s || !s->impl) return 0;
+  const HostPlan& h = s->impl->plan;
+  const size_t M2 = 2 * (size_t)h.M;
+  return 2 * M2 + 8 * M2 * ((size_t)h.ny * h.nz + (size_t)h.nx * h.nz + (size_t)h.nx * h.ny);
+}
+
+int msc_solver_get_ntff(msc_solver* s, double* out, size_t n) {
+  SOLVER_OK(s);
+  if (!out || n < msc_solver_ntff_count(s)) return fail(MSC_EINVAL, "output buffer too small");
+  return s->impl->get_ntff(out);
+}
+
+int msc_solver_get_probes(msc_solver* s, double* out, size_t n) {
+  SOLVER_OK(s);
+  const HostPlan& h = s->impl->plan;
+  if (n < (size_t)6 * h.T * h.p.nprobes || (!out && h.p.nprobes)) return fail(MSC_EINVAL, "output buffer too small");
+  return s->impl->get_probes(out);
+}
+
+size_t msc_solver_frame_count(const msc_solver* s) {
+  if (!s || !s->impl) return 0;
+  return s->impl->frame_doubles() + 2 * (size_t)s->impl->plan.nx;
+}
+
+int msc_solver_nframes(const msc_solver* s) {
+  if (!s || !s->impl) return -1;
+  const int t = s->impl->t_host;
+  return t == 0 ? 0 : (t - 1) / s->impl->plan.cadence + 1;
+}
+
+int msc_solver_get_frame(msc_solver* s, int f, double* e, double* h, double* emax) {
+  SOLVER_OK(s);
+  return s->impl->get_frame(f, e, h, emax);
+}
+
+int msc_solver_max_abs_ex(msc_solver* s, double* out) {
+  SOLVER_OK(s);
+  const HostPlan& h = s->impl->plan;
+  std::vector<double> ex((size_t)h.nx * h.ny * h.nz);
+  int rc = s->impl->get_field(MSC_EX, ex.data());
+  if (rc) return rc;
+  double m = 0;
+  for (double v : ex) m = std::fabs(v) > m ? std::fabs(v) : m;
+  *out = m;
+  return MSC_OK;
+}
+
+size_t msc_solver_device_bytes(const msc_solver* s) { return (s && s->impl) ? s->impl->dev_bytes : 0; }
+
+int msc_solver_set_timing(msc_solver* s, int on) {
+  SOLVER_OK(s);
+  s->impl->timing = on != 0;
+  return MSC_OK;
+}
+
+double msc_solver_kernel_ms(const msc_solver* s) { return (s && s->impl) ? s->impl->kernel_ms : 0.0; }
+
+void msc_solver_destroy(msc_solver* s) { delete s; }
+
+}  // extern "C"

@@ -1,0 +1,208 @@
+"""HIP path (libmsc.so) against the FP64 oracle on the same inputs.
+
+Field error of a kind (E, H or J): max |gpu - oracle| over its three
+components divided by the running peak max |oracle| of that kind.
+Tolerances:
+  FP64 path: 1e-10 - same algorithm, differing only in rounding (reciprocal
+             multiplies, FMA contraction, device exp);
+  FP32 path: 2e-5 over the 80-160-step horizons below (FP32 fields, J and
+             CPML; FP64 plasma coefficients and NTFF sums); NTFF payload
+             2e-5 of its largest entry.
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import cases
+import summary
+from meteorscattering_amd import _abi, runio
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+@pytest.fixture(scope="module")
+def msc():
+    from meteorscattering_amd.solver import Solver, run_dir
+
+    return Solver, run_dir
+
+
+def relerr(a, b):
+    scale = max(np.abs(b).max(), 1e-300)
+    return float(np.abs(a - b).max() / scale)
+
+
+def worst(a, b):
+    d = np.abs(a - b)
+    idx = np.unravel_index(np.argmax(d), d.shape)
+    return f"at {tuple(int(x) for x in idx)}: gpu={a[idx]!r} oracle={b[idx]!r}"
+
+
+def run_pair(Solver, oracle_lib, p, wp, nu, precision, nsteps, check_every, tol, label):
+    """Step GPU and oracle together.  Error of a field kind (E, H or J) is
+    max |gpu - oracle| over its three components divided by the running peak
+    max |oracle| of that kind over the run so far."""
+    g = Solver(p, wp, nu, precision=precision)
+    o = oracle_lib.Oracle(p, wp, nu)
+    report = []
+    peak = [0.0, 0.0, 0.0]
+    errs = [0.0, 0.0, 0.0]
+    for t in range(nsteps):
+        g.step(1)
+        o.step(1)
+        if (t + 1) % check_every and t + 1 != nsteps:
+            continue
+        pairs = [(g.field(c), o.field(c)) for c in range(9)]
+        for kind in range(3):
+            peak[kind] = max(peak[kind], max(np.abs(b).max() for a, b in pairs[3 * kind:3 * kind + 3]))
+        for c, (a, b) in enumerate(pairs):
+            kind = c // 3
+            if peak[kind] == 0:
+                continue
+            e = float(np.abs(a - b).max() / peak[kind])
+            errs[kind] = max(errs[kind], e)
+            if e > tol:
+                report.append(f"{label} step {t} {_abi.COMPONENTS[c]} rel {e:.3e} {worst(a, b)}")
+        if report:
+            break
+    print(f"{label}: max-rel field error E {errs[0]:.2e} H {errs[1]:.2e} J {errs[2]:.2e}")
+    return g, o, report
+
+
+STABLE = ["lk_ricker_ez", "di_modgauss_ey", "plate_ricker_45", "sphere_sin_rhcp", "lk_nopml_lhcp"]
+
+
+@pytest.mark.parametrize("name", STABLE + ["lk_overdense_offset"])
+def test_fp64_fields_match_oracle_every_step(msc, oracle_lib, name):
+    Solver, _ = msc
+    p, wp, nu = cases.case(name)
+    g, o, rep = run_pair(Solver, oracle_lib, p, wp, nu, 64, p["tsteps"], 10, 1e-10, name)
+    assert not rep, "\n".join(rep[:12])
+    # NTFF payload and probes
+    a, b = g.ntff(), o.ntff()
+    assert relerr(a, b) < 1e-10
+    np.testing.assert_allclose(g.probes(), o.probes(), rtol=0, atol=1e-10 * np.abs(o.probes()).max())
+
+
+@pytest.mark.parametrize("name", STABLE)
+def test_fp32_fields_match_oracle(msc, oracle_lib, name):
+    Solver, _ = msc
+    p, wp, nu = cases.case(name)
+    g, o, rep = run_pair(Solver, oracle_lib, p, wp, nu, 32, p["tsteps"], 20, 2e-5, name)
+    assert not rep, "\n".join(rep[:12])
+    a, b = g.ntff(), o.ntff()
+    assert relerr(a, b) < 2e-5
+
+
+def test_fp64_frames_probes_and_log_maxima(msc, oracle_lib):
+    Solver, _ = msc
+    p, wp, nu = cases.case("lk_ricker_ez")
+    r = oracle_lib.run_like_reference(p, wp, nu)
+    g = Solver(p, wp, nu, precision=64)
+    g.step(p["tsteps"])
+    g.sync()
+    assert g.nframes == len(r["eframes"])
+    for f in range(g.nframes):
+        e, h, em = g.frame(f)
+        assert relerr(e, r["eframes"][f]) < 1e-10
+        assert relerr(h, r["hframes"][f]) < 1e-10
+        assert abs(em - r["emax"][f]) <= 1e-10 * max(r["emax"].max(), 1e-300)
+    np.testing.assert_allclose(g.probes(), r["probes"], rtol=0, atol=1e-10 * np.abs(r["probes"]).max())
+
+
+def test_noncubic_grid_matches_oracle(msc, oracle_lib):
+    """Non-cubic grids (the reference's z coordinates are undefined there)."""
+    Solver, _ = msc
+    p, _, _ = cases.case("lk_ricker_ez")
+    p.update(nx=46, ny=50, nz=57, tsteps=120, probex=[23, 20], probey=[25, 27], probez=[28, 26])
+    wp, nu = cases.trail_maps(p, n0=2e13, r0=1.5, nu0=1e6, centre=(0.5, -0.25))
+    for prec, tol in ((64, 1e-10), (32, 2e-5)):
+        g, o, rep = run_pair(Solver, oracle_lib, p, wp, nu, prec, p["tsteps"], 30, tol, f"noncubic{prec}")
+        assert not rep, "\n".join(rep[:12])
+
+
+def test_tilings_are_bit_identical(msc, oracle_lib, monkeypatch):
+    """Every cell is computed by the same code whatever the tile shape or
+    x-chunk length (halo cells are recomputed from read-only data)."""
+    Solver, _ = msc
+    p, wp, nu = cases.case("lk_ricker_ez")
+    p.update(tsteps=80, nx=47, ny=45, nz=70)
+    wp, nu = cases.trail_maps(p)
+    outs = []
+    for ty, chunk in ((7, 32), (3, 5), (15, 1), (7, 47)):
+        monkeypatch.setenv("MSC_TY", str(ty))
+        monkeypatch.setenv("MSC_CHUNK", str(chunk))
+        g = Solver(p, wp, nu, precision=32)
+        g.step(80)
+        outs.append((np.stack([g.field(c) for c in range(9)]), g.ntff()))
+        g.close()
+    for f, n in outs[1:]:
+        assert np.array_equal(f, outs[0][0])
+        assert np.array_equal(n, outs[0][1])
+
+
+def test_vacuum_tfsf_leakage_fp32(msc):
+    Solver, _ = msc
+    p, wp, nu = cases.case("lk_ricker_ez")
+    p.update(tsteps=150, doB0=0, sourcepol=4)
+    g = Solver(p, wp * 0, nu, precision=32)
+    L, n = 10, p["nx"]
+    b1, b2 = L + 10, n - L - 10
+    worst_in = worst_out = 0.0
+    for t in range(15):
+        g.step(10)
+        for c in (0, 1, 2):
+            f = g.field(c)
+            worst_in = max(worst_in, np.abs(f[b1 + 1:b2 - 1, b1 + 1:b2 - 1, b1 + 1:b2 - 1]).max())
+            m = np.ones(f.shape, bool)
+            m[b1 - 2:b2 + 2, b1 - 2:b2 + 2, b1 - 2:b2 + 2] = False
+            worst_out = max(worst_out, np.abs(f[m]).max())
+    assert worst_in > 0.2
+    assert worst_out < 1e-6 * worst_in
+
+
+@pytest.mark.parametrize("name", ["lk_ricker_ez", "sphere_sin_rhcp"])
+def test_dropin_run_dir_matches_reference_fixture(msc, name, tmp_path):
+    """The `meteor` executable in a run directory written from the fixture's
+    inputs produces files matching the reference's (FP64 path)."""
+    f = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    p = json.loads(str(f["params"]))
+    runio.write_run_dir(str(tmp_path), p, f["wp"], f["nu"])
+    exe = os.path.join(os.path.dirname(_abi.LIB_PATH), "meteor")
+    env = dict(os.environ, MSC_PRECISION="64")
+    r = subprocess.run([exe], cwd=str(tmp_path), env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    nx, ny, nz = p["nx"], p["ny"], p["nz"]
+    ntf = runio.read_ntf(str(tmp_path / "output_NTF.dat"), nx, ny, nz)
+    b = int(f["ntf_header"][1])
+    assert ntf["ntfb"] == b and ntf["nntf"] == p["nntf"]
+    for fn in runio.ntf_face_names():
+        a = ntf[fn]
+        assert summary.compare_projection(fn, a, f["proj_" + fn], 0) < 1e-10, fn
+        np.testing.assert_allclose(a[b:a.shape[0] - b + 1, b:a.shape[1] - b + 1], f["carve_" + fn],
+                                   rtol=0, atol=1e-10 * max(np.abs(f["carve_" + fn]).max(), 1e-300))
+    probe = runio.read_probe(str(tmp_path / "Probe.dat"), p["tsteps"])
+    np.testing.assert_allclose(probe["data"], f["probes"], rtol=0, atol=1e-10 * np.abs(f["probes"]).max())
+    ef = runio.read_frames(str(tmp_path / "output_E.dat"), nx, ny, nz)
+    hf = runio.read_frames(str(tmp_path / "output_H.dat"), nx, ny, nz)
+    jf = runio.read_frames(str(tmp_path / "output_J.dat"), nx, ny, nz, with_inc=False)
+    assert ef.shape[0] == int(f["nframes"][0]) and jf.shape[0] == int(f["nframes"][2])
+    assert np.abs(jf).max() == 0.0
+    for kind, frames in (("E", ef), ("H", hf)):
+        for n, fr in enumerate(frames):
+            parts = runio.split_frame(fr, nx, ny, nz)
+            for key in ("slicez", "slicey", "slicex"):
+                assert summary.compare_projection(f"{kind}{n}_{key}", parts[key], f[f"proj_{kind}{n}_{key}"], 0) < 1e-10
+            np.testing.assert_array_equal(np.concatenate([parts["inc1"], parts["inc2"]]), f[f"{kind}{n}_inc"])
+    log = open(tmp_path / "log.txt").read()
+    assert "All done!" in log
+    emax = [float(x) for x in __import__("re").findall(r"Maximum abs\(Ex\) is ([^\s]+)", log)]
+    np.testing.assert_allclose(emax, f["emax_log"], rtol=2e-5, atol=1e-12)
