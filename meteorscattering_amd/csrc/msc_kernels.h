@@ -10,6 +10,15 @@ namespace msc {
 // the k0-1 halo, so a tile owns 63 cells along z.
 constexpr int kLanes = 64;
 constexpr int kOwnZ = kLanes - 1;
+constexpr int kMaxLkDeg = 24;
+
+// current models selected per launch
+enum { kCurDI = 0, kCurLKPoly = 1, kCurLKClosed = 2 };
+
+// packed tile descriptor: z-tile (8 bits) | y-tile (12 bits) | x-chunk (12 bits)
+__host__ __device__ inline uint32_t pack_tile(int tz, int ty, int ch) {
+  return (uint32_t)tz | ((uint32_t)ty << 8) | ((uint32_t)ch << 20);
+}
 
 // Everything one fused time step needs.  Passed by value as the kernel
 // argument; it does not change from step to step (the step index lives in
@@ -34,6 +43,10 @@ struct StepArgs {
   // Lee-Kalluri constants: R = Ce1 w w^T + cos I + Se1 [w x]
   double wx, wy, wz, w0, cth, sth, Se1, Ce1;
   double R[9];
+  // Lee-Kalluri per-cell coefficients as polynomials in x = nu dt:
+  // lkc[0] -> e^{-x}, lkc[1] -> kI, lkc[2] -> kX, lkc[3] -> kP (degree lkdeg)
+  int lkdeg;
+  double lkc[4][kMaxLkDeg + 1];
   // device pointers
   T* e[2][3];        // ping-pong E: [parity][x,y,z]
   T* h[2][3];        // ping-pong H

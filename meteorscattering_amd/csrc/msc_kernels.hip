@@ -47,55 +47,88 @@ __device__ __forceinline__ int slab_e(int p, int n, int L) {
   return -1;
 }
 
-// Lee & Kalluri (doB0=1, meteor-fork.cpp:1546-1595) or Young DI (doB0=0,
-// :1525-1536) current update in FP64 for one cell.  The per-cell
-// coefficients follow the reference's closed form; nu == 0 and |B| == 0 use
-// the finite limits of the same integrals (the reference divides by zero).
-template <typename T>
-__device__ __forceinline__ void current_update(const StepArgs<T>& a, double nu, double w,
-                                               double& jx, double& jy, double& jz,
-                                               double ex, double ey, double ez) {
-  if (!a.doB0) {
+// Plasma current update for one cell (meteor-fork.cpp:1523-1597):
+//   CUR == kCurDI:       Young's direct integration (doB0 = 0, :1525-1536)
+//   CUR == kCurLKPoly:   Lee & Kalluri (doB0 = 1, :1546-1595) with the
+//                        per-cell coefficients e^{-x}, kI, kX, kP evaluated in
+//                        FP64 as Horner polynomials in x = nu dt whose FP64
+//                        coefficients the host derives from the closed-form
+//                        integrals (exact to FP64 over the solve's nu range)
+//   CUR == kCurLKClosed: Lee & Kalluri with the reference's closed form in
+//                        FP64 (finite limits at nu == 0 and |B| == 0)
+// J^(n+1) = e^{-x} R J^n + w (kI E + kX (w_b x E) + kP w_b (w_b . E)), w = eps0 wp^2,
+// with E the node-averaged field for Lee & Kalluri and the collocated one for DI.
+template <typename T, int CUR>
+__device__ __forceinline__ void current_update(const StepArgs<T>& a, T nu_t, T w_t, T& jx, T& jy, T& jz,
+                                               T ex, T ey, T ez) {
+  const double nu = nu_t, w = w_t;
+  if constexpr (CUR == kCurDI) {
     const double g = 2 + nu * a.dt;
     const double f = (2 - nu * a.dt) / g;
     const double s = 2 * ((w * a.dt) / g);
-    jx = f * jx + s * ex;
-    jy = f * jy + s * ey;
-    jz = f * jz + s * ez;
+    jx = (T)(f * (double)jx + s * (double)ex);
+    jy = (T)(f * (double)jy + s * (double)ey);
+    jz = (T)(f * (double)jz + s * (double)ez);
     return;
-  }
-  double Ee1, kP, kI, kX;
-  if (nu > 0 && a.w0 > 0) {
-    Ee1 = exp(-nu * a.dt);
-    const double Ee2 = 1 / (a.w0 * a.w0 + nu * nu);
-    const double Ce2 = (1 - Ee1) / nu - Ee1 * nu * a.Ce1 - Ee1 * a.Se1;
-    const double Ce3 = nu * (1 - Ee1 * a.cth) + Ee1 * a.w0 * a.sth;
-    const double Ce4 = 1 - Ee1 * a.cth - Ee1 * nu * a.Se1;
-    kP = Ee2 * Ce2;
-    kI = Ee2 * Ce3;
-    kX = Ee2 * Ce4;
-  } else if (a.w0 > 0) {
-    Ee1 = 1;
-    kI = a.Se1;
-    kX = a.Ce1;
-    kP = (a.dt - a.Se1) / (a.w0 * a.w0);
   } else {
-    Ee1 = exp(-nu * a.dt);
-    kI = nu > 0 ? -expm1(-nu * a.dt) / nu : a.dt;
-    kX = 0;
-    kP = 0;
+    double Ee1, kP, kI, kX;
+    if constexpr (CUR == kCurLKPoly) {
+      const double x = nu * a.dt;
+      const int d = a.lkdeg;
+      double pa = a.lkc[0][d], pi = a.lkc[1][d], px = a.lkc[2][d], pp = a.lkc[3][d];
+      for (int n = d - 1; n >= 0; --n) {
+        pa = fma(pa, x, a.lkc[0][n]);
+        pi = fma(pi, x, a.lkc[1][n]);
+        px = fma(px, x, a.lkc[2][n]);
+        pp = fma(pp, x, a.lkc[3][n]);
+      }
+      Ee1 = pa; kI = pi; kX = px; kP = pp;
+    } else {
+      if (nu > 0 && a.w0 > 0) {
+        Ee1 = exp(-nu * a.dt);
+        const double Ee2 = 1 / (a.w0 * a.w0 + nu * nu);
+        const double Ce2 = (1 - Ee1) / nu - Ee1 * nu * a.Ce1 - Ee1 * a.Se1;
+        const double Ce3 = nu * (1 - Ee1 * a.cth) + Ee1 * a.w0 * a.sth;
+        const double Ce4 = 1 - Ee1 * a.cth - Ee1 * nu * a.Se1;
+        kP = Ee2 * Ce2;
+        kI = Ee2 * Ce3;
+        kX = Ee2 * Ce4;
+      } else if (a.w0 > 0) {
+        Ee1 = 1;
+        kI = a.Se1;
+        kX = a.Ce1;
+        kP = (a.dt - a.Se1) / (a.w0 * a.w0);
+      } else {
+        Ee1 = exp(-nu * a.dt);
+        kI = nu > 0 ? -expm1(-nu * a.dt) / nu : a.dt;
+        kX = 0;
+        kP = 0;
+      }
+    }
+    // per-cell coefficient products in FP64, then the update in T
+    const T ca = (T)Ee1, ci = (T)(w * kI), cx = (T)(w * kX), cp = (T)(w * kP);
+    const T wx = (T)a.wx, wy = (T)a.wy, wz = (T)a.wz;
+    const T rx = (T)a.R[0] * jx + (T)a.R[1] * jy + (T)a.R[2] * jz;
+    const T ry = (T)a.R[3] * jx + (T)a.R[4] * jy + (T)a.R[5] * jz;
+    const T rz = (T)a.R[6] * jx + (T)a.R[7] * jy + (T)a.R[8] * jz;
+    const T wE = cp * (wx * ex + wy * ey + wz * ez);
+    jx = ca * rx + ci * ex + cx * (wy * ez - wz * ey) + wE * wx;
+    jy = ca * ry + ci * ey + cx * (wz * ex - wx * ez) + wE * wy;
+    jz = ca * rz + ci * ez + cx * (wx * ey - wy * ex) + wE * wz;
   }
-  // A J = Ee1 R J with R uniform; K E = kP w (w.E) + kI E + kX (w x E)
-  const double rx = a.R[0] * jx + a.R[1] * jy + a.R[2] * jz;
-  const double ry = a.R[3] * jx + a.R[4] * jy + a.R[5] * jz;
-  const double rz = a.R[6] * jx + a.R[7] * jy + a.R[8] * jz;
-  const double wE = a.wx * ex + a.wy * ey + a.wz * ez;
-  const double cx = a.wy * ez - a.wz * ey;
-  const double cy = a.wz * ex - a.wx * ez;
-  const double cz = a.wx * ey - a.wy * ex;
-  jx = Ee1 * rx + w * (kP * a.wx * wE + kI * ex + kX * cx);
-  jy = Ee1 * ry + w * (kP * a.wy * wE + kI * ey + kX * cy);
-  jz = Ee1 * rz + w * (kP * a.wz * wE + kI * ez + kX * cz);
+}
+
+// Global access through a uniform base pointer and a 32-bit byte offset, so
+// the compiler can use the scalar-base + 32-bit vector-offset addressing
+// form and keep one VGPR per address instead of two.  The host only takes
+// this path when every array is below 4 GiB.
+template <typename T>
+__device__ __forceinline__ T ldo(const T* __restrict__ base, uint32_t boff) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+}
+template <typename T>
+__device__ __forceinline__ void sto(T* __restrict__ base, uint32_t boff, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
 }
 
 template <typename T>
@@ -107,20 +140,20 @@ __device__ __forceinline__ bool in_sphere(const StepArgs<T>& a, int i, int j, in
   return sqrt(__dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z))) < a.rsph;
 }
 
-template <typename T, int TY>
+template <typename T, int TY, int CUR, bool PML, bool SLICE>
 __global__ void __launch_bounds__((TY + 1) * kLanes)
-fdtd_step(StepArgs<T> a) {
+fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
   constexpr int R = TY + 1;  // rows including the j0-1 halo row
+  constexpr uint32_t W = sizeof(T);
   __shared__ T sE[2][3][R + 1][kLanes + 1];
   __shared__ T sH[3][R][kLanes];
 
   const int lane = threadIdx.x & (kLanes - 1);
   const int r = threadIdx.x / kLanes;
-  int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int tz = b % a.tiles_z;
-  b /= a.tiles_z;
-  const int ty = b % a.tiles_y;
-  const int chunk = b / a.tiles_y;
+  const uint32_t tile = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+  const int tz = tile & 0xff;
+  const int ty = (tile >> 8) & 0xfff;
+  const int chunk = tile >> 20;
 
   const int nx = a.nx, ny = a.ny, nz = a.nz, L = a.L;
   const int j = ty * TY - 1 + r;
@@ -145,32 +178,31 @@ fdtd_step(StepArgs<T> a) {
   T* __restrict__ Hxn = a.h[par ^ 1][0];
   T* __restrict__ Hyn = a.h[par ^ 1][1];
   T* __restrict__ Hzn = a.h[par ^ 1][2];
-  T* const* psho = a.psih[par];
-  T* const* pshn = a.psih[par ^ 1];
   const T* __restrict__ inc = a.inc + (size_t)t * 4 * nx;
-  const bool do_slice = a.eslice != nullptr && (t % a.cadence) == 0;
   const T c2h = a.c2h, c2e0 = a.c2e0, inv = a.inv_d;
-  const long long sx = a.sx, sy = a.sy;
+  // byte strides (every array is < 4 GiB on this path)
+  const uint32_t sxb = (uint32_t)a.sx * W, syb = (uint32_t)a.sy * W;
 
   const bool xrow = (r == R - 1);
   const bool xcol = (lane == kLanes - 1);
   const bool rowok = xrow && (j + 1) < ny && k >= 0 && k < nz;
   const bool colok = xcol && j >= 0 && j < ny && (k + 1) < nz;
-  const long long off = (long long)j * sy + k;  // in-plane offset, valid only if `valid`
+  // byte offset of (0, j, k); only dereferenced when `valid`
+  const uint32_t ob = valid || rowok || colok ? ((uint32_t)j * (uint32_t)a.sy + (uint32_t)k) * W : 0u;
 
   // E^n at the current plane (registers) and its +1 row / +1 column halo.
   T ecx = 0, ecy = 0, ecz = 0, rx = 0, rz = 0, cx = 0, cy = 0;
   {
-    const long long c = (long long)istart * sx + off;
-    if (valid) { ecx = Exo[c]; ecy = Eyo[c]; ecz = Ezo[c]; }
-    if (rowok) { rx = Exo[c + sy]; rz = Ezo[c + sy]; }
-    if (colok) { cx = Exo[c + 1]; cy = Eyo[c + 1]; }
+    const uint32_t c = (uint32_t)istart * sxb + ob;
+    if (valid) { ecx = ldo(Exo, c); ecy = ldo(Eyo, c); ecz = ldo(Ezo, c); }
+    if (rowok) { rx = ldo(Exo, c + syb); rz = ldo(Ezo, c + syb); }
+    if (colok) { cx = ldo(Exo, c + W); cy = ldo(Eyo, c + W); }
   }
   T exprev = 0, hyprev = 0, hzprev = 0;
   int buf = 0;
 
   for (int i = istart; i < ie; ++i, buf ^= 1) {
-    const long long c = (long long)i * sx + off;
+    const uint32_t c = (uint32_t)i * sxb + ob;
     // ---- phase A: stage plane i of E^n, prefetch plane i+1 and H^(n-1/2)
     sE[buf][0][r][lane] = ecx;
     sE[buf][1][r][lane] = ecy;
@@ -178,15 +210,14 @@ fdtd_step(StepArgs<T> a) {
     if (xrow) { sE[buf][0][R][lane] = rx; sE[buf][2][R][lane] = rz; }
     if (xcol) { sE[buf][0][r][kLanes] = cx; sE[buf][1][r][kLanes] = cy; }
     T enx = 0, eny = 0, enz = 0, nrx = 0, nrz = 0, ncx = 0, ncy = 0;
-    const bool nxt = (i + 1) < nx;
-    if (nxt) {
-      const long long cn = c + sx;
-      if (valid) { enx = Exo[cn]; eny = Eyo[cn]; enz = Ezo[cn]; }
-      if (rowok) { nrx = Exo[cn + sy]; nrz = Ezo[cn + sy]; }
-      if (colok) { ncx = Exo[cn + 1]; ncy = Eyo[cn + 1]; }
+    if ((i + 1) < nx) {
+      const uint32_t cn = c + sxb;
+      if (valid) { enx = ldo(Exo, cn); eny = ldo(Eyo, cn); enz = ldo(Ezo, cn); }
+      if (rowok) { nrx = ldo(Exo, cn + syb); nrz = ldo(Ezo, cn + syb); }
+      if (colok) { ncx = ldo(Exo, cn + W); ncy = ldo(Eyo, cn + W); }
     }
     T hx = 0, hy = 0, hz = 0;
-    if (valid) { hx = Hxo[c]; hy = Hyo[c]; hz = Hzo[c]; }
+    if (valid) { hx = ldo(Hxo, c); hy = ldo(Hyo, c); hz = ldo(Hzo, c); }
     __syncthreads();
 
     // ---- phase B: H^(n+1/2) at plane i (owned cells and the low halo)
@@ -211,23 +242,23 @@ fdtd_step(StepArgs<T> a) {
             if (k == a.bz2 - 1) hx = hx + inc[nx + i];
           }
         }
-        if (a.dopml) {  // (:690-714, :824-850)
+        if (PML) {  // (:690-714, :824-850)
           const int jj = slab_h(j, ny, L);
           if (jj >= 0) {
-            const size_t q = ((size_t)i * 2 * L + jj) * nz + k;
-            const T p = a.bh[jj] * psho[0][q] + a.ah[jj] * dEz;
-            if (wr) pshn[0][q] = p;
+            const uint32_t q = (((uint32_t)i * 2 * L + jj) * nz + k) * W;
+            const T p = a.bh[jj] * ldo(a.psih[par][0], q) + a.ah[jj] * dEz;
+            if (wr) sto(a.psih[par ^ 1][0], q, p);
             hx = hx - c2h * p;
           }
           const int kk = slab_h(k, nz, L);
           if (kk >= 0) {
-            const size_t q = ((size_t)i * ny + j) * 2 * L + kk;
-            const T p = a.bh[kk] * psho[1][q] + a.ah[kk] * dEy;
-            if (wr) pshn[1][q] = p;
+            const uint32_t q = (((uint32_t)i * ny + j) * 2 * L + kk) * W;
+            const T p = a.bh[kk] * ldo(a.psih[par][1], q) + a.ah[kk] * dEy;
+            if (wr) sto(a.psih[par ^ 1][1], q, p);
             hx = hx + c2h * p;
           }
         }
-        if (wr) Hxn[c] = hx;
+        if (wr) sto(Hxn, c, hx);
       }
       // Hy: i in [0,nx-1), j in [0,ny), k in [0,nz-1)   (:854-865)
       if (i < nx - 1 && k < nz - 1) {
@@ -238,23 +269,23 @@ fdtd_step(StepArgs<T> a) {
           if (i == a.bx1 - 1) hy = hy - inc[a.bx1];
           if (i == a.bx2 - 1) hy = hy + inc[a.bx2 - 1];
         }
-        if (a.dopml) {  // (:718-744, :877-910)
+        if (PML) {  // (:718-744, :877-910)
           const int ii = slab_h(i, nx, L);
           if (ii >= 0) {
-            const size_t q = ((size_t)ii * ny + j) * nz + k;
-            const T p = a.bh[ii] * psho[2][q] + a.ah[ii] * dEz;
-            if (wr) pshn[2][q] = p;
+            const uint32_t q = (((uint32_t)ii * ny + j) * nz + k) * W;
+            const T p = a.bh[ii] * ldo(a.psih[par][2], q) + a.ah[ii] * dEz;
+            if (wr) sto(a.psih[par ^ 1][2], q, p);
             hy = hy + c2h * p;
           }
           const int kk = slab_h(k, nz, L);
           if (kk >= 0) {
-            const size_t q = ((size_t)i * ny + j) * 2 * L + kk;
-            const T p = a.bh[kk] * psho[3][q] + a.ah[kk] * dEx;
-            if (wr) pshn[3][q] = p;
+            const uint32_t q = (((uint32_t)i * ny + j) * 2 * L + kk) * W;
+            const T p = a.bh[kk] * ldo(a.psih[par][3], q) + a.ah[kk] * dEx;
+            if (wr) sto(a.psih[par ^ 1][3], q, p);
             hy = hy - c2h * p;
           }
         }
-        if (wr) Hyn[c] = hy;
+        if (wr) sto(Hyn, c, hy);
       }
       // Hz: i in [0,nx-1), j in [0,ny-1), k in [0,nz)   (:913-924)
       if (i < nx - 1 && j < ny - 1) {
@@ -265,23 +296,23 @@ fdtd_step(StepArgs<T> a) {
           if (i == a.bx1 - 1) hz = hz + inc[nx + a.bx1];
           if (i == a.bx2 - 1) hz = hz - inc[nx + a.bx2 - 1];
         }
-        if (a.dopml) {  // (:747-791, :936-968)
+        if (PML) {  // (:747-791, :936-968)
           const int ii = slab_h(i, nx, L);
           if (ii >= 0) {
-            const size_t q = ((size_t)ii * ny + j) * nz + k;
-            const T p = a.bh[ii] * psho[4][q] + a.ah[ii] * dEy;
-            if (wr) pshn[4][q] = p;
+            const uint32_t q = (((uint32_t)ii * ny + j) * nz + k) * W;
+            const T p = a.bh[ii] * ldo(a.psih[par][4], q) + a.ah[ii] * dEy;
+            if (wr) sto(a.psih[par ^ 1][4], q, p);
             hz = hz - c2h * p;
           }
           const int jj = slab_h(j, ny, L);
           if (jj >= 0) {
-            const size_t q = ((size_t)i * 2 * L + jj) * nz + k;
-            const T p = a.bh[jj] * psho[5][q] + a.ah[jj] * dEx;
-            if (wr) pshn[5][q] = p;
+            const uint32_t q = (((uint32_t)i * 2 * L + jj) * nz + k) * W;
+            const T p = a.bh[jj] * ldo(a.psih[par][5], q) + a.ah[jj] * dEx;
+            if (wr) sto(a.psih[par ^ 1][5], q, p);
             hz = hz + c2h * p;
           }
         }
-        if (wr) Hzn[c] = hz;
+        if (wr) sto(Hzn, c, hz);
       }
     }
     sH[0][r][lane] = hx;
@@ -292,37 +323,31 @@ fdtd_step(StepArgs<T> a) {
     // ---- phase C: J^n (deferred) and E^(n+1) on owned cells
     if (wr) {
       // J^n from J^(n-1) and E^n; domain [1,N-1)^3   (:1523-1597)
-      double jxv = 0, jyv = 0, jzv = 0;
+      T jx = 0, jy = 0, jz = 0;
       if (i >= 1 && i < nx - 1 && j >= 1 && j < ny - 1 && k >= 1 && k < nz - 1) {
-        jxv = a.j[0][c];
-        jyv = a.j[1][c];
-        jzv = a.j[2][c];
-        const double w = a.wpe2[c];
-        const double nu = a.nue[c];
-        double mx, my, mz;
-        if (a.doB0) {
-          mx = ((double)ecx + (double)exprev) / 2;
-          my = ((double)ecy + (double)sE[buf][1][r - 1][lane]) / 2;
-          mz = ((double)ecz + (double)sE[buf][2][r][lane - 1]) / 2;
+        jx = ldo(a.j[0], c);
+        jy = ldo(a.j[1], c);
+        jz = ldo(a.j[2], c);
+        const T w = ldo(a.wpe2, c);
+        const T nu = ldo(a.nue, c);
+        if constexpr (CUR == kCurDI) {
+          current_update<T, CUR>(a, nu, w, jx, jy, jz, ecx, ecy, ecz);
         } else {
-          mx = ecx; my = ecy; mz = ecz;
+          const T mx = (ecx + exprev) / 2;
+          const T my = (ecy + sE[buf][1][r - 1][lane]) / 2;
+          const T mz = (ecz + sE[buf][2][r][lane - 1]) / 2;
+          current_update<T, CUR>(a, nu, w, jx, jy, jz, mx, my, mz);
         }
-        current_update(a, nu, w, jxv, jyv, jzv, mx, my, mz);
-        a.j[0][c] = (T)jxv;
-        a.j[1][c] = (T)jyv;
-        a.j[2][c] = (T)jzv;
+        sto(a.j[0], c, jx);
+        sto(a.j[1], c, jy);
+        sto(a.j[2], c, jz);
       }
-      const T jx = (T)jxv, jy = (T)jyv, jz = (T)jzv;
       T c1 = 1, c2 = c2e0;
       if (a.doplate == 2 && in_sphere(a, i, j, k)) { c1 = a.c1s; c2 = a.c2s; }
-      const T hzjm = sH[2][r - 1][lane];
-      const T hykm = sH[1][r][lane - 1];
-      const T hxkm = sH[0][r][lane - 1];
-      const T hxjm = sH[0][r - 1][lane];
       // Ex: i in [0,nx-1), j in [1,ny-1), k in [1,nz-1)   (:1161-1216)
       if (i < nx - 1 && j >= 1 && j < ny - 1 && k >= 1 && k < nz - 1) {
-        const T dHz = (hz - hzjm) * inv;
-        const T dHy = (hy - hykm) * inv;
+        const T dHz = (hz - sH[2][r - 1][lane]) * inv;
+        const T dHy = (hy - sH[1][r][lane - 1]) * inv;
         T ex = c1 * ecx + c2 * (dHz - dHy) - c2 * jx;
         if (i >= a.bx1 && i < a.bx2 - 1) {
           if (j >= a.by1 && j < a.by2) {
@@ -334,59 +359,59 @@ fdtd_step(StepArgs<T> a) {
             if (j == a.by2) ex = ex + inc[3 * nx + i];
           }
         }
-        if (a.dopml) {
+        if (PML) {
           const int jj = slab_e(j, ny, L);
           if (jj >= 0) {
-            const size_t q = ((size_t)i * 2 * L + jj) * nz + k;
-            const T p = a.be[jj] * a.psie[0][q] + a.ae[jj] * dHz;
-            a.psie[0][q] = p;
+            const uint32_t q = (((uint32_t)i * 2 * L + jj) * nz + k) * W;
+            const T p = a.be[jj] * ldo(a.psie[0], q) + a.ae[jj] * dHz;
+            sto(a.psie[0], q, p);
             ex = ex + c2e0 * p;
           }
           const int kk = slab_e(k, nz, L);
           if (kk >= 0) {
-            const size_t q = ((size_t)i * ny + j) * 2 * L + kk;
-            const T p = a.be[kk] * a.psie[1][q] + a.ae[kk] * dHy;
-            a.psie[1][q] = p;
+            const uint32_t q = (((uint32_t)i * ny + j) * 2 * L + kk) * W;
+            const T p = a.be[kk] * ldo(a.psie[1], q) + a.ae[kk] * dHy;
+            sto(a.psie[1], q, p);
             ex = ex - c2e0 * p;
           }
         }
-        Exn[c] = ex;
+        sto(Exn, c, ex);
       }
       // Ey: i in [1,nx-1), j in [0,ny), k in [1,nz)   (:1221-1275)
       T ey = ecy;
       if (i >= 1 && i < nx - 1 && k >= 1) {
-        const T dHx = (hx - hxkm) * inv;
+        const T dHx = (hx - sH[0][r][lane - 1]) * inv;
         const T dHz = (hz - hzprev) * inv;
         ey = c1 * ecy + c2 * (dHx - dHz) - c2 * jy;
         if (j >= a.by1 && j < a.by2 && k >= a.bz1 && k < a.bz2) {
           if (i == a.bx1) ey = ey + inc[3 * nx + a.bx1 - 1];
           if (i == a.bx2 - 1) ey = ey - inc[3 * nx + a.bx2 - 1];
         }
-        if (a.dopml) {
+        if (PML) {
           const int ii = slab_e(i, nx, L);
           if (ii >= 0 && j < ny - 1 && k < nz - 1) {
-            const size_t q = ((size_t)ii * ny + j) * nz + k;
-            const T p = a.be[ii] * a.psie[2][q] + a.ae[ii] * dHz;
-            a.psie[2][q] = p;
+            const uint32_t q = (((uint32_t)ii * ny + j) * nz + k) * W;
+            const T p = a.be[ii] * ldo(a.psie[2], q) + a.ae[ii] * dHz;
+            sto(a.psie[2], q, p);
             ey = ey - c2e0 * p;
           }
           const int kk = slab_e(k, nz, L);
           if (kk >= 0 && j < ny - 1) {
-            const size_t q = ((size_t)i * ny + j) * 2 * L + kk;
-            const T p = a.be[kk] * a.psie[3][q] + a.ae[kk] * dHx;
-            a.psie[3][q] = p;
+            const uint32_t q = (((uint32_t)i * ny + j) * 2 * L + kk) * W;
+            const T p = a.be[kk] * ldo(a.psie[3], q) + a.ae[kk] * dHx;
+            sto(a.psie[3], q, p);
             ey = ey + c2e0 * p;
           }
         }
         if (a.doplate == 1 && i == a.ssx && j >= a.plate_lo && j < a.plate_up - 1 && k >= a.plate_lo && k < a.plate_up)
           ey = 0;  // (:1337-1343)
-        Eyn[c] = ey;
+        sto(Eyn, c, ey);
       }
       // Ez: i in [1,nx-1), j in [1,ny), k in [0,nz)   (:1279-1302, :1373-1405)
       T ez = ecz, ezpre = ecz;
       if (i >= 1 && i < nx - 1 && j >= 1) {
         const T dHy = (hy - hyprev) * inv;
-        const T dHx = (hx - hxjm) * inv;
+        const T dHx = (hx - sH[0][r - 1][lane]) * inv;
         ez = c1 * ecz + c2 * (dHy - dHx) - c2 * jz;
         if (j >= a.by1 && j < a.by2 && k >= a.bz1 && k < a.bz2) {
           if (i == a.bx1) ez = ez - inc[2 * nx + a.bx1 - 1];
@@ -395,25 +420,25 @@ fdtd_step(StepArgs<T> a) {
         if (a.doplate == 1 && i == a.ssx && j >= a.plate_lo && j < a.plate_up && k >= a.plate_lo && k < a.plate_up - 1)
           ez = 0;  // (:1344-1350)
         ezpre = ez;
-        if (a.dopml) {
+        if (PML) {
           const int ii = slab_e(i, nx, L);
           if (ii >= 0 && j < ny - 1 && k < nz - 1) {
-            const size_t q = ((size_t)ii * ny + j) * nz + k;
-            const T p = a.be[ii] * a.psie[4][q] + a.ae[ii] * dHy;
-            a.psie[4][q] = p;
+            const uint32_t q = (((uint32_t)ii * ny + j) * nz + k) * W;
+            const T p = a.be[ii] * ldo(a.psie[4], q) + a.ae[ii] * dHy;
+            sto(a.psie[4], q, p);
             ez = ez + c2e0 * p;
           }
           const int jj = slab_e(j, ny, L);
           if (jj >= 0 && k < nz - 1) {
-            const size_t q = ((size_t)i * 2 * L + jj) * nz + k;
-            const T p = a.be[jj] * a.psie[5][q] + a.ae[jj] * dHx;
-            a.psie[5][q] = p;
+            const uint32_t q = (((uint32_t)i * 2 * L + jj) * nz + k) * W;
+            const T p = a.be[jj] * ldo(a.psie[5], q) + a.ae[jj] * dHx;
+            sto(a.psie[5], q, p);
             ez = ez - c2e0 * p;
           }
         }
-        Ezn[c] = ez;
+        sto(Ezn, c, ez);
       }
-      if (do_slice) {  // E slices before the Ez CPML correction (:1353-1371)
+      if (SLICE) {  // E slices before the Ez CPML correction (:1353-1371)
         T* s = a.eslice;
         const size_t zxy = (size_t)nx * ny, zxz = (size_t)nx * nz;
         if (k == a.ssz) { s[(size_t)i * ny + j] = ey; s[zxy + (size_t)i * ny + j] = ezpre; }
@@ -434,7 +459,7 @@ fdtd_step(StepArgs<T> a) {
 
 // J^(t+1) from J^t and E^(t+1) into `out` (what the reference holds after
 // the same number of steps); used only to read J back.
-template <typename T>
+template <typename T, int CUR>
 __global__ void current_catchup(StepArgs<T> a, T* ox, T* oy, T* oz) {
   const long long n = (long long)a.nx * a.ny * a.nz;
   const int t = *a.tdev;
@@ -445,21 +470,18 @@ __global__ void current_catchup(StepArgs<T> a, T* ox, T* oy, T* oz) {
     const int k = (int)(c % a.nz);
     const int j = (int)((c / a.nz) % a.ny);
     const int i = (int)(c / a.sx);
-    double jx = a.j[0][c], jy = a.j[1][c], jz = a.j[2][c];
+    T jx = a.j[0][c], jy = a.j[1][c], jz = a.j[2][c];
     if (i >= 1 && i < a.nx - 1 && j >= 1 && j < a.ny - 1 && k >= 1 && k < a.nz - 1) {
-      double mx, my, mz;
-      if (a.doB0) {
-        mx = ((double)Ex[c] + (double)Ex[c - a.sx]) / 2;
-        my = ((double)Ey[c] + (double)Ey[c - a.sy]) / 2;
-        mz = ((double)Ez[c] + (double)Ez[c - 1]) / 2;
+      if constexpr (CUR == kCurDI) {
+        current_update<T, CUR>(a, a.nue[c], a.wpe2[c], jx, jy, jz, Ex[c], Ey[c], Ez[c]);
       } else {
-        mx = Ex[c]; my = Ey[c]; mz = Ez[c];
+        current_update<T, CUR>(a, a.nue[c], a.wpe2[c], jx, jy, jz, (Ex[c] + Ex[c - a.sx]) / 2,
+                               (Ey[c] + Ey[c - a.sy]) / 2, (Ez[c] + Ez[c - 1]) / 2);
       }
-      current_update(a, (double)a.nue[c], (double)a.wpe2[c], jx, jy, jz, mx, my, mz);
     }
-    ox[c] = (T)jx;
-    oy[c] = (T)jy;
-    oz[c] = (T)jz;
+    ox[c] = jx;
+    oy[c] = jy;
+    oz[c] = jz;
   }
 }
 
@@ -628,16 +650,42 @@ __global__ void __launch_bounds__(64) probes_and_tick(AuxArgs<T> a) {
 
 // ---------------------------------------------------------------- launchers
 
-template <typename T>
-hipError_t launch_step(const StepArgs<T>& a, int ty, hipStream_t s) {
-  const int nblocks = a.tiles_y * a.tiles_z * a.nchunks;
-  switch (ty) {
-    case 3: hipLaunchKernelGGL((fdtd_step<T, 3>), dim3(nblocks), dim3(4 * kLanes), 0, s, a); break;
-    case 7: hipLaunchKernelGGL((fdtd_step<T, 7>), dim3(nblocks), dim3(8 * kLanes), 0, s, a); break;
-    case 15: hipLaunchKernelGGL((fdtd_step<T, 15>), dim3(nblocks), dim3(16 * kLanes), 0, s, a); break;
-    default: return hipErrorInvalidValue;
-  }
+template <typename T, int TY, int CUR, bool PML, bool SLICE>
+static hipError_t launch_v(const StepArgs<T>& a, const uint32_t* tiles, int n, hipStream_t s) {
+  hipLaunchKernelGGL((fdtd_step<T, TY, CUR, PML, SLICE>), dim3(n), dim3((TY + 1) * kLanes), 0, s, a, tiles);
   return hipGetLastError();
+}
+
+template <typename T, int TY, int CUR>
+static hipError_t launch_ps(const StepArgs<T>& a, bool pml, bool slice, const uint32_t* tiles, int n,
+                            hipStream_t s) {
+  if (pml) return slice ? launch_v<T, TY, CUR, true, true>(a, tiles, n, s) : launch_v<T, TY, CUR, true, false>(a, tiles, n, s);
+  return slice ? launch_v<T, TY, CUR, false, true>(a, tiles, n, s) : launch_v<T, TY, CUR, false, false>(a, tiles, n, s);
+}
+
+template <typename T, int TY>
+static hipError_t launch_c(const StepArgs<T>& a, int cur, bool pml, bool slice, const uint32_t* tiles, int n,
+                           hipStream_t s) {
+  switch (cur) {
+    case kCurDI: return launch_ps<T, TY, kCurDI>(a, pml, slice, tiles, n, s);
+    case kCurLKPoly: return launch_ps<T, TY, kCurLKPoly>(a, pml, slice, tiles, n, s);
+    case kCurLKClosed: return launch_ps<T, TY, kCurLKClosed>(a, pml, slice, tiles, n, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+// One launch of the fused step over `n` tiles of the list `tiles`; `pml`
+// selects the variant carrying the CPML code (tiles that touch a slab),
+// `slice` the one capturing E slices (output-cadence steps).
+template <typename T>
+hipError_t launch_step(const StepArgs<T>& a, int ty, int cur, bool pml, bool slice, const uint32_t* tiles, int n,
+                       hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  switch (ty) {
+    case 3: return launch_c<T, 3>(a, cur, pml, slice, tiles, n, s);
+    case 7: return launch_c<T, 7>(a, cur, pml, slice, tiles, n, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 template <typename T>
@@ -657,18 +705,22 @@ hipError_t launch_aux(const AuxArgs<T>& a, bool frames, hipStream_t s) {
 }
 
 template <typename T>
-hipError_t launch_catchup(const StepArgs<T>& a, T* ox, T* oy, T* oz, hipStream_t s) {
-  hipLaunchKernelGGL((current_catchup<T>), dim3(2048), dim3(256), 0, s, a, ox, oy, oz);
+hipError_t launch_catchup(const StepArgs<T>& a, int cur, T* ox, T* oy, T* oz, hipStream_t s) {
+  switch (cur) {
+    case kCurDI: hipLaunchKernelGGL((current_catchup<T, kCurDI>), dim3(2048), dim3(256), 0, s, a, ox, oy, oz); break;
+    case kCurLKPoly: hipLaunchKernelGGL((current_catchup<T, kCurLKPoly>), dim3(2048), dim3(256), 0, s, a, ox, oy, oz); break;
+    default: hipLaunchKernelGGL((current_catchup<T, kCurLKClosed>), dim3(2048), dim3(256), 0, s, a, ox, oy, oz); break;
+  }
   return hipGetLastError();
 }
 
-template hipError_t launch_step<float>(const StepArgs<float>&, int, hipStream_t);
-template hipError_t launch_step<double>(const StepArgs<double>&, int, hipStream_t);
+template hipError_t launch_step<float>(const StepArgs<float>&, int, int, bool, bool, const uint32_t*, int, hipStream_t);
+template hipError_t launch_step<double>(const StepArgs<double>&, int, int, bool, bool, const uint32_t*, int, hipStream_t);
 template hipError_t launch_ntff<float>(const NtffArgs<float>&, hipStream_t);
 template hipError_t launch_ntff<double>(const NtffArgs<double>&, hipStream_t);
 template hipError_t launch_aux<float>(const AuxArgs<float>&, bool, hipStream_t);
 template hipError_t launch_aux<double>(const AuxArgs<double>&, bool, hipStream_t);
-template hipError_t launch_catchup<float>(const StepArgs<float>&, float*, float*, float*, hipStream_t);
-template hipError_t launch_catchup<double>(const StepArgs<double>&, double*, double*, double*, hipStream_t);
+template hipError_t launch_catchup<float>(const StepArgs<float>&, int, float*, float*, float*, hipStream_t);
+template hipError_t launch_catchup<double>(const StepArgs<double>&, int, double*, double*, double*, hipStream_t);
 
 }  // namespace msc

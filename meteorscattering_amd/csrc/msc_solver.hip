@@ -17,13 +17,14 @@
 namespace msc {
 
 template <typename T>
-hipError_t launch_step(const StepArgs<T>& a, int ty, hipStream_t s);
+hipError_t launch_step(const StepArgs<T>& a, int ty, int cur, bool pml, bool slice, const uint32_t* tiles, int n,
+                       hipStream_t s);
 template <typename T>
 hipError_t launch_ntff(const NtffArgs<T>& a, hipStream_t s);
 template <typename T>
 hipError_t launch_aux(const AuxArgs<T>& a, bool frames, hipStream_t s);
 template <typename T>
-hipError_t launch_catchup(const StepArgs<T>& a, T* ox, T* oy, T* oz, hipStream_t s);
+hipError_t launch_catchup(const StepArgs<T>& a, int cur, T* ox, T* oy, T* oz, hipStream_t s);
 
 #define HIPCHK(expr)                                                                  \
   do {                                                                                \
@@ -183,6 +184,48 @@ static void plan_incident(HostPlan& h) {
   }
 }
 
+// Lee-Kalluri coefficients as polynomials in x = nu dt (FP64 Taylor
+// coefficients of the closed-form integrals, meteor-fork.cpp:1546-1573):
+//   e^{-x}                               = sum (-x)^n / n!
+//   kI = int_0^dt e^{-nu s} cos(w0 s) ds  = dt   sum (-x)^n/n! int_0^1 u^n cos(th u) du
+//   kX = int e^{-nu s} sin(w0 s)/w0 ds    = dt^2 sum (-x)^n/n! int_0^1 u^n sin(th u)/th du
+//   kP = int e^{-nu s}(1-cos w0 s)/w0^2 ds = dt^3 sum (-x)^n/n! int_0^1 u^n (1-cos th u)/th^2 du
+// with th = w0 dt and the u-moments summed as series in th.  Returns false
+// (closed form instead) when the series would not be exact to FP64 over
+// [0, xmax].
+static bool plan_lk_poly(const HostPlan& h, double xmax, int& deg, double c[4][kMaxLkDeg + 1]) {
+  const double th = h.w0 * h.dt, dt = h.dt;
+  if (!(xmax >= 0) || xmax > 2.0 || th > 1.0) return false;
+  deg = -1;
+  double term = 1;  // xmax^(d+1)/(d+1)!
+  for (int d = 0; d <= kMaxLkDeg; ++d) {
+    term *= xmax / (d + 1);
+    if (term < 1e-18) { deg = d; break; }
+  }
+  if (deg < 0) return false;
+  if (deg < 1) deg = 1;
+  double fact = 1;  // n!
+  for (int n = 0; n <= kMaxLkDeg; ++n) {
+    if (n > 0) fact *= n;
+    double mc = 0, ms = 0, mp = 0, th2m = 1, f2m = 1;  // th^(2m), (2m)!
+    for (int m = 0; m < 40; ++m) {
+      if (m > 0) { th2m *= th * th; f2m *= (2.0 * m - 1) * (2.0 * m); }
+      const double sg = (m & 1) ? -1.0 : 1.0;
+      const double tc = sg * th2m / f2m / (n + 2 * m + 1);
+      const double ts = sg * th2m / (f2m * (2 * m + 1)) / (n + 2 * m + 2);
+      const double tp = sg * th2m / (f2m * (2 * m + 1) * (2 * m + 2)) / (n + 2 * m + 3);
+      mc += tc; ms += ts; mp += tp;
+      if (std::fabs(tc) < 1e-22 && m > 0) break;
+    }
+    const double sn = ((n & 1) ? -1.0 : 1.0) / fact;
+    c[0][n] = sn;
+    c[1][n] = dt * sn * mc;
+    c[2][n] = dt * dt * sn * ms;
+    c[3][n] = dt * dt * dt * sn * mp;
+  }
+  return true;
+}
+
 static int make_plan(const msc_params* p, HostPlan& h) {
   int rc = msc_params_validate(p);
   if (rc) return rc;
@@ -242,6 +285,9 @@ struct SolverBase {
   hipStream_t stream = nullptr;
   int t_host = 0;
   int ty = 7, chunk = 32;
+  int cur = 0;                    // kCurDI / kCurLKPoly / kCurLKClosed
+  uint32_t* tiles = nullptr;      // interior tiles then boundary tiles
+  int n_inner = 0, n_bound = 0;
   bool timing = false;
   std::vector<hipEvent_t> ev;  // pairs around step kernels of the last batch
   double kernel_ms = 0;
@@ -312,7 +358,9 @@ struct Solver : SolverBase {
     int rc;
     if (const char* s = std::getenv("MSC_TY")) ty = std::atoi(s);
     if (const char* s = std::getenv("MSC_CHUNK")) chunk = std::max(1, std::atoi(s));
-    if (ty != 3 && ty != 7 && ty != 15) return fail(MSC_EINVAL, "MSC_TY must be 3, 7 or 15");
+    if (ty != 3 && ty != 7) return fail(MSC_EINVAL, "MSC_TY must be 3 or 7");
+    if ((double)N * sizeof(T) >= 4294967296.0)
+      return fail(MSC_EINVAL, "grid of %zu cells exceeds the 4 GiB-per-array limit of the 32-bit offset kernel", N);
     HIPCHK(hipStreamCreate(&stream));
     for (int q = 0; q < 2; q++)
       for (int c = 0; c < 3; c++) {
@@ -354,6 +402,7 @@ struct Solver : SolverBase {
     // media: eps0 wpe^2 and nu per cell, computed in FP64 as meteor-fork.cpp:336-366
     {
       std::vector<T> pw((size_t)ny * nz), pn((size_t)ny * nz);
+      double numax = 0;
       std::vector<double> g(nz), gn(nz);
       const double msig0 = h.p.metsize[0];
       for (int k = 0; k < nz; k++) {
@@ -377,11 +426,51 @@ struct Solver : SolverBase {
             }
             pw[(size_t)j * nz + k] = (T)(kE0 * wpe * wpe);
             pn[(size_t)j * nz + k] = (T)nue;
+            numax = nue > numax ? nue : numax;
           }
         HIPCHK(hipMemcpyAsync(w2 + (size_t)i * ny * nz, pw.data(), pw.size() * sizeof(T), hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(nud + (size_t)i * ny * nz, pn.data(), pn.size() * sizeof(T), hipMemcpyHostToDevice, stream));
         HIPCHK(hipStreamSynchronize(stream));
       }
+      // current model: DI, or Lee-Kalluri with polynomial (default) / closed-form coefficients
+      cur = kCurDI;
+      if (h.p.doB0) {
+        const char* force = std::getenv("MSC_LK_CLOSED");
+        const bool want_closed = force && std::atoi(force) != 0;
+        // the FP32 media rounding can lift nu slightly above the FP64 maximum
+        if (!want_closed && plan_lk_poly(h, numax * 1.001 * h.dt, sa.lkdeg, sa.lkc)) cur = kCurLKPoly;
+        else cur = kCurLKClosed;
+      }
+    }
+    // tiles: those whose cells (halo included) reach a CPML slab run the
+    // variant carrying the CPML code, the rest the interior variant
+    {
+      const int tiles_y = (ny + ty - 1) / ty, tiles_z = (nz + kOwnZ - 1) / kOwnZ;
+      if (chunk < (nx + 4094) / 4095) chunk = (nx + 4094) / 4095;
+      const int nch = (nx + chunk - 1) / chunk;
+      if (tiles_z > 255 || tiles_y > 4095) return fail(MSC_EINVAL, "grid too large for the tile descriptor");
+      auto zone = [&](int lo, int hi, int n) {  // [lo, hi) intersects p <= L or p >= n-L-1
+        lo = std::max(lo, 0);
+        hi = std::min(hi, n);
+        return lo < hi && (lo <= L || hi - 1 >= n - L - 1);
+      };
+      std::vector<uint32_t> inner, bound;
+      for (int c = 0; c < nch; c++)
+        for (int yy = 0; yy < tiles_y; yy++)
+          for (int zz = 0; zz < tiles_z; zz++) {
+            const bool b = h.p.dopml && (zone(c * chunk - 1, (c + 1) * chunk, nx) || zone(yy * ty - 1, yy * ty + ty, ny) ||
+                                         zone(zz * kOwnZ - 1, zz * kOwnZ + kOwnZ, nz));
+            (b ? bound : inner).push_back(pack_tile(zz, yy, c));
+          }
+      n_inner = (int)inner.size();
+      n_bound = (int)bound.size();
+      inner.insert(inner.end(), bound.begin(), bound.end());
+      if ((rc = alloc(&tiles, inner.size()))) return rc;
+      HIPCHK(hipMemcpyAsync(tiles, inner.data(), inner.size() * 4, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      sa.tiles_y = tiles_y;
+      sa.tiles_z = tiles_z;
+      sa.nchunks = nch;
     }
     if ((rc = alloc(&tdev, 1))) return rc;
     const size_t fsz = frame_doubles();
@@ -419,9 +508,6 @@ struct Solver : SolverBase {
     sa.plate_lo = (int)(sa.ssy - h.p.platesize / 2 / h.dx);  // meteor-fork.cpp:1339-1340
     sa.plate_up = (int)(sa.ssy + h.p.platesize / 2 / h.dx);
     sa.chunk = chunk;
-    sa.tiles_y = (ny + ty - 1) / ty;
-    sa.tiles_z = (nz + kOwnZ - 1) / kOwnZ;
-    sa.nchunks = (nx + chunk - 1) / chunk;
     sa.cadence = h.cadence;
     sa.sy = nz; sa.sx = (long long)ny * nz;
     sa.c2h = (T)h.c2h; sa.c2e0 = (T)h.c2e0; sa.inv_d = (T)(1.0 / h.dx);
@@ -462,11 +548,13 @@ struct Solver : SolverBase {
       for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     }
     for (int s = 0; s < n; s++) {
+      const bool cad = (t_host % plan.cadence) == 0;
+      const bool slice = cad && plan.p.savefields;
       if (timing) HIPCHK(hipEventRecord(ev[2 * s], stream));
-      HIPCHK(launch_step(sa, ty, stream));
+      HIPCHK(launch_step(sa, ty, cur, false, slice, tiles, n_inner, stream));
+      HIPCHK(launch_step(sa, ty, cur, true, slice, tiles + n_inner, n_bound, stream));
       if (timing) HIPCHK(hipEventRecord(ev[2 * s + 1], stream));
       HIPCHK(launch_ntff(na, stream));
-      const bool cad = (t_host % plan.cadence) == 0;
       HIPCHK(launch_aux(xa, cad, stream));
       ++t_host;
     }
@@ -485,7 +573,7 @@ struct Solver : SolverBase {
     else {
       // the device J lags one step (deferred update); catch it up into scratch
       for (int c = 0; c < 3; c++) HIPCHK(hipMalloc(&scratch[c], N * sizeof(T)));
-      hipError_t e = launch_catchup(sa, scratch[0], scratch[1], scratch[2], stream);
+      hipError_t e = launch_catchup(sa, cur, scratch[0], scratch[1], scratch[2], stream);
       if (e != hipSuccess) {
         for (int c = 0; c < 3; c++) hipFree(scratch[c]);
         return fail(MSC_EHIP, "catch-up kernel: %s", hipGetErrorString(e));

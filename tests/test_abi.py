@@ -1,0 +1,168 @@
+"""CPU tests of the C-ABI library: it loads, exports every symbol the public
+header declares, and its host-side entry points (parameter validation,
+run-directory I/O) agree with the reference's byte layouts.  No compute
+call needs a GPU here."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import cases
+from meteorscattering_amd import _abi, runio
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "msc.h")
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(_abi.LIB_PATH):
+        pytest.fail("libmsc.so not built: run __graft_entry__.build()")
+    return _abi.load_library()
+
+
+def test_every_declared_symbol_is_exported(lib):
+    names = _abi.exported_symbols(HEADER)
+    assert len(names) >= 20
+    out = subprocess.run(["nm", "-D", "--defined-only", _abi.LIB_PATH], capture_output=True, text=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    for n in names:
+        getattr(lib, n)  # resolvable through the dynamic loader
+
+
+def test_params_struct_layout_matches_header(lib):
+    """The ctypes mirror has the C struct's size: msc_params_default writes
+    exactly sizeof(msc_params) bytes into a guarded buffer."""
+    size = C.sizeof(_abi.MscParams)
+    buf = (C.c_ubyte * (size + 64))()
+    for n in range(size, size + 64):
+        buf[n] = 0xAB
+    lib.msc_params_default(C.cast(buf, C.POINTER(_abi.MscParams)))
+    assert all(buf[n] == 0xAB for n in range(size, size + 64))
+    p = _abi.struct_to_params(_abi.MscParams.from_buffer(buf))
+    assert p["nx"] == 201 and p["f0"] == 150e6 and p["nntf"] == 15 and p["sourcepol"] == 3
+
+
+def _run_case():
+    p, wp, nu = cases.case("lk_ricker_ez")
+    return p, wp, nu
+
+
+def test_c_reader_reads_python_writer(lib, tmp_path):
+    p, wp, nu = _run_case()
+    runio.write_run_dir(str(tmp_path), p, wp, nu)
+    s = _abi.MscParams()
+    a, b = C.POINTER(C.c_double)(), C.POINTER(C.c_double)()
+    rc = lib.msc_read_run_dir(str(tmp_path).encode(), C.byref(s), C.byref(a), C.byref(b))
+    assert rc == 0, lib.msc_last_error()
+    q = _abi.struct_to_params(s)
+    for k in ("dopml", "savefields", "f0", "fsig", "sourcetype", "sourcepol", "gridfactor", "dt", "tsteps",
+              "nx", "ny", "nz", "numfiles", "nprobes", "nntf", "doplate", "platesize", "doB0"):
+        assert q[k] == p[k], k
+    assert q["probex"] == p["probex"] and q["fntf"] == p["fntf"]
+    assert q["B0"] == pytest.approx(p["B0"], abs=0)
+    n = p["nx"] * p["ny"]
+    np.testing.assert_array_equal(np.ctypeslib.as_array(a, (n,)), wp.ravel())
+    np.testing.assert_array_equal(np.ctypeslib.as_array(b, (n,)), nu.ravel())
+    lib.msc_free(a)
+    lib.msc_free(b)
+
+
+def test_c_writer_bytes_equal_python_writer(lib, tmp_path):
+    p, wp, nu = _run_case()
+    p = dict(p, pmllen=11, nu_scale_height=7000.0)
+    d1, d2 = tmp_path / "c", tmp_path / "py"
+    d1.mkdir()
+    runio.write_run_dir(str(d2), p, wp, nu)
+    s = _abi.params_to_struct(p)
+    rc = lib.msc_write_run_dir(str(d1).encode(), C.byref(s), _abi.dptr(np.ascontiguousarray(wp)),
+                               _abi.dptr(np.ascontiguousarray(nu)))
+    assert rc == 0, lib.msc_last_error()
+    for f in ("inputs.dat", "B0.dat", "wp.dat", "nu.dat"):
+        assert (d1 / f).read_bytes() == (d2 / f).read_bytes(), f
+    q = runio.decode_inputs_dat((d1 / "inputs.dat").read_bytes())
+    assert q["pmllen"] == 11 and q["nu_scale_height"] == 7000.0
+
+
+def test_missing_doB0_reads_as_zero(lib, tmp_path):
+    """setup/meteorrun.m:44-70 never writes doB0; the fork reads past EOF."""
+    p, wp, nu = _run_case()
+    runio.write_run_dir(str(tmp_path), p, wp, nu)
+    (tmp_path / "inputs.dat").write_bytes(runio.encode_inputs_dat(p, with_doB0=False))
+    s = _abi.MscParams()
+    a, b = C.POINTER(C.c_double)(), C.POINTER(C.c_double)()
+    assert lib.msc_read_run_dir(str(tmp_path).encode(), C.byref(s), C.byref(a), C.byref(b)) == 0
+    assert s.doB0 == 0
+    lib.msc_free(a)
+    lib.msc_free(b)
+
+
+def test_truncated_and_invalid_inputs_fail_cleanly(lib, tmp_path):
+    p, wp, nu = _run_case()
+    runio.write_run_dir(str(tmp_path), p, wp, nu)
+    full = (tmp_path / "inputs.dat").read_bytes()
+    s = _abi.MscParams()
+    a, b = C.POINTER(C.c_double)(), C.POINTER(C.c_double)()
+    (tmp_path / "inputs.dat").write_bytes(full[:60])
+    assert lib.msc_read_run_dir(str(tmp_path).encode(), C.byref(s), C.byref(a), C.byref(b)) == -2
+    assert b"truncated" in lib.msc_last_error()
+    # numfiles > tsteps: the reference divides by zero (meteor-fork.cpp:1616)
+    q = dict(p, numfiles=p["tsteps"] + 1)
+    (tmp_path / "inputs.dat").write_bytes(runio.encode_inputs_dat(q))
+    assert lib.msc_read_run_dir(str(tmp_path).encode(), C.byref(s), C.byref(a), C.byref(b)) == -1
+    assert b"numfiles" in lib.msc_last_error()
+    # short wp.dat
+    (tmp_path / "inputs.dat").write_bytes(full)
+    (tmp_path / "wp.dat").write_bytes(wp.tobytes()[:100])
+    assert lib.msc_read_run_dir(str(tmp_path).encode(), C.byref(s), C.byref(a), C.byref(b)) == -2
+    # probe on index 0 (the probe average reads index-1)
+    bad = _abi.params_to_struct(dict(p, probex=[0, 5]))
+    assert lib.msc_params_validate(C.byref(bad)) == -1
+
+
+def test_solver_create_validates_before_touching_the_device(lib):
+    p, wp, nu = _run_case()
+    bad = _abi.params_to_struct(dict(p, nx=10))
+    h = C.c_void_p()
+    rc = lib.msc_solver_create(C.byref(bad), _abi.dptr(wp), _abi.dptr(nu), 32, C.byref(h))
+    assert rc != 0 and not h.value
+    rc = lib.msc_solver_create(C.byref(_abi.params_to_struct(p)), _abi.dptr(wp), _abi.dptr(nu), 16, C.byref(h))
+    assert rc == -1 and b"precision" in lib.msc_last_error()
+
+
+def test_inputs_dat_layout_matches_reference_reader_offsets():
+    """Field offsets of inputs.dat as the reference reads them (meteor-fork.cpp:84-115)."""
+    p, _, _ = _run_case()
+    b = runio.encode_inputs_dat(p)
+    assert struct.unpack_from("<2i", b, 0) == (p["dopml"], p["savefields"])
+    assert struct.unpack_from("<d", b, 8)[0] == p["f0"]
+    assert struct.unpack_from("<3i", b, 52) == (p["nx"], p["ny"], p["nz"])
+    np_ = p["nprobes"]
+    off = 4 * 2 + 8 * 2 + 4 * 2 + 8 * 2 + 4 * 4 + 8 * 2 + 8 * 2 + 8 * 3 + 8 + 4 * 3
+    assert off == 140
+    assert struct.unpack_from("<i", b, off - 4)[0] == np_
+    off += 12 * np_
+    assert struct.unpack_from("<i", b, off)[0] == p["nntf"]
+    off += 4 + 8 * p["nntf"]
+    assert struct.unpack_from("<i", b, off)[0] == p["doplate"]
+    assert struct.unpack_from("<i", b, off + 12)[0] == p["doB0"]
+    assert len(b) == off + 16
+
+
+def test_ntf_file_roundtrip():
+    p, _, _ = _run_case()
+    nx, ny, nz, M = p["nx"], p["ny"], p["nz"], p["nntf"]
+    n = 4 * M + 8 * 2 * M * (ny * nz + nx * nz + nx * ny)
+    pay = np.random.default_rng(1).standard_normal(n)
+    b = runio.encode_ntf(M, 15, p["fntf"], pay)
+    d = runio.parse_ntf(b, nx, ny, nz)
+    assert d["ntfb"] == 15 and list(d["fntf"]) == p["fntf"]
+    np.testing.assert_array_equal(d["Eydft"], pay[:2 * M])
+    np.testing.assert_array_equal(d["My_front"].ravel(), pay[4 * M:4 * M + 2 * M * ny * nz])

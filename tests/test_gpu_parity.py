@@ -137,7 +137,7 @@ def test_tilings_are_bit_identical(msc, oracle_lib, monkeypatch):
     p.update(tsteps=80, nx=47, ny=45, nz=70)
     wp, nu = cases.trail_maps(p)
     outs = []
-    for ty, chunk in ((7, 32), (3, 5), (15, 1), (7, 47)):
+    for ty, chunk in ((7, 32), (3, 5), (3, 1), (7, 47)):
         monkeypatch.setenv("MSC_TY", str(ty))
         monkeypatch.setenv("MSC_CHUNK", str(chunk))
         g = Solver(p, wp, nu, precision=32)
@@ -147,6 +147,31 @@ def test_tilings_are_bit_identical(msc, oracle_lib, monkeypatch):
     for f, n in outs[1:]:
         assert np.array_equal(f, outs[0][0])
         assert np.array_equal(n, outs[0][1])
+
+
+@pytest.mark.parametrize("precision,tol", [(64, 1e-10), (32, 2e-5)])
+def test_lee_kalluri_closed_form_variant_matches_oracle(msc, oracle_lib, monkeypatch, precision, tol):
+    """The closed-form coefficient kernel (used when the polynomial would not
+    be exact, forced here) against the oracle."""
+    Solver, _ = msc
+    monkeypatch.setenv("MSC_LK_CLOSED", "1")
+    p, wp, nu = cases.case("lk_nopml_lhcp")
+    g, o, rep = run_pair(Solver, oracle_lib, p, wp, nu, precision, p["tsteps"], 40, tol, f"lkclosed{precision}")
+    assert not rep, "\n".join(rep[:12])
+
+
+def test_lee_kalluri_zero_collision_and_zero_field_limits(msc, oracle_lib):
+    """nu = 0 in part of the grid and B = 0: finite limits (the reference
+    divides by zero there), GPU polynomial path against the oracle's limits."""
+    Solver, _ = msc
+    p, wp, nu = cases.case("lk_ricker_ez")
+    p.update(tsteps=100)
+    nu = nu.copy()
+    nu[:, : nu.shape[1] // 2] = 0.0
+    for B in ([4.33e-5, 0.0, 2.5e-5], [0.0, 0.0, 0.0]):
+        p["B0"] = B
+        g, o, rep = run_pair(Solver, oracle_lib, p, wp, nu, 64, p["tsteps"], 50, 1e-10, f"limits{B}")
+        assert not rep, "\n".join(rep[:12])
 
 
 def test_vacuum_tfsf_leakage_fp32(msc):
