@@ -70,6 +70,8 @@ def main():
     ap.add_argument("--precision", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--copy-only", action="store_true",
+                    help="run only the streaming-copy reference (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: only one GPU is used in this build (asked for {args.gpus}); "
@@ -77,7 +79,14 @@ def main():
         sys.exit(2)
 
     from meteorscattering_amd import configs
-    from meteorscattering_amd.solver import Solver
+    from meteorscattering_amd.solver import Solver, stream_copy_gbps
+
+    copy_bytes = 2 << 30
+    copy_gbps = stream_copy_gbps(copy_bytes, 20)
+    log(f"stream copy {copy_bytes >> 20} MiB: {copy_gbps:.0f} GB/s (read + write)")
+    if args.copy_only:
+        print(json.dumps({"stream_copy_bytes": copy_bytes, "stream_copy_gbps": round(copy_gbps, 1)}))
+        return
 
     t0 = time.perf_counter()
     if args.workload == "config2":
@@ -125,8 +134,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": traffic_from_profile(wname),
-                     "kernel": "fdtd_step", "kernel_ms": round(kms, 4),
-                     "algorithmic_bytes_per_launch": algo},
+                     "kernel": "fdtd_step (interior + CPML-boundary launches)", "kernel_ms": round(kms, 4),
+                     "algorithmic_bytes_per_launch": algo,
+                     "measured_stream_copy_gbps": round(copy_gbps, 1),
+                     "frac_of_copy": round(achieved / copy_gbps, 4) if achieved else None},
     }
     if not args.no_cpu:
         t2 = time.perf_counter()

@@ -189,6 +189,13 @@ double msc_solver_kernel_ms(const msc_solver* s);
 
 void msc_solver_destroy(msc_solver* s);
 
+/* Streaming copy of `bytes` between two fresh device buffers, `iters` times;
+ * *gbps = bytes read + written per second.  No counterpart in the
+ * reference: it measures the device's attainable HBM bandwidth next to the
+ * solver's roofline and serves as the known-size calibration of the HBM byte
+ * counters. */
+int msc_stream_copy(size_t bytes, int iters, double* gbps);
+
 /*
  * Whole drop-in run in `dir`: read the files, solve, write output_E.dat,
  * output_H.dat, output_J.dat (when savefields), output_NTF.dat, Probe.dat

@@ -143,6 +143,7 @@ def _declare(lib: C.CDLL) -> None:
         "msc_solver_kernel_ms": (C.c_double, [vp]),
         "msc_solver_destroy": (None, [vp]),
         "msc_run_dir": (C.c_int, [C.c_char_p, C.c_int]),
+        "msc_stream_copy": (C.c_int, [C.c_size_t, C.c_int, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

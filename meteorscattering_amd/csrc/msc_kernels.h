@@ -15,7 +15,8 @@ constexpr int kMaxLkDeg = 24;
 // current models selected per launch
 enum { kCurDI = 0, kCurLKPoly = 1, kCurLKClosed = 2 };
 
-// packed tile descriptor: z-tile (8 bits) | y-tile (12 bits) | x-chunk (12 bits)
+// packed tile descriptor: z-tile (8 bits) | y-tile (12 bits) | x-chunk (12 bits),
+// indices into StepArgs::zstart / ystart / xstart
 __host__ __device__ inline uint32_t pack_tile(int tz, int ty, int ch) {
   return (uint32_t)tz | ((uint32_t)ty << 8) | ((uint32_t)ch << 20);
 }
@@ -33,6 +34,9 @@ struct StepArgs {
   int dopml, doplate, doB0;
   int plate_lo, plate_up;
   int chunk, tiles_y, tiles_z, nchunks; // launch decomposition
+  const int* xstart;  // [nchunks+1] x-chunk starts (aligned to the CPML slab edges)
+  const int* ystart;  // [tiles_y+1] row-tile starts
+  const int* zstart;  // [tiles_z+1] z-tile starts
   int cadence;                          // tsteps/numfiles
   long long sy, sx;                     // strides: sy = nz, sx = ny*nz
   T c2h, c2e0;                          // dt/mu0, dt/eps0 (sigma = 0)

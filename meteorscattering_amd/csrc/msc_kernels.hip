@@ -154,15 +154,18 @@ fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
   const int tz = tile & 0xff;
   const int ty = (tile >> 8) & 0xfff;
   const int chunk = tile >> 20;
+  const int k0 = a.zstart[tz], k1 = a.zstart[tz + 1];
+  const int j0 = a.ystart[ty], j1 = a.ystart[ty + 1];
+  const int ib = a.xstart[chunk], ie = a.xstart[chunk + 1];
 
   const int nx = a.nx, ny = a.ny, nz = a.nz, L = a.L;
-  const int j = ty * TY - 1 + r;
-  const int k = tz * kOwnZ - 1 + lane;
+  const int j = j0 - 1 + r;
+  const int k = k0 - 1 + lane;
   const bool valid = j >= 0 && j < ny && k >= 0 && k < nz;
-  const bool own = valid && r >= 1 && lane >= 1;
-  const int ib = chunk * a.chunk;
-  const int ie = min(ib + a.chunk, nx);
+  const bool own = valid && r >= 1 && lane >= 1 && j < j1 && k < k1;
   const int istart = ib > 0 ? ib - 1 : 0;
+  // owned cells inside the current's domain [1,N-1)^3 (the x test is per plane)
+  const bool jdom = own && j >= 1 && j < ny - 1 && k >= 1 && k < nz - 1;
 
   const int t = *a.tdev;
   const int par = t & 1;
@@ -190,34 +193,57 @@ fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
   // byte offset of (0, j, k); only dereferenced when `valid`
   const uint32_t ob = valid || rowok || colok ? ((uint32_t)j * (uint32_t)a.sy + (uint32_t)k) * W : 0u;
 
-  // E^n at the current plane (registers) and its +1 row / +1 column halo.
-  T ecx = 0, ecy = 0, ecz = 0, rx = 0, rz = 0, cx = 0, cy = 0;
+  // Software pipeline along x: E^n two planes ahead, H^(n-1/2), J and the
+  // media one plane ahead, the +1 row / +1 column halo of E one plane ahead.
+  T e0x = 0, e0y = 0, e0z = 0;  // E^n(i)
+  T e1x = 0, e1y = 0, e1z = 0;  // E^n(i+1)
+  T rx = 0, rz = 0, cx = 0, cy = 0;
+  T h0x = 0, h0y = 0, h0z = 0;
+  T q0x = 0, q0y = 0, q0z = 0, w0 = 0, n0 = 0;  // J^(n-1), eps0 wp^2, nu at plane i
   {
     const uint32_t c = (uint32_t)istart * sxb + ob;
-    if (valid) { ecx = ldo(Exo, c); ecy = ldo(Eyo, c); ecz = ldo(Ezo, c); }
+    if (valid) {
+      e0x = ldo(Exo, c); e0y = ldo(Eyo, c); e0z = ldo(Ezo, c);
+      h0x = ldo(Hxo, c); h0y = ldo(Hyo, c); h0z = ldo(Hzo, c);
+      if (istart + 1 < nx) { e1x = ldo(Exo, c + sxb); e1y = ldo(Eyo, c + sxb); e1z = ldo(Ezo, c + sxb); }
+    }
     if (rowok) { rx = ldo(Exo, c + syb); rz = ldo(Ezo, c + syb); }
     if (colok) { cx = ldo(Exo, c + W); cy = ldo(Eyo, c + W); }
+    if (jdom && istart >= 1 && istart < nx - 1) {
+      q0x = ldo(a.j[0], c); q0y = ldo(a.j[1], c); q0z = ldo(a.j[2], c);
+      w0 = ldo(a.wpe2, c); n0 = ldo(a.nue, c);
+    }
   }
   T exprev = 0, hyprev = 0, hzprev = 0;
   int buf = 0;
 
   for (int i = istart; i < ie; ++i, buf ^= 1) {
     const uint32_t c = (uint32_t)i * sxb + ob;
-    // ---- phase A: stage plane i of E^n, prefetch plane i+1 and H^(n-1/2)
-    sE[buf][0][r][lane] = ecx;
-    sE[buf][1][r][lane] = ecy;
-    sE[buf][2][r][lane] = ecz;
+    // ---- phase A: stage plane i of E^n; issue the loads of the next planes
+    sE[buf][0][r][lane] = e0x;
+    sE[buf][1][r][lane] = e0y;
+    sE[buf][2][r][lane] = e0z;
     if (xrow) { sE[buf][0][R][lane] = rx; sE[buf][2][R][lane] = rz; }
     if (xcol) { sE[buf][0][r][kLanes] = cx; sE[buf][1][r][kLanes] = cy; }
-    T enx = 0, eny = 0, enz = 0, nrx = 0, nrz = 0, ncx = 0, ncy = 0;
-    if ((i + 1) < nx) {
+    T e2x = 0, e2y = 0, e2z = 0, nrx = 0, nrz = 0, ncx = 0, ncy = 0;
+    T h1x = 0, h1y = 0, h1z = 0, q1x = 0, q1y = 0, q1z = 0, w1 = 0, n1 = 0;
+    if (i + 1 < ie) {
       const uint32_t cn = c + sxb;
-      if (valid) { enx = ldo(Exo, cn); eny = ldo(Eyo, cn); enz = ldo(Ezo, cn); }
+      if (valid) {
+        h1x = ldo(Hxo, cn); h1y = ldo(Hyo, cn); h1z = ldo(Hzo, cn);
+        if (i + 2 < nx) { e2x = ldo(Exo, cn + sxb); e2y = ldo(Eyo, cn + sxb); e2z = ldo(Ezo, cn + sxb); }
+      }
       if (rowok) { nrx = ldo(Exo, cn + syb); nrz = ldo(Ezo, cn + syb); }
       if (colok) { ncx = ldo(Exo, cn + W); ncy = ldo(Eyo, cn + W); }
+      if (jdom && i + 1 < nx - 1) {
+        q1x = ldo(a.j[0], cn); q1y = ldo(a.j[1], cn); q1z = ldo(a.j[2], cn);
+        w1 = ldo(a.wpe2, cn); n1 = ldo(a.nue, cn);
+      }
     }
-    T hx = 0, hy = 0, hz = 0;
-    if (valid) { hx = ldo(Hxo, c); hy = ldo(Hyo, c); hz = ldo(Hzo, c); }
+    const T ecx = e0x, ecy = e0y, ecz = e0z;
+    const T enx = e1x, eny = e1y, enz = e1z;
+    T hx = h0x, hy = h0y, hz = h0z;
+    (void)enx;
     __syncthreads();
 
     // ---- phase B: H^(n+1/2) at plane i (owned cells and the low halo)
@@ -324,12 +350,12 @@ fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
     if (wr) {
       // J^n from J^(n-1) and E^n; domain [1,N-1)^3   (:1523-1597)
       T jx = 0, jy = 0, jz = 0;
-      if (i >= 1 && i < nx - 1 && j >= 1 && j < ny - 1 && k >= 1 && k < nz - 1) {
-        jx = ldo(a.j[0], c);
-        jy = ldo(a.j[1], c);
-        jz = ldo(a.j[2], c);
-        const T w = ldo(a.wpe2, c);
-        const T nu = ldo(a.nue, c);
+      if (jdom && i >= 1 && i < nx - 1) {
+        jx = q0x;
+        jy = q0y;
+        jz = q0z;
+        const T w = w0;
+        const T nu = n0;
         if constexpr (CUR == kCurDI) {
           current_update<T, CUR>(a, nu, w, jx, jy, jz, ecx, ecy, ecz);
         } else {
@@ -448,12 +474,15 @@ fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
         if (i == a.ssx) { s[(size_t)j * nz + k] = ey; s[(size_t)ny * nz + (size_t)j * nz + k] = ezpre; }
       }
     }
-    // ---- rotate the x-direction registers
+    // ---- rotate the x-direction pipeline
     exprev = ecx;
     hyprev = hy;
     hzprev = hz;
-    ecx = enx; ecy = eny; ecz = enz;
+    e0x = e1x; e0y = e1y; e0z = e1z;
+    e1x = e2x; e1y = e2y; e1z = e2z;
     rx = nrx; rz = nrz; cx = ncx; cy = ncy;
+    h0x = h1x; h0y = h1y; h0z = h1z;
+    q0x = q1x; q0y = q1y; q0z = q1z; w0 = w1; n0 = n1;
   }
 }
 
@@ -646,6 +675,19 @@ __global__ void __launch_bounds__(64) probes_and_tick(AuxArgs<T> a) {
   }
   __syncthreads();
   if (threadIdx.x == 0) *a.tdev = t + 1;
+}
+
+// Plain streaming copy (16 B per lane per access): the bandwidth reference
+// for the roofline and the known-size calibration of the HBM byte counters.
+__global__ void __launch_bounds__(256) stream_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+    dst[q] = src[q];
+}
+
+hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+  const size_t n = bytes / 16;
+  hipLaunchKernelGGL(stream_copy, dim3(4096), dim3(256), 0, s, (const float4*)src, (float4*)dst, n);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- launchers

@@ -25,6 +25,7 @@ template <typename T>
 hipError_t launch_aux(const AuxArgs<T>& a, bool frames, hipStream_t s);
 template <typename T>
 hipError_t launch_catchup(const StepArgs<T>& a, int cur, T* ox, T* oy, T* oz, hipStream_t s);
+hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 #define HIPCHK(expr)                                                                  \
   do {                                                                                \
@@ -442,13 +443,30 @@ struct Solver : SolverBase {
         else cur = kCurLKClosed;
       }
     }
-    // tiles: those whose cells (halo included) reach a CPML slab run the
-    // variant carrying the CPML code, the rest the interior variant
+    // tiles: each axis is cut at the CPML slab edges (L+1 and n-L-1) for x
+    // and y, so that a tile is either inside a slab zone on some axis (the
+    // variant carrying the CPML code) or clear of every slab, halo included
+    // (the interior variant); z keeps full 63-wide tiles from 0, the first
+    // and last of which hold the z slabs.
     {
-      const int tiles_y = (ny + ty - 1) / ty, tiles_z = (nz + kOwnZ - 1) / kOwnZ;
-      if (chunk < (nx + 4094) / 4095) chunk = (nx + 4094) / 4095;
-      const int nch = (nx + chunk - 1) / chunk;
-      if (tiles_z > 255 || tiles_y > 4095) return fail(MSC_EINVAL, "grid too large for the tile descriptor");
+      auto cut = [&](int n, int piece, bool align) {
+        std::vector<int> st;
+        auto span = [&](int lo, int hi) {
+          for (int q = lo; q < hi; q += piece) st.push_back(q);
+        };
+        if (align && h.p.dopml) {
+          span(0, L + 1);
+          span(L + 1, n - L - 1);
+          span(n - L - 1, n);
+        } else {
+          span(0, n);
+        }
+        st.push_back(n);
+        return st;
+      };
+      const std::vector<int> xs = cut(nx, chunk, true), ys = cut(ny, ty, true), zs = cut(nz, kOwnZ, false);
+      const int nch = (int)xs.size() - 1, tiles_y = (int)ys.size() - 1, tiles_z = (int)zs.size() - 1;
+      if (tiles_z > 255 || tiles_y > 4095 || nch > 4095) return fail(MSC_EINVAL, "grid too large for the tile descriptor");
       auto zone = [&](int lo, int hi, int n) {  // [lo, hi) intersects p <= L or p >= n-L-1
         lo = std::max(lo, 0);
         hi = std::min(hi, n);
@@ -458,8 +476,8 @@ struct Solver : SolverBase {
       for (int c = 0; c < nch; c++)
         for (int yy = 0; yy < tiles_y; yy++)
           for (int zz = 0; zz < tiles_z; zz++) {
-            const bool b = h.p.dopml && (zone(c * chunk - 1, (c + 1) * chunk, nx) || zone(yy * ty - 1, yy * ty + ty, ny) ||
-                                         zone(zz * kOwnZ - 1, zz * kOwnZ + kOwnZ, nz));
+            const bool b = h.p.dopml && (zone(xs[c] - 1, xs[c + 1], nx) || zone(ys[yy] - 1, ys[yy + 1], ny) ||
+                                         zone(zs[zz] - 1, zs[zz + 1], nz));
             (b ? bound : inner).push_back(pack_tile(zz, yy, c));
           }
       n_inner = (int)inner.size();
@@ -467,7 +485,15 @@ struct Solver : SolverBase {
       inner.insert(inner.end(), bound.begin(), bound.end());
       if ((rc = alloc(&tiles, inner.size()))) return rc;
       HIPCHK(hipMemcpyAsync(tiles, inner.data(), inner.size() * 4, hipMemcpyHostToDevice, stream));
+      int *dx = nullptr, *dy = nullptr, *dz = nullptr;
+      if ((rc = alloc(&dx, xs.size())) || (rc = alloc(&dy, ys.size())) || (rc = alloc(&dz, zs.size()))) return rc;
+      HIPCHK(hipMemcpyAsync(dx, xs.data(), xs.size() * 4, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(dy, ys.data(), ys.size() * 4, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(dz, zs.data(), zs.size() * 4, hipMemcpyHostToDevice, stream));
       HIPCHK(hipStreamSynchronize(stream));
+      sa.xstart = dx;
+      sa.ystart = dy;
+      sa.zstart = dz;
       sa.tiles_y = tiles_y;
       sa.tiles_z = tiles_z;
       sa.nchunks = nch;
@@ -748,5 +774,36 @@ int msc_solver_set_timing(msc_solver* s, int on) {
 double msc_solver_kernel_ms(const msc_solver* s) { return (s && s->impl) ? s->impl->kernel_ms : 0.0; }
 
 void msc_solver_destroy(msc_solver* s) { delete s; }
+
+int msc_stream_copy(size_t bytes, int iters, double* gbps) {
+  if (bytes < 16 || iters < 1 || !gbps) return fail(MSC_EINVAL, "bad arguments");
+  bytes &= ~(size_t)15;
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = MSC_OK;
+  float ms = 0;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+    rc = fail(MSC_ENOMEM, "hipMalloc of 2 x %zu bytes failed", bytes);
+  } else if (hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = fail(MSC_EHIP, "stream/event creation failed");
+  } else {
+    hipError_t e = hipMemsetAsync(a, 0, bytes, st);
+    if (e == hipSuccess) e = launch_stream_copy(a, b, bytes, st);  // warm
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int q = 0; q < iters && e == hipSuccess; ++q) e = launch_stream_copy(a, b, bytes, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) rc = fail(MSC_EHIP, "stream copy: %s", hipGetErrorString(e));
+    else *gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (st) hipStreamDestroy(st);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  return rc;
+}
 
 }  // extern "C"

@@ -108,6 +108,14 @@ class Solver:
             pass
 
 
+def stream_copy_gbps(nbytes: int, iters: int = 10) -> float:
+    """Attainable device bandwidth (read + write) of a plain streaming copy."""
+    lib = _abi.load_library()
+    v = C.c_double()
+    _check(lib.msc_stream_copy(int(nbytes), int(iters), C.byref(v)), lib)
+    return v.value
+
+
 def run_dir(path: str, precision: int = 32) -> None:
     """The drop-in process (msc_run_dir): read, solve and write in `path`."""
     lib = _abi.load_library()
