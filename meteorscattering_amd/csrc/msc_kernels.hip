@@ -131,6 +131,16 @@ __device__ __forceinline__ void sto(T* __restrict__ base, uint32_t boff, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
 }
 
+// Workgroup barrier that orders LDS only.  __syncthreads() is a
+// workgroup-scope fence and so also waits for every outstanding global load,
+// which would drain the x-direction prefetch pipeline twice per plane.  The
+// barriers in fdtd_step only publish LDS data: no thread of a workgroup reads
+// global memory another thread of it writes during the step (E and H are
+// ping-pong buffers, J and psi_E are owner-only).
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 template <typename T>
 __device__ __forceinline__ bool in_sphere(const StepArgs<T>& a, int i, int j, int k) {
   // meteor-fork.cpp:203-209, coordinates as :137-141 (one loop per axis)
@@ -145,7 +155,11 @@ __global__ void __launch_bounds__((TY + 1) * kLanes)
 fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
   constexpr int R = TY + 1;  // rows including the j0-1 halo row
   constexpr uint32_t W = sizeof(T);
-  __shared__ T sE[2][3][R + 1][kLanes + 1];
+  // sE[b]: one x-plane of E^n with a +1 row and +1 column halo.  Three
+  // buffers rotate: plane i (read), plane i+1 (staged at the start of plane
+  // i), and plane i-1, which slower threads may still read in phase C of
+  // plane i-1 while plane i+1 is being staged.
+  __shared__ T sE[3][3][R + 1][kLanes + 1];
   __shared__ T sH[3][R][kLanes];
 
   const int lane = threadIdx.x & (kLanes - 1);
@@ -183,6 +197,11 @@ fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
   T* __restrict__ Hzn = a.h[par ^ 1][2];
   const T* __restrict__ inc = a.inc + (size_t)t * 4 * nx;
   const T c2h = a.c2h, c2e0 = a.c2e0, inv = a.inv_d;
+  // TF/SF values at fixed planes (meteor-fork.cpp:870-872, 929-931, 1238-1240, 1297-1299)
+  const T tfEzB1 = inc[a.bx1], tfEzB2 = inc[a.bx2 - 1];
+  const T tfEyB1 = inc[nx + a.bx1], tfEyB2 = inc[nx + a.bx2 - 1];
+  const T tfHyB1 = inc[2 * nx + a.bx1 - 1], tfHyB2 = inc[2 * nx + a.bx2 - 1];
+  const T tfHzB1 = inc[3 * nx + a.bx1 - 1], tfHzB2 = inc[3 * nx + a.bx2 - 1];
   // byte strides (every array is < 4 GiB on this path)
   const uint32_t sxb = (uint32_t)a.sx * W, syb = (uint32_t)a.sy * W;
 
@@ -190,69 +209,84 @@ fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
   const bool xcol = (lane == kLanes - 1);
   const bool rowok = xrow && (j + 1) < ny && k >= 0 && k < nz;
   const bool colok = xcol && j >= 0 && j < ny && (k + 1) < nz;
-  // byte offset of (0, j, k); only dereferenced when `valid`
+  // byte offset of (0, j, k); only dereferenced when valid / rowok / colok
   const uint32_t ob = valid || rowok || colok ? ((uint32_t)j * (uint32_t)a.sy + (uint32_t)k) * W : 0u;
 
-  // Software pipeline along x: E^n two planes ahead, H^(n-1/2), J and the
-  // media one plane ahead, the +1 row / +1 column halo of E one plane ahead.
-  T e0x = 0, e0y = 0, e0z = 0;  // E^n(i)
-  T e1x = 0, e1y = 0, e1z = 0;  // E^n(i+1)
-  T rx = 0, rz = 0, cx = 0, cy = 0;
-  T h0x = 0, h0y = 0, h0z = 0;
-  T q0x = 0, q0y = 0, q0z = 0, w0 = 0, n0 = 0;  // J^(n-1), eps0 wp^2, nu at plane i
+  // Pipeline along x: one batch of loads per plane, issued in phase A of
+  // plane i and consumed in plane i+1, so the only wait on global loads is at
+  // the start of the next plane, one full plane of work later.  The batch
+  // holds E^n(i+2) (staged into LDS at the start of plane i+1), and
+  // H^(n-1/2), J^(n-1), the media and the incident-line terms of plane i+1.
+  T px = 0, py = 0, pz = 0, prx = 0, prz = 0, pcx = 0, pcy = 0;  // E^n staging registers
+  T nhx = 0, nhy = 0, nhz = 0;                                   // H^(n-1/2), next plane
+  T nqx = 0, nqy = 0, nqz = 0, nqw = 0, nqn = 0;                 // J^(n-1), eps0 wp^2, nu, next plane
+  T nt0 = 0, nt1 = 0, nt2 = 0, nt3 = 0;                          // incident-line terms, next plane
   {
     const uint32_t c = (uint32_t)istart * sxb + ob;
+    T ex = 0, ey = 0, ez = 0, rx = 0, rz = 0, cx = 0, cy = 0;
     if (valid) {
-      e0x = ldo(Exo, c); e0y = ldo(Eyo, c); e0z = ldo(Ezo, c);
-      h0x = ldo(Hxo, c); h0y = ldo(Hyo, c); h0z = ldo(Hzo, c);
-      if (istart + 1 < nx) { e1x = ldo(Exo, c + sxb); e1y = ldo(Eyo, c + sxb); e1z = ldo(Ezo, c + sxb); }
+      ex = ldo(Exo, c); ey = ldo(Eyo, c); ez = ldo(Ezo, c);
+      nhx = ldo(Hxo, c); nhy = ldo(Hyo, c); nhz = ldo(Hzo, c);
     }
     if (rowok) { rx = ldo(Exo, c + syb); rz = ldo(Ezo, c + syb); }
     if (colok) { cx = ldo(Exo, c + W); cy = ldo(Eyo, c + W); }
-    if (jdom && istart >= 1 && istart < nx - 1) {
-      q0x = ldo(a.j[0], c); q0y = ldo(a.j[1], c); q0z = ldo(a.j[2], c);
-      w0 = ldo(a.wpe2, c); n0 = ldo(a.nue, c);
+    sE[0][0][r][lane] = ex; sE[0][1][r][lane] = ey; sE[0][2][r][lane] = ez;
+    if (xrow) { sE[0][0][R][lane] = rx; sE[0][2][R][lane] = rz; }
+    if (xcol) { sE[0][0][r][kLanes] = cx; sE[0][1][r][kLanes] = cy; }
+    if (istart + 1 < nx) {
+      const uint32_t cn = c + sxb;
+      if (valid) { px = ldo(Exo, cn); py = ldo(Eyo, cn); pz = ldo(Ezo, cn); }
+      if (rowok) { prx = ldo(Exo, cn + syb); prz = ldo(Ezo, cn + syb); }
+      if (colok) { pcx = ldo(Exo, cn + W); pcy = ldo(Eyo, cn + W); }
     }
+    if (jdom && istart >= 1 && istart < nx - 1) {
+      nqx = ldo(a.j[0], c); nqy = ldo(a.j[1], c); nqz = ldo(a.j[2], c);
+      nqw = ldo(a.wpe2, c); nqn = ldo(a.nue, c);
+    }
+    nt0 = inc[istart]; nt1 = inc[nx + istart]; nt2 = inc[2 * nx + istart]; nt3 = inc[3 * nx + istart];
   }
   T exprev = 0, hyprev = 0, hzprev = 0;
   int buf = 0;
 
-  for (int i = istart; i < ie; ++i, buf ^= 1) {
+  for (int i = istart; i < ie; ++i, buf = (buf == 2 ? 0 : buf + 1)) {
     const uint32_t c = (uint32_t)i * sxb + ob;
-    // ---- phase A: stage plane i of E^n; issue the loads of the next planes
-    sE[buf][0][r][lane] = e0x;
-    sE[buf][1][r][lane] = e0y;
-    sE[buf][2][r][lane] = e0z;
-    if (xrow) { sE[buf][0][R][lane] = rx; sE[buf][2][R][lane] = rz; }
-    if (xcol) { sE[buf][0][r][kLanes] = cx; sE[buf][1][r][kLanes] = cy; }
-    T e2x = 0, e2y = 0, e2z = 0, nrx = 0, nrz = 0, ncx = 0, ncy = 0;
-    T h1x = 0, h1y = 0, h1z = 0, q1x = 0, q1y = 0, q1z = 0, w1 = 0, n1 = 0;
+    const int nb = buf == 2 ? 0 : buf + 1;
+    // ---- phase A: take the previous batch, stage E^n(i+1), issue the next batch
+    const T hx0 = nhx, hy0 = nhy, hz0 = nhz;
+    const T qx = nqx, qy = nqy, qz = nqz, qw = nqw, qn = nqn;
+    const T tf0 = nt0, tf1 = nt1, tf2 = nt2, tf3 = nt3;
+    if (i + 1 < nx) {
+      sE[nb][0][r][lane] = px; sE[nb][1][r][lane] = py; sE[nb][2][r][lane] = pz;
+      if (xrow) { sE[nb][0][R][lane] = prx; sE[nb][2][R][lane] = prz; }
+      if (xcol) { sE[nb][0][r][kLanes] = pcx; sE[nb][1][r][kLanes] = pcy; }
+    }
     if (i + 1 < ie) {
       const uint32_t cn = c + sxb;
-      if (valid) {
-        h1x = ldo(Hxo, cn); h1y = ldo(Hyo, cn); h1z = ldo(Hzo, cn);
-        if (i + 2 < nx) { e2x = ldo(Exo, cn + sxb); e2y = ldo(Eyo, cn + sxb); e2z = ldo(Ezo, cn + sxb); }
+      if (i + 2 < nx) {
+        const uint32_t c2 = cn + sxb;
+        if (valid) { px = ldo(Exo, c2); py = ldo(Eyo, c2); pz = ldo(Ezo, c2); }
+        if (rowok) { prx = ldo(Exo, c2 + syb); prz = ldo(Ezo, c2 + syb); }
+        if (colok) { pcx = ldo(Exo, c2 + W); pcy = ldo(Eyo, c2 + W); }
       }
-      if (rowok) { nrx = ldo(Exo, cn + syb); nrz = ldo(Ezo, cn + syb); }
-      if (colok) { ncx = ldo(Exo, cn + W); ncy = ldo(Eyo, cn + W); }
-      if (jdom && i + 1 < nx - 1) {
-        q1x = ldo(a.j[0], cn); q1y = ldo(a.j[1], cn); q1z = ldo(a.j[2], cn);
-        w1 = ldo(a.wpe2, cn); n1 = ldo(a.nue, cn);
+      if (valid) { nhx = ldo(Hxo, cn); nhy = ldo(Hyo, cn); nhz = ldo(Hzo, cn); }
+      if (jdom && i + 1 >= 1 && i + 1 < nx - 1) {
+        nqx = ldo(a.j[0], cn); nqy = ldo(a.j[1], cn); nqz = ldo(a.j[2], cn);
+        nqw = ldo(a.wpe2, cn); nqn = ldo(a.nue, cn);
       }
+      nt0 = inc[i + 1]; nt1 = inc[nx + i + 1]; nt2 = inc[2 * nx + i + 1]; nt3 = inc[3 * nx + i + 1];
     }
-    const T ecx = e0x, ecy = e0y, ecz = e0z;
-    const T enx = e1x, eny = e1y, enz = e1z;
-    T hx = h0x, hy = h0y, hz = h0z;
-    (void)enx;
-    __syncthreads();
+    lds_barrier();
 
     // ---- phase B: H^(n+1/2) at plane i (owned cells and the low halo)
     const bool wr = own && i >= ib;
+    T hx = hx0, hy = hy0, hz = hz0;
+    const T ecx = sE[buf][0][r][lane], ecy = sE[buf][1][r][lane], ecz = sE[buf][2][r][lane];
     if (valid) {
       const T ezjp = sE[buf][2][r + 1][lane];
       const T eykp = sE[buf][1][r][lane + 1];
       const T exkp = sE[buf][0][r][lane + 1];
       const T exjp = sE[buf][0][r + 1][lane];
+      const T eny = sE[nb][1][r][lane], enz = sE[nb][2][r][lane];
       // Hx: i in [0,nx), j in [0,ny-1), k in [0,nz-1)   (:797-806)
       if (j < ny - 1 && k < nz - 1) {
         const T dEz = (ezjp - ecz) * inv;
@@ -260,12 +294,12 @@ fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
         hx = hx - c2h * (dEz - dEy);
         if (i >= a.bx1 && i < a.bx2) {  // TF/SF (:808-822)
           if (k >= a.bz1 && k < a.bz2) {
-            if (j == a.by1 - 1) hx = hx + inc[i];
-            if (j == a.by2 - 1) hx = hx - inc[i];
+            if (j == a.by1 - 1) hx = hx + tf0;
+            if (j == a.by2 - 1) hx = hx - tf0;
           }
           if (j >= a.by1 && j < a.by2) {
-            if (k == a.bz1 - 1) hx = hx - inc[nx + i];
-            if (k == a.bz2 - 1) hx = hx + inc[nx + i];
+            if (k == a.bz1 - 1) hx = hx - tf1;
+            if (k == a.bz2 - 1) hx = hx + tf1;
           }
         }
         if (PML) {  // (:690-714, :824-850)
@@ -292,8 +326,8 @@ fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
         const T dEz = (enz - ecz) * inv;
         hy = hy - c2h * (dEx - dEz);
         if (j >= a.by1 && j < a.by2 && k >= a.bz1 && k < a.bz2) {  // (:868-875)
-          if (i == a.bx1 - 1) hy = hy - inc[a.bx1];
-          if (i == a.bx2 - 1) hy = hy + inc[a.bx2 - 1];
+          if (i == a.bx1 - 1) hy = hy - tfEzB1;
+          if (i == a.bx2 - 1) hy = hy + tfEzB2;
         }
         if (PML) {  // (:718-744, :877-910)
           const int ii = slab_h(i, nx, L);
@@ -319,8 +353,8 @@ fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
         const T dEx = (exjp - ecx) * inv;
         hz = hz - c2h * (dEy - dEx);
         if (j >= a.by1 && j < a.by2 && k >= a.bz1 && k < a.bz2) {  // (:927-934)
-          if (i == a.bx1 - 1) hz = hz + inc[nx + a.bx1];
-          if (i == a.bx2 - 1) hz = hz - inc[nx + a.bx2 - 1];
+          if (i == a.bx1 - 1) hz = hz + tfEyB1;
+          if (i == a.bx2 - 1) hz = hz - tfEyB2;
         }
         if (PML) {  // (:747-791, :936-968)
           const int ii = slab_h(i, nx, L);
@@ -344,25 +378,23 @@ fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
     sH[0][r][lane] = hx;
     sH[1][r][lane] = hy;
     sH[2][r][lane] = hz;
-    __syncthreads();
+    lds_barrier();
 
     // ---- phase C: J^n (deferred) and E^(n+1) on owned cells
     if (wr) {
       // J^n from J^(n-1) and E^n; domain [1,N-1)^3   (:1523-1597)
       T jx = 0, jy = 0, jz = 0;
       if (jdom && i >= 1 && i < nx - 1) {
-        jx = q0x;
-        jy = q0y;
-        jz = q0z;
-        const T w = w0;
-        const T nu = n0;
+        jx = qx;
+        jy = qy;
+        jz = qz;
         if constexpr (CUR == kCurDI) {
-          current_update<T, CUR>(a, nu, w, jx, jy, jz, ecx, ecy, ecz);
+          current_update<T, CUR>(a, qn, qw, jx, jy, jz, ecx, ecy, ecz);
         } else {
           const T mx = (ecx + exprev) / 2;
           const T my = (ecy + sE[buf][1][r - 1][lane]) / 2;
           const T mz = (ecz + sE[buf][2][r][lane - 1]) / 2;
-          current_update<T, CUR>(a, nu, w, jx, jy, jz, mx, my, mz);
+          current_update<T, CUR>(a, qn, qw, jx, jy, jz, mx, my, mz);
         }
         sto(a.j[0], c, jx);
         sto(a.j[1], c, jy);
@@ -377,12 +409,12 @@ fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
         T ex = c1 * ecx + c2 * (dHz - dHy) - c2 * jx;
         if (i >= a.bx1 && i < a.bx2 - 1) {
           if (j >= a.by1 && j < a.by2) {
-            if (k == a.bz1) ex = ex + inc[2 * nx + i];
-            if (k == a.bz2) ex = ex - inc[2 * nx + i];
+            if (k == a.bz1) ex = ex + tf2;
+            if (k == a.bz2) ex = ex - tf2;
           }
           if (k >= a.bz1 && k < a.bz2) {
-            if (j == a.by1) ex = ex - inc[3 * nx + i];
-            if (j == a.by2) ex = ex + inc[3 * nx + i];
+            if (j == a.by1) ex = ex - tf3;
+            if (j == a.by2) ex = ex + tf3;
           }
         }
         if (PML) {
@@ -410,8 +442,8 @@ fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
         const T dHz = (hz - hzprev) * inv;
         ey = c1 * ecy + c2 * (dHx - dHz) - c2 * jy;
         if (j >= a.by1 && j < a.by2 && k >= a.bz1 && k < a.bz2) {
-          if (i == a.bx1) ey = ey + inc[3 * nx + a.bx1 - 1];
-          if (i == a.bx2 - 1) ey = ey - inc[3 * nx + a.bx2 - 1];
+          if (i == a.bx1) ey = ey + tfHzB1;
+          if (i == a.bx2 - 1) ey = ey - tfHzB2;
         }
         if (PML) {
           const int ii = slab_e(i, nx, L);
@@ -440,8 +472,8 @@ fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
         const T dHx = (hx - sH[0][r - 1][lane]) * inv;
         ez = c1 * ecz + c2 * (dHy - dHx) - c2 * jz;
         if (j >= a.by1 && j < a.by2 && k >= a.bz1 && k < a.bz2) {
-          if (i == a.bx1) ez = ez - inc[2 * nx + a.bx1 - 1];
-          if (i == a.bx2 - 1) ez = ez + inc[2 * nx + a.bx2 - 1];
+          if (i == a.bx1) ez = ez - tfHyB1;
+          if (i == a.bx2 - 1) ez = ez + tfHyB2;
         }
         if (a.doplate == 1 && i == a.ssx && j >= a.plate_lo && j < a.plate_up && k >= a.plate_lo && k < a.plate_up - 1)
           ez = 0;  // (:1344-1350)
@@ -474,15 +506,9 @@ fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
         if (i == a.ssx) { s[(size_t)j * nz + k] = ey; s[(size_t)ny * nz + (size_t)j * nz + k] = ezpre; }
       }
     }
-    // ---- rotate the x-direction pipeline
     exprev = ecx;
     hyprev = hy;
     hzprev = hz;
-    e0x = e1x; e0y = e1y; e0z = e1z;
-    e1x = e2x; e1y = e2y; e1z = e2z;
-    rx = nrx; rz = nrz; cx = ncx; cy = ncy;
-    h0x = h1x; h0y = h1y; h0z = h1z;
-    q0x = q1x; q0y = q1y; q0z = q1z; w0 = w1; n0 = n1;
   }
 }
 

@@ -285,7 +285,7 @@ struct SolverBase {
   HostPlan plan;
   hipStream_t stream = nullptr;
   int t_host = 0;
-  int ty = 7, chunk = 32;
+  int ty = 7, chunk = 16;
   int cur = 0;                    // kCurDI / kCurLKPoly / kCurLKClosed
   uint32_t* tiles = nullptr;      // interior tiles then boundary tiles
   int n_inner = 0, n_bound = 0;

@@ -129,6 +129,23 @@ def test_noncubic_grid_matches_oracle(msc, oracle_lib):
         assert not rep, "\n".join(rep[:12])
 
 
+def test_interior_tiles_with_cpml_match_oracle(msc, oracle_lib, monkeypatch):
+    """A grid large enough that tiles clear of every CPML slab (and of the
+    slab halos) run the interior variant while the rest run the CPML variant:
+    z = 160 gives a middle 63-cell z tile clear of both z slabs, 8-plane x
+    chunks and 7-row y tiles leave interior chunks and rows inside 60 x 48."""
+    Solver, _ = msc
+    monkeypatch.setenv("MSC_CHUNK", "8")
+    monkeypatch.setenv("MSC_TY", "7")
+    p, _, _ = cases.case("lk_ricker_ez")
+    p.update(nx=60, ny=48, nz=160, tsteps=100, nprobes=2, probex=[30, 20], probey=[24, 30], probez=[80, 120])
+    wp, nu = cases.trail_maps(p, n0=2e13, r0=1.5, nu0=1e6, centre=(0.5, -0.25))
+    for prec, tol in ((64, 1e-10), (32, 2e-5)):
+        g, o, rep = run_pair(Solver, oracle_lib, p, wp, nu, prec, p["tsteps"], 25, tol, f"interior{prec}")
+        assert not rep, "\n".join(rep[:12])
+        assert relerr(g.ntff(), o.ntff()) < tol
+
+
 def test_tilings_are_bit_identical(msc, oracle_lib, monkeypatch):
     """Every cell is computed by the same code whatever the tile shape or
     x-chunk length (halo cells are recomputed from read-only data)."""
