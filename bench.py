@@ -69,7 +69,7 @@ def main():
     ap.add_argument("--workload", default="config2")
     ap.add_argument("--precision", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the streaming-copy reference (counter calibration)")
     args = ap.parse_args()
