@@ -52,8 +52,10 @@ def build(verbose: bool = False) -> str:
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
     os.replace(tmp, lib)
     exe = os.path.join(PKG, "meteor")
+    # $ORIGIN first, then the build directory itself: setup/meteorrun.m:162
+    # copies the executable alone into each run directory
     _run(["g++", *CXX_FLAGS, os.path.join(SRC, "meteor_main.cpp"), "-o", exe, "-L" + PKG, "-lmsc",
-          "-Wl,-rpath,$ORIGIN"], verbose)
+          "-Wl,-rpath,$ORIGIN:" + os.path.realpath(PKG)], verbose)
     return lib
 
 

@@ -166,3 +166,17 @@ def test_ntf_file_roundtrip():
     assert d["ntfb"] == 15 and list(d["fntf"]) == p["fntf"]
     np.testing.assert_array_equal(d["Eydft"], pay[:2 * M])
     np.testing.assert_array_equal(d["My_front"].ravel(), pay[4 * M:4 * M + 2 * M * ny * nz])
+
+
+def test_copied_executable_still_finds_the_library(tmp_path):
+    """setup/meteorrun.m:162 copies the executable alone into the run
+    directory; the dynamic loader must still resolve libmsc.so."""
+    exe = os.path.join(os.path.dirname(_abi.LIB_PATH), "meteor")
+    if not os.path.exists(exe):
+        pytest.fail("meteor not built: run __graft_entry__.build()")
+    dst = tmp_path / "meteor"
+    dst.write_bytes(open(exe, "rb").read())
+    dst.chmod(0o755)
+    out = subprocess.run(["ldd", str(dst)], capture_output=True, text=True).stdout
+    line = [ln for ln in out.splitlines() if "libmsc.so" in ln]
+    assert line and "not found" not in line[0], out
