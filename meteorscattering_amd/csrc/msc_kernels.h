@@ -11,6 +11,7 @@ namespace msc {
 constexpr int kLanes = 64;
 constexpr int kOwnZ = kLanes - 1;
 constexpr int kMaxLkDeg = 24;
+constexpr int kNtffBlock = 256;  // surface points per NTFF workgroup
 
 // current models selected per launch
 enum { kCurDI = 0, kCurLKPoly = 1, kCurLKClosed = 2 };
@@ -21,10 +22,11 @@ __host__ __device__ inline uint32_t pack_tile(int tz, int ty, int ch) {
   return (uint32_t)tz | ((uint32_t)ty << 8) | ((uint32_t)ch << 20);
 }
 
-// Everything one fused time step needs.  Passed by value as the kernel
-// argument; it does not change from step to step (the step index lives in
-// device memory at `tdev`, and the ping-pong buffers are selected by its
-// parity), so the per-step launch sequence can be captured in a hipGraph.
+// Everything one fused time step of one solve needs.  Kept in device memory
+// (one element of an array indexed by solve id); it does not change from
+// step to step (the step index lives in device memory at `tdev`, and the
+// ping-pong buffers are selected by its parity), so a launch sequence can be
+// replayed or captured in a hipGraph unchanged.
 template <typename T>
 struct StepArgs {
   int nx, ny, nz, L;
@@ -93,5 +95,18 @@ struct AuxArgs {
   unsigned long long* emax; // [nframes] bit patterns of max|Ex| (as double)
   int* tdev;
 };
+
+template <typename T>
+hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int cur, bool pml, bool slice,
+                       const uint32_t* tiles, int n, hipStream_t s);
+template <typename T>
+hipError_t launch_ntff(const NtffArgs<T>* a, const uint2* blocks, int nblocks, hipStream_t s);
+template <typename T>
+hipError_t launch_frames(const AuxArgs<T>* a, hipStream_t s);
+template <typename T>
+hipError_t launch_probes(const AuxArgs<T>* a, int nsolves, hipStream_t s);
+template <typename T>
+hipError_t launch_catchup(const StepArgs<T>* a, int cur, T* ox, T* oy, T* oz, hipStream_t s);
+hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 }  // namespace msc

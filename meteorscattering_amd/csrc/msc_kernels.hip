@@ -17,11 +17,26 @@
 // E and H are ping-pong buffered: each workgroup recomputes the H values of
 // its one-cell low halo (row j0-1, column k0-1) from read-only step-n data,
 // which removes any ordering between workgroups.
+//
+// Every kernel reads its arguments from device memory (an array of per-solve
+// argument blocks indexed by a per-workgroup solve id), so one launch can
+// advance the tiles of many independent solves together.
 #include <hip/hip_runtime.h>
 
 #include "msc_kernels.h"
 
 namespace msc {
+
+// Argument blocks are read through the constant address space: the compiler
+// then treats them as invariant for the whole kernel and keeps them in
+// scalar registers / reloads them with scalar loads, as it does for kernel
+// arguments, even across the kernel's own global stores.
+template <typename A>
+using ConstArgs = __attribute__((address_space(4))) const A;
+template <typename A>
+__device__ __forceinline__ ConstArgs<A>& as_const_space(const A* p) {
+  return *(ConstArgs<A>*)p;
+}
 
 // Spread consecutive logical tiles over one XCD: hardware dispatch sends
 // workgroup b to XCD b % 8, so logical tile ids are laid out XCD-major and
@@ -58,8 +73,8 @@ __device__ __forceinline__ int slab_e(int p, int n, int L) {
 //                        FP64 (finite limits at nu == 0 and |B| == 0)
 // J^(n+1) = e^{-x} R J^n + w (kI E + kX (w_b x E) + kP w_b (w_b . E)), w = eps0 wp^2,
 // with E the node-averaged field for Lee & Kalluri and the collocated one for DI.
-template <typename T, int CUR>
-__device__ __forceinline__ void current_update(const StepArgs<T>& a, T nu_t, T w_t, T& jx, T& jy, T& jz,
+template <typename T, int CUR, typename A>
+__device__ __forceinline__ void current_update(const A& a, T nu_t, T w_t, T& jx, T& jy, T& jz,
                                                T ex, T ey, T ez) {
   const double nu = nu_t, w = w_t;
   if constexpr (CUR == kCurDI) {
@@ -141,8 +156,8 @@ __device__ __forceinline__ void lds_barrier() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-template <typename T>
-__device__ __forceinline__ bool in_sphere(const StepArgs<T>& a, int i, int j, int k) {
+template <typename A>
+__device__ __forceinline__ bool in_sphere(const A& a, int i, int j, int k) {
   // meteor-fork.cpp:203-209, coordinates as :137-141 (one loop per axis)
   const double x = __dadd_rn(a.x0, __dmul_rn(a.dd, (double)i));
   const double y = __dadd_rn(a.y0, __dmul_rn(a.dd, (double)j));
@@ -152,7 +167,8 @@ __device__ __forceinline__ bool in_sphere(const StepArgs<T>& a, int i, int j, in
 
 template <typename T, int TY, int CUR, bool PML, bool SLICE>
 __global__ void __launch_bounds__((TY + 1) * kLanes)
-fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
+fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ tiles,
+          const uint16_t* __restrict__ sids) {
   constexpr int R = TY + 1;  // rows including the j0-1 halo row
   constexpr uint32_t W = sizeof(T);
   // sE[b]: one x-plane of E^n with a +1 row and +1 column halo.  Three
@@ -164,7 +180,9 @@ fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
 
   const int lane = threadIdx.x & (kLanes - 1);
   const int r = threadIdx.x / kLanes;
-  const uint32_t tile = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t tile = tiles[blk];
+  const ConstArgs<StepArgs<T>>& a = as_const_space(args + (sids ? sids[blk] : 0));
   const int tz = tile & 0xff;
   const int ty = (tile >> 8) & 0xfff;
   const int chunk = tile >> 20;
@@ -239,10 +257,8 @@ fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
       if (rowok) { prx = ldo(Exo, cn + syb); prz = ldo(Ezo, cn + syb); }
       if (colok) { pcx = ldo(Exo, cn + W); pcy = ldo(Eyo, cn + W); }
     }
-    if (jdom && istart >= 1 && istart < nx - 1) {
-      nqx = ldo(a.j[0], c); nqy = ldo(a.j[1], c); nqz = ldo(a.j[2], c);
-      nqw = ldo(a.wpe2, c); nqn = ldo(a.nue, c);
-    }
+    // (no J or media for plane istart: it is either the warm-up plane ib-1,
+    // which only recomputes H, or plane 0, outside the current's domain)
     nt0 = inc[istart]; nt1 = inc[nx + istart]; nt2 = inc[2 * nx + istart]; nt3 = inc[3 * nx + istart];
   }
   T exprev = 0, hyprev = 0, hzprev = 0;
@@ -515,7 +531,8 @@ fdtd_step(StepArgs<T> a, const uint32_t* __restrict__ tiles) {
 // J^(t+1) from J^t and E^(t+1) into `out` (what the reference holds after
 // the same number of steps); used only to read J back.
 template <typename T, int CUR>
-__global__ void current_catchup(StepArgs<T> a, T* ox, T* oy, T* oz) {
+__global__ void current_catchup(const StepArgs<T>* __restrict__ ap, T* ox, T* oy, T* oz) {
+  const ConstArgs<StepArgs<T>>& a = as_const_space(ap);
   const long long n = (long long)a.nx * a.ny * a.nz;
   const int t = *a.tdev;
   const T* Ex = a.e[t & 1][0];
@@ -543,15 +560,19 @@ __global__ void current_catchup(StepArgs<T> a, T* ox, T* oy, T* oz) {
 // Running DFT of the six NTFF planes (meteor-fork.cpp:971-1023 for the
 // face-averaged H, :1417-1470 for E), FP64 accumulators, both at phase t*dt.
 template <typename T>
-__global__ void __launch_bounds__(256) ntff_accumulate(NtffArgs<T> a) {
-  const int face = blockIdx.y;
+__global__ void __launch_bounds__(256) ntff_accumulate(const NtffArgs<T>* __restrict__ args,
+                                                        const uint2* __restrict__ blocks) {
+  // blocks[b] = {solve id, face << 29 | block index within the face}
+  const uint2 d = blocks[blockIdx.x];
+  const ConstArgs<NtffArgs<T>>& a = as_const_space(args + d.x);
+  const int face = (int)(d.y >> 29);
   const int nx = a.nx, ny = a.ny, nz = a.nz, b = a.b;
   int n2;
   long long npts;
   if (face < 2) { n2 = nz; npts = (long long)ny * nz; }
   else if (face < 4) { n2 = nz; npts = (long long)nx * nz; }
   else { n2 = ny; npts = (long long)nx * ny; }
-  const long long pt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long pt = (long long)(d.y & 0x1fffffffu) * blockDim.x + threadIdx.x;
   if (pt >= npts) return;
   const int u = (int)(pt / n2), v = (int)(pt % n2);
   const int t = *a.tdev;
@@ -618,7 +639,8 @@ __global__ void __launch_bounds__(256) ntff_accumulate(NtffArgs<T> a) {
 // Output cadence (t % cadence == 0): H slices (meteor-fork.cpp:1025-1045),
 // staged E slices, and max|Ex| (:1621-1630) into the frame store.
 template <typename T>
-__global__ void __launch_bounds__(256) cadence_frames(AuxArgs<T> a) {
+__global__ void __launch_bounds__(256) cadence_frames(const AuxArgs<T>* __restrict__ ap) {
+  const ConstArgs<AuxArgs<T>>& a = as_const_space(ap);
   const int t = *a.tdev;
   if (t % a.cadence) return;
   const int f = t / a.cadence;
@@ -677,9 +699,10 @@ __global__ void __launch_bounds__(256) cadence_frames(AuxArgs<T> a) {
   }
 }
 
-// Probes (meteor-fork.cpp:1474-1486) and the step counter.
+// Probes (meteor-fork.cpp:1474-1486) and the step counter; one workgroup per solve.
 template <typename T>
-__global__ void __launch_bounds__(64) probes_and_tick(AuxArgs<T> a) {
+__global__ void __launch_bounds__(64) probes_and_tick(const AuxArgs<T>* __restrict__ args) {
+  const ConstArgs<AuxArgs<T>>& a = as_const_space(args + blockIdx.x);
   const int t = *a.tdev;
   const int par = (t & 1) ^ 1;
   const T* Ex = a.e[par][0];
@@ -719,61 +742,67 @@ hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStrea
 // ---------------------------------------------------------------- launchers
 
 template <typename T, int TY, int CUR, bool PML, bool SLICE>
-static hipError_t launch_v(const StepArgs<T>& a, const uint32_t* tiles, int n, hipStream_t s) {
-  hipLaunchKernelGGL((fdtd_step<T, TY, CUR, PML, SLICE>), dim3(n), dim3((TY + 1) * kLanes), 0, s, a, tiles);
+static hipError_t launch_v(const StepArgs<T>* a, const uint16_t* sids, const uint32_t* tiles, int n, hipStream_t s) {
+  hipLaunchKernelGGL((fdtd_step<T, TY, CUR, PML, SLICE>), dim3(n), dim3((TY + 1) * kLanes), 0, s, a, tiles, sids);
   return hipGetLastError();
 }
 
 template <typename T, int TY, int CUR>
-static hipError_t launch_ps(const StepArgs<T>& a, bool pml, bool slice, const uint32_t* tiles, int n,
-                            hipStream_t s) {
-  if (pml) return slice ? launch_v<T, TY, CUR, true, true>(a, tiles, n, s) : launch_v<T, TY, CUR, true, false>(a, tiles, n, s);
-  return slice ? launch_v<T, TY, CUR, false, true>(a, tiles, n, s) : launch_v<T, TY, CUR, false, false>(a, tiles, n, s);
+static hipError_t launch_ps(const StepArgs<T>* a, const uint16_t* sids, bool pml, bool slice, const uint32_t* tiles,
+                            int n, hipStream_t s) {
+  if (pml)
+    return slice ? launch_v<T, TY, CUR, true, true>(a, sids, tiles, n, s) : launch_v<T, TY, CUR, true, false>(a, sids, tiles, n, s);
+  return slice ? launch_v<T, TY, CUR, false, true>(a, sids, tiles, n, s) : launch_v<T, TY, CUR, false, false>(a, sids, tiles, n, s);
 }
 
 template <typename T, int TY>
-static hipError_t launch_c(const StepArgs<T>& a, int cur, bool pml, bool slice, const uint32_t* tiles, int n,
-                           hipStream_t s) {
+static hipError_t launch_c(const StepArgs<T>* a, const uint16_t* sids, int cur, bool pml, bool slice,
+                           const uint32_t* tiles, int n, hipStream_t s) {
   switch (cur) {
-    case kCurDI: return launch_ps<T, TY, kCurDI>(a, pml, slice, tiles, n, s);
-    case kCurLKPoly: return launch_ps<T, TY, kCurLKPoly>(a, pml, slice, tiles, n, s);
-    case kCurLKClosed: return launch_ps<T, TY, kCurLKClosed>(a, pml, slice, tiles, n, s);
+    case kCurDI: return launch_ps<T, TY, kCurDI>(a, sids, pml, slice, tiles, n, s);
+    case kCurLKPoly: return launch_ps<T, TY, kCurLKPoly>(a, sids, pml, slice, tiles, n, s);
+    case kCurLKClosed: return launch_ps<T, TY, kCurLKClosed>(a, sids, pml, slice, tiles, n, s);
   }
   return hipErrorInvalidValue;
 }
 
-// One launch of the fused step over `n` tiles of the list `tiles`; `pml`
-// selects the variant carrying the CPML code (tiles that touch a slab),
-// `slice` the one capturing E slices (output-cadence steps).
+// One launch of the fused step over `n` tiles of the list `tiles`, tile b
+// belonging to solve sids[b] of the argument array `a` (sids == nullptr:
+// every tile is solve 0); `pml` selects the variant carrying the CPML code
+// (tiles that touch a slab), `slice` the one capturing E slices.
 template <typename T>
-hipError_t launch_step(const StepArgs<T>& a, int ty, int cur, bool pml, bool slice, const uint32_t* tiles, int n,
-                       hipStream_t s) {
+hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int cur, bool pml, bool slice,
+                       const uint32_t* tiles, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   switch (ty) {
-    case 3: return launch_c<T, 3>(a, cur, pml, slice, tiles, n, s);
-    case 7: return launch_c<T, 7>(a, cur, pml, slice, tiles, n, s);
+    case 3: return launch_c<T, 3>(a, sids, cur, pml, slice, tiles, n, s);
+    case 7: return launch_c<T, 7>(a, sids, cur, pml, slice, tiles, n, s);
   }
   return hipErrorInvalidValue;
 }
 
 template <typename T>
-hipError_t launch_ntff(const NtffArgs<T>& a, hipStream_t s) {
-  long long mx = (long long)a.ny * a.nz;
-  mx = mx > (long long)a.nx * a.nz ? mx : (long long)a.nx * a.nz;
-  mx = mx > (long long)a.nx * a.ny ? mx : (long long)a.nx * a.ny;
-  hipLaunchKernelGGL((ntff_accumulate<T>), dim3((unsigned)((mx + 255) / 256), 6), dim3(256), 0, s, a);
+hipError_t launch_ntff(const NtffArgs<T>* a, const uint2* blocks, int nblocks, hipStream_t s) {
+  if (nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL((ntff_accumulate<T>), dim3(nblocks), dim3(kNtffBlock), 0, s, a, blocks);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_aux(const AuxArgs<T>& a, bool frames, hipStream_t s) {
-  if (frames) hipLaunchKernelGGL((cadence_frames<T>), dim3(1024), dim3(256), 0, s, a);
-  hipLaunchKernelGGL((probes_and_tick<T>), dim3(1), dim3(64), 0, s, a);
+hipError_t launch_frames(const AuxArgs<T>* a, hipStream_t s) {
+  hipLaunchKernelGGL((cadence_frames<T>), dim3(1024), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_catchup(const StepArgs<T>& a, int cur, T* ox, T* oy, T* oz, hipStream_t s) {
+hipError_t launch_probes(const AuxArgs<T>* a, int nsolves, hipStream_t s) {
+  if (nsolves <= 0) return hipSuccess;
+  hipLaunchKernelGGL((probes_and_tick<T>), dim3(nsolves), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_catchup(const StepArgs<T>* a, int cur, T* ox, T* oy, T* oz, hipStream_t s) {
   switch (cur) {
     case kCurDI: hipLaunchKernelGGL((current_catchup<T, kCurDI>), dim3(2048), dim3(256), 0, s, a, ox, oy, oz); break;
     case kCurLKPoly: hipLaunchKernelGGL((current_catchup<T, kCurLKPoly>), dim3(2048), dim3(256), 0, s, a, ox, oy, oz); break;
@@ -782,13 +811,15 @@ hipError_t launch_catchup(const StepArgs<T>& a, int cur, T* ox, T* oy, T* oz, hi
   return hipGetLastError();
 }
 
-template hipError_t launch_step<float>(const StepArgs<float>&, int, int, bool, bool, const uint32_t*, int, hipStream_t);
-template hipError_t launch_step<double>(const StepArgs<double>&, int, int, bool, bool, const uint32_t*, int, hipStream_t);
-template hipError_t launch_ntff<float>(const NtffArgs<float>&, hipStream_t);
-template hipError_t launch_ntff<double>(const NtffArgs<double>&, hipStream_t);
-template hipError_t launch_aux<float>(const AuxArgs<float>&, bool, hipStream_t);
-template hipError_t launch_aux<double>(const AuxArgs<double>&, bool, hipStream_t);
-template hipError_t launch_catchup<float>(const StepArgs<float>&, int, float*, float*, float*, hipStream_t);
-template hipError_t launch_catchup<double>(const StepArgs<double>&, int, double*, double*, double*, hipStream_t);
+#define MSC_INST(T)                                                                                                \
+  template hipError_t launch_step<T>(const StepArgs<T>*, const uint16_t*, int, int, bool, bool, const uint32_t*, int, \
+                                     hipStream_t);                                                                 \
+  template hipError_t launch_ntff<T>(const NtffArgs<T>*, const uint2*, int, hipStream_t);                          \
+  template hipError_t launch_frames<T>(const AuxArgs<T>*, hipStream_t);                                            \
+  template hipError_t launch_probes<T>(const AuxArgs<T>*, int, hipStream_t);                                       \
+  template hipError_t launch_catchup<T>(const StepArgs<T>*, int, T*, T*, T*, hipStream_t);
+MSC_INST(float)
+MSC_INST(double)
+#undef MSC_INST
 
 }  // namespace msc

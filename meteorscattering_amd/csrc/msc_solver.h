@@ -1,0 +1,136 @@
+// Host plan and per-solve device state, shared by the single-solve handle
+// (msc_solver.hip) and the multi-solve server (msc_server.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "msc_common.h"
+#include "msc_kernels.h"
+
+namespace msc {
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) return fail(MSC_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// FP64 host precompute of one solve (meteor-fork.cpp:119-646).
+struct HostPlan {
+  msc_params p;
+  int nx, ny, nz, L, M, T, cadence, nframes;
+  double dx, dt, x0, y0, z0;
+  double c1h, c2h, c1e0, c2e0, c1s, c2s;
+  double wce[3], w0, cth, sth, Se1, Ce1, R[9];
+  std::vector<double> be, ae, bh, ah;  // 2L
+  std::vector<double> srcA, srcB;      // T
+  std::vector<double> inc;             // [T][4][nx]
+  std::vector<double> eline, hline;    // [nframes][2][nx] incident lines at frame steps
+  std::vector<double> eydft, ezdft;    // 2M
+  std::vector<double> sincos;          // [T][M][2]
+  std::vector<double> zcoord;          // nz
+};
+
+int make_plan(const msc_params* p, HostPlan& h);
+
+// Device bytes of one solve (its single block), without allocating.
+size_t solve_footprint(const msc_params* p, int precision);
+
+// One device allocation carved into solve blocks: first fit over a free
+// list, 256-byte aligned, coalescing on release.  Thread-safe.
+class Arena {
+ public:
+  ~Arena();
+  int init(size_t bytes);
+  void* alloc(size_t bytes);  // nullptr when no free range fits
+  void release(void* ptr);
+  size_t capacity() const { return cap_; }
+  size_t used() const;
+  size_t largest_free() const;
+
+ private:
+  char* base_ = nullptr;
+  size_t cap_ = 0, used_ = 0;
+  std::map<size_t, size_t> free_;  // offset -> length
+  std::map<size_t, size_t> live_;  // offset -> length
+  mutable std::mutex mu_;
+};
+
+// Device state of one solve.  All of it lives in ONE block (hipMalloc, or a
+// range of a server's Arena), laid out array after array.
+struct SolverBase {
+  HostPlan plan;
+  hipStream_t stream = nullptr;
+  bool own_stream = true;
+  Arena* arena = nullptr;
+  char* blk = nullptr;
+  size_t dev_bytes = 0;
+  int t_host = 0;
+  int ty = 7, chunk = 16;
+  int cur = 0;                      // kCurDI / kCurLKPoly / kCurLKClosed
+  std::vector<uint32_t> tiles_host; // interior tiles, then CPML-boundary tiles
+  int n_inner = 0, n_bound = 0;
+  std::vector<uint2> ntff_host;     // NTFF workgroup descriptors (solve id 0)
+  bool timing = false;
+  std::vector<hipEvent_t> ev;       // pairs around the step kernels of the last step() call
+  double kernel_ms = 0;
+  virtual ~SolverBase();
+  // Plan the device block: builds the tile and NTFF lists and returns the
+  // block's size (no device call).
+  virtual size_t layout() = 0;
+  // Allocate (arena or hipMalloc), clear and fill the block on `stream`.
+  virtual int init(const double* wp, const double* nu) = 0;
+  virtual int step(int n) = 0;
+  virtual int get_field(int comp, double* out) = 0;
+  virtual int get_ntff(double* out) = 0;
+  virtual int get_probes(double* out) = 0;
+  virtual int get_frame(int f, double* e, double* h, double* emax) = 0;
+  int sync();
+  size_t frame_doubles() const {
+    return 2 * ((size_t)plan.nx * plan.ny + (size_t)plan.nx * plan.nz + (size_t)plan.ny * plan.nz);
+  }
+  size_t ntff_doubles() const;
+};
+
+template <typename T>
+struct Solver : SolverBase {
+  // host copies of the device-resident argument blocks
+  StepArgs<T> sa{};
+  NtffArgs<T> na{};
+  AuxArgs<T> xa{};
+  StepArgs<T>* sa_dev = nullptr;
+  NtffArgs<T>* na_dev = nullptr;
+  AuxArgs<T>* xa_dev = nullptr;
+  uint32_t* tiles = nullptr;
+  uint2* ntff_blocks = nullptr;
+  size_t acc_n = 0;
+  double* acc = nullptr;
+  double* probes = nullptr;
+  double* eframes = nullptr;
+  double* hframes = nullptr;
+  unsigned long long* emax = nullptr;
+  int* tdev = nullptr;
+
+  size_t layout() override;
+  int init(const double* wp, const double* nu) override;
+  int step(int n) override;
+  int get_field(int comp, double* out) override;
+  int get_ntff(double* out) override;
+  int get_probes(double* out) override;
+  int get_frame(int f, double* e, double* h, double* emax) override;
+
+ private:
+  // CPML-aligned tile starts per axis, computed by layout()
+  std::vector<int> xs_, ys_, zs_;
+  size_t carve(char* base);
+};
+
+extern template struct Solver<float>;
+extern template struct Solver<double>;
+
+}  // namespace msc

@@ -1,7 +1,7 @@
 // Solver handle: FP64 host precompute, device state, step sequencing and
 // read-back.  The set-up mirrors meteor-fork.cpp:119-646; the per-step work
 // is the kernel sequence of msc_kernels.hip.
-#include <hip/hip_runtime.h>
+#include "msc_solver.h"
 
 #include <algorithm>
 #include <cmath>
@@ -9,46 +9,13 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
-
-#include "msc_common.h"
-#include "msc_kernels.h"
 
 namespace msc {
 
-template <typename T>
-hipError_t launch_step(const StepArgs<T>& a, int ty, int cur, bool pml, bool slice, const uint32_t* tiles, int n,
-                       hipStream_t s);
-template <typename T>
-hipError_t launch_ntff(const NtffArgs<T>& a, hipStream_t s);
-template <typename T>
-hipError_t launch_aux(const AuxArgs<T>& a, bool frames, hipStream_t s);
-template <typename T>
-hipError_t launch_catchup(const StepArgs<T>& a, int cur, T* ox, T* oy, T* oz, hipStream_t s);
-hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
-
-#define HIPCHK(expr)                                                                  \
-  do {                                                                                \
-    hipError_t e_ = (expr);                                                           \
-    if (e_ != hipSuccess) return fail(MSC_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
-
 // ------------------------------------------------------------ host precompute
 
-struct HostPlan {
-  msc_params p;
-  int nx, ny, nz, L, M, T, cadence, nframes;
-  double dx, dt, x0, y0, z0;
-  double c1h, c2h, c1e0, c2e0, c1s, c2s;
-  double wce[3], w0, cth, sth, Se1, Ce1, R[9];
-  std::vector<double> be, ae, bh, ah;  // 2L
-  std::vector<double> srcA, srcB;      // T
-  std::vector<double> inc;             // [T][4][nx]
-  std::vector<double> eline, hline;    // [nframes][2][nx] incident lines at frame steps
-  std::vector<double> eydft, ezdft;    // 2M
-  std::vector<double> sincos;          // [T][M][2]
-  std::vector<double> zcoord;          // nz
-};
 
 // Source waveforms, meteor-fork.cpp:153-179 (+ extension type 4).
 static void plan_sources(HostPlan& h) {
@@ -227,7 +194,7 @@ static bool plan_lk_poly(const HostPlan& h, double xmax, int& deg, double c[4][k
   return true;
 }
 
-static int make_plan(const msc_params* p, HostPlan& h) {
+int make_plan(const msc_params* p, HostPlan& h) {
   int rc = msc_params_validate(p);
   if (rc) return rc;
   h.p = *p;
@@ -279,389 +246,494 @@ static int make_plan(const msc_params* p, HostPlan& h) {
   return MSC_OK;
 }
 
+// ---------------------------------------------------------------- arena
+
+Arena::~Arena() {
+  if (base_) hipFree(base_);
+}
+
+int Arena::init(size_t bytes) {
+  bytes &= ~(size_t)255;
+  if (bytes == 0) return fail(MSC_EINVAL, "arena of 0 bytes");
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return fail(MSC_ENOMEM, "hipMalloc of a %zu-byte arena: %s", bytes, hipGetErrorString(e));
+  base_ = (char*)q;
+  cap_ = bytes;
+  free_[0] = bytes;
+  return MSC_OK;
+}
+
+void* Arena::alloc(size_t bytes) {
+  std::lock_guard<std::mutex> g(mu_);
+  bytes = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+  for (auto it = free_.begin(); it != free_.end(); ++it) {
+    if (it->second < bytes) continue;
+    const size_t off = it->first, len = it->second;
+    free_.erase(it);
+    if (len > bytes) free_[off + bytes] = len - bytes;
+    live_[off] = bytes;
+    used_ += bytes;
+    return base_ + off;
+  }
+  return nullptr;
+}
+
+void Arena::release(void* ptr) {
+  if (!ptr) return;
+  std::lock_guard<std::mutex> g(mu_);
+  const size_t off = (size_t)((char*)ptr - base_);
+  auto lv = live_.find(off);
+  if (lv == live_.end()) return;
+  size_t o = off, len = lv->second;
+  live_.erase(lv);
+  used_ -= len;
+  auto nx = free_.lower_bound(o);
+  if (nx != free_.end() && nx->first == o + len) {  // merge with the following range
+    len += nx->second;
+    nx = free_.erase(nx);
+  }
+  if (nx != free_.begin()) {  // and with the preceding one
+    auto pv = std::prev(nx);
+    if (pv->first + pv->second == o) {
+      o = pv->first;
+      len += pv->second;
+      free_.erase(pv);
+    }
+  }
+  free_[o] = len;
+}
+
+size_t Arena::used() const {
+  std::lock_guard<std::mutex> g(mu_);
+  return used_;
+}
+
+size_t Arena::largest_free() const {
+  std::lock_guard<std::mutex> g(mu_);
+  size_t m = 0;
+  for (const auto& f : free_) m = std::max(m, f.second);
+  return m;
+}
+
 // ---------------------------------------------------------------- solver
 
-struct SolverBase {
-  HostPlan plan;
-  hipStream_t stream = nullptr;
-  int t_host = 0;
-  int ty = 7, chunk = 16;
-  int cur = 0;                    // kCurDI / kCurLKPoly / kCurLKClosed
-  uint32_t* tiles = nullptr;      // interior tiles then boundary tiles
-  int n_inner = 0, n_bound = 0;
-  bool timing = false;
-  std::vector<hipEvent_t> ev;  // pairs around step kernels of the last batch
-  double kernel_ms = 0;
-  size_t dev_bytes = 0;
-  std::vector<void*> allocs;
-  virtual ~SolverBase() {
-    if (stream) hipStreamSynchronize(stream);
-    for (void* q : allocs) hipFree(q);
-    for (hipEvent_t e : ev) hipEventDestroy(e);
-    if (stream) hipStreamDestroy(stream);
+SolverBase::~SolverBase() {
+  if (stream) hipStreamSynchronize(stream);
+  if (blk) {
+    if (arena) arena->release(blk);
+    else hipFree(blk);
   }
-  template <typename U>
-  int alloc(U** out, size_t n) {
-    void* q = nullptr;
-    size_t bytes = std::max<size_t>(n, 1) * sizeof(U);
-    hipError_t e = hipMalloc(&q, bytes);
-    if (e != hipSuccess) return fail(MSC_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
-    allocs.push_back(q);
-    dev_bytes += bytes;
-    e = hipMemsetAsync(q, 0, bytes, stream);
-    if (e != hipSuccess) return fail(MSC_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
-    *out = (U*)q;
-    return MSC_OK;
-  }
-  virtual int init(const double* wp, const double* nu) = 0;
-  virtual int step(int n) = 0;
-  virtual int get_field(int comp, double* out) = 0;
-  virtual int get_ntff(double* out) = 0;
-  virtual int get_probes(double* out) = 0;
-  virtual int get_frame(int f, double* e, double* h, double* emax) = 0;
-  int sync() {
-    HIPCHK(hipStreamSynchronize(stream));
-    if (timing && !ev.empty()) {
-      double tot = 0;
-      for (size_t q = 0; q + 1 < ev.size(); q += 2) {
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
-        tot += ms;
-      }
-      kernel_ms = tot / (double)(ev.size() / 2);
+  for (hipEvent_t e : ev) hipEventDestroy(e);
+  if (stream && own_stream) hipStreamDestroy(stream);
+}
+
+int SolverBase::sync() {
+  HIPCHK(hipStreamSynchronize(stream));
+  if (timing && !ev.empty()) {
+    double tot = 0;
+    for (size_t q = 0; q + 1 < ev.size(); q += 2) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+      tot += ms;
     }
-    return MSC_OK;
+    kernel_ms = tot / (double)(ev.size() / 2);
   }
-  size_t frame_doubles() const {
-    return 2 * ((size_t)plan.nx * plan.ny + (size_t)plan.nx * plan.nz + (size_t)plan.ny * plan.nz);
+  return MSC_OK;
+}
+
+size_t SolverBase::ntff_doubles() const {
+  const size_t M2 = 2 * (size_t)plan.M;
+  return 2 * M2 + 8 * M2 * ((size_t)plan.ny * plan.nz + (size_t)plan.nx * plan.nz + (size_t)plan.nx * plan.ny);
+}
+
+// Tile lists and NTFF workgroup list, then the block layout.
+template <typename T>
+size_t Solver<T>::layout() {
+  const HostPlan& h = plan;
+  const int nx = h.nx, ny = h.ny, nz = h.nz, L = h.L;
+  if (const char* s = std::getenv("MSC_TY")) ty = std::atoi(s);
+  if (const char* s = std::getenv("MSC_CHUNK")) chunk = std::max(1, std::atoi(s));
+  // Tiles: x and y are cut at the CPML slab edges (L+1 and n-L-1), so a tile
+  // either meets a slab on some axis (the variant carrying the CPML code) or
+  // meets none (the interior variant); z keeps full 63-wide tiles from 0,
+  // the first and last of which hold the z slabs.  A tile's one-cell low
+  // halo (row/column/plane L when the tile starts at L+1) needs no CPML
+  // term: slab_h(L) is outside both H slabs, and no E is computed there.
+  auto cut = [&](int n, int piece, bool align) {
+    std::vector<int> st;
+    auto span = [&](int lo, int hi) {
+      for (int q = lo; q < hi; q += piece) st.push_back(q);
+    };
+    if (align && h.p.dopml) {
+      span(0, L + 1);
+      span(L + 1, n - L - 1);
+      span(n - L - 1, n);
+    } else {
+      span(0, n);
+    }
+    st.push_back(n);
+    return st;
+  };
+  xs_ = cut(nx, chunk, true);
+  ys_ = cut(ny, ty, true);
+  zs_ = cut(nz, kOwnZ, false);
+  const int nch = (int)xs_.size() - 1, tiles_y = (int)ys_.size() - 1, tiles_z = (int)zs_.size() - 1;
+  auto zone = [&](int lo, int hi, int n) {  // owned range [lo, hi) meets p <= L or p >= n-L-1
+    return lo < hi && (lo <= L || hi - 1 >= n - L - 1);
+  };
+  std::vector<uint32_t> bound;
+  tiles_host.clear();
+  if (tiles_z <= 255 && tiles_y <= 4095 && nch <= 4095) {
+    for (int c = 0; c < nch; c++)
+      for (int yy = 0; yy < tiles_y; yy++)
+        for (int zz = 0; zz < tiles_z; zz++) {
+          const bool b = h.p.dopml && (zone(xs_[c], xs_[c + 1], nx) || zone(ys_[yy], ys_[yy + 1], ny) ||
+                                       zone(zs_[zz], zs_[zz + 1], nz));
+          (b ? bound : tiles_host).push_back(pack_tile(zz, yy, c));
+        }
   }
-};
+  n_inner = (int)tiles_host.size();
+  n_bound = (int)bound.size();
+  tiles_host.insert(tiles_host.end(), bound.begin(), bound.end());
+  // NTFF: faces x (ny*nz points), y (nx*nz), z (nx*ny), two each
+  ntff_host.clear();
+  for (int face = 0; face < 6; face++) {
+    const long long npts = face < 2 ? (long long)ny * nz : face < 4 ? (long long)nx * nz : (long long)nx * ny;
+    const long long nb = (npts + kNtffBlock - 1) / kNtffBlock;
+    for (long long b = 0; b < nb; b++) ntff_host.push_back(make_uint2(0u, ((uint32_t)face << 29) | (uint32_t)b));
+  }
+  const size_t M2 = 2 * (size_t)h.M;
+  const size_t fb = (size_t)ny * nz, tb = (size_t)nx * nz, lr = (size_t)nx * ny;
+  size_t off = 0;
+  for (int q = 0; q < 24; q++) {
+    na.off[q] = (long long)off;
+    off += M2 * (q < 8 ? fb : q < 16 ? tb : lr);
+  }
+  acc_n = off;
+  return carve(nullptr);
+}
+
+// Assigns every device pointer inside the block at `base` (nullptr: sizes
+// only) and returns the block's length.  Arrays are 256-byte aligned.
+template <typename T>
+size_t Solver<T>::carve(char* base) {
+  const HostPlan& h = plan;
+  const int nx = h.nx, ny = h.ny, nz = h.nz, L = h.L;
+  const size_t N = (size_t)nx * ny * nz;
+  size_t total = 0;
+  auto take = [&](auto& ptr, size_t n) {
+    using P = std::remove_reference_t<decltype(ptr)>;
+    using U = std::remove_const_t<std::remove_pointer_t<P>>;
+    const size_t o = (total + 255) & ~(size_t)255;
+    total = o + std::max<size_t>(n, 1) * sizeof(U);
+    ptr = base ? reinterpret_cast<P>(base + o) : nullptr;
+  };
+  for (int q = 0; q < 2; q++)
+    for (int c = 0; c < 3; c++) {
+      take(sa.e[q][c], N);
+      take(sa.h[q][c], N);
+    }
+  for (int c = 0; c < 3; c++) take(sa.j[c], N);
+  take(sa.wpe2, N);
+  take(sa.nue, N);
+  // psi slabs: H ping-pong, E in place
+  const size_t sxy = (size_t)nx * 2 * L * nz, sxz = (size_t)nx * ny * 2 * L, syx = (size_t)2 * L * ny * nz;
+  const size_t psz[6] = {sxy, sxz, syx, sxz, syx, sxy};  // hxy hxz hyx hyz hzx hzy (and exy ... ezy)
+  for (int q = 0; q < 6; q++) {
+    if (h.p.dopml) {
+      take(sa.psih[0][q], psz[q]);
+      take(sa.psih[1][q], psz[q]);
+      take(sa.psie[q], psz[q]);
+    } else {
+      sa.psih[0][q] = sa.psih[1][q] = sa.psie[q] = nullptr;
+    }
+  }
+  take(sa.bh, 2 * L);
+  take(sa.ah, 2 * L);
+  take(sa.be, 2 * L);
+  take(sa.ae, 2 * L);
+  take(sa.inc, (size_t)h.T * 4 * nx);
+  take(tdev, 1);
+  const size_t fsz = frame_doubles();
+  if (h.p.savefields) {
+    take(sa.eslice, fsz);
+    take(eframes, fsz * h.nframes);
+    take(hframes, fsz * h.nframes);
+  } else {
+    sa.eslice = nullptr;
+    eframes = hframes = nullptr;
+  }
+  take(emax, h.nframes);
+  take(acc, acc_n);
+  take(na.sincos, (size_t)h.T * h.M * 2);
+  take(probes, (size_t)6 * h.T * std::max(1, h.p.nprobes));
+  take(xa.px, MSC_MAX_PROBES);
+  take(xa.py, MSC_MAX_PROBES);
+  take(xa.pz, MSC_MAX_PROBES);
+  take(tiles, tiles_host.size());
+  take(sa.xstart, xs_.size());
+  take(sa.ystart, ys_.size());
+  take(sa.zstart, zs_.size());
+  take(ntff_blocks, ntff_host.size());
+  take(sa_dev, 1);
+  take(na_dev, 1);
+  take(xa_dev, 1);
+  return (total + 255) & ~(size_t)255;
+}
 
 template <typename T>
-struct Solver : SolverBase {
-  StepArgs<T> sa{};
-  NtffArgs<T> na{};
-  AuxArgs<T> xa{};
-  T* eslice = nullptr;
-  double* acc = nullptr;
-  size_t acc_n = 0;
-  double* probes = nullptr;
-  double* eframes = nullptr;
-  double* hframes = nullptr;
-  unsigned long long* emax = nullptr;
-  int* tdev = nullptr;
-  int* pidx = nullptr;
-
-  int init(const double* wp, const double* nu) override {
-    const HostPlan& h = plan;
-    const int nx = h.nx, ny = h.ny, nz = h.nz, L = h.L;
-    const size_t N = (size_t)nx * ny * nz;
-    int rc;
-    if (const char* s = std::getenv("MSC_TY")) ty = std::atoi(s);
-    if (const char* s = std::getenv("MSC_CHUNK")) chunk = std::max(1, std::atoi(s));
-    if (ty != 3 && ty != 7) return fail(MSC_EINVAL, "MSC_TY must be 3 or 7");
-    if ((double)N * sizeof(T) >= 4294967296.0)
-      return fail(MSC_EINVAL, "grid of %zu cells exceeds the 4 GiB-per-array limit of the 32-bit offset kernel", N);
-    HIPCHK(hipStreamCreate(&stream));
-    for (int q = 0; q < 2; q++)
-      for (int c = 0; c < 3; c++) {
-        if ((rc = alloc(&sa.e[q][c], N))) return rc;
-        if ((rc = alloc(&sa.h[q][c], N))) return rc;
-      }
-    for (int c = 0; c < 3; c++)
-      if ((rc = alloc(&sa.j[c], N))) return rc;
-    T *w2 = nullptr, *nud = nullptr;
-    if ((rc = alloc(&w2, N)) || (rc = alloc(&nud, N))) return rc;
-    sa.wpe2 = w2;
-    sa.nue = nud;
-    // psi slabs: H ping-pong, E in place
-    const size_t sxy = (size_t)nx * 2 * L * nz, sxz = (size_t)nx * ny * 2 * L, syx = (size_t)2 * L * ny * nz;
-    const size_t psz[6] = {sxy, sxz, syx, sxz, syx, sxy};  // hxy hxz hyx hyz hzx hzy (and exy exz eyx eyz ezx ezy)
-    for (int q = 0; q < 6; q++) {
-      if (h.p.dopml) {
-        if ((rc = alloc(&sa.psih[0][q], psz[q])) || (rc = alloc(&sa.psih[1][q], psz[q])) ||
-            (rc = alloc(&sa.psie[q], psz[q])))
-          return rc;
-      } else {
-        sa.psih[0][q] = sa.psih[1][q] = sa.psie[q] = nullptr;
-      }
+int Solver<T>::init(const double* wp, const double* nu) {
+  const HostPlan& h = plan;
+  const int nx = h.nx, ny = h.ny, nz = h.nz, L = h.L;
+  const size_t N = (size_t)nx * ny * nz;
+  if (ty != 3 && ty != 7) return fail(MSC_EINVAL, "MSC_TY must be 3 or 7");
+  if ((double)N * sizeof(T) >= 4294967296.0)
+    return fail(MSC_EINVAL, "grid of %zu cells exceeds the 4 GiB-per-array limit of the 32-bit offset kernel", N);
+  const size_t bytes = layout();
+  if ((int)zs_.size() - 1 > 255 || (int)ys_.size() - 1 > 4095 || (int)xs_.size() - 1 > 4095)
+    return fail(MSC_EINVAL, "grid too large for the tile descriptor");
+  if (!stream) {
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    own_stream = true;
+  }
+  if (arena) {
+    blk = (char*)arena->alloc(bytes);
+    if (!blk) return fail(MSC_EBUSY, "arena has no free range of %zu bytes", bytes);
+  } else {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail(MSC_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    blk = (char*)q;
+  }
+  dev_bytes = bytes;
+  carve(blk);
+  HIPCHK(hipMemsetAsync(blk, 0, bytes, stream));
+  // host buffers stay alive until the final synchronisation
+  std::vector<std::vector<T>> keep;
+  auto upload = [&](const std::vector<double>& src, const T* dst) -> int {
+    keep.emplace_back(src.begin(), src.end());
+    HIPCHK(hipMemcpyAsync((void*)dst, keep.back().data(), keep.back().size() * sizeof(T), hipMemcpyHostToDevice, stream));
+    return MSC_OK;
+  };
+  int rc;
+  if ((rc = upload(h.bh, sa.bh)) || (rc = upload(h.ah, sa.ah)) || (rc = upload(h.be, sa.be)) ||
+      (rc = upload(h.ae, sa.ae)) || (rc = upload(h.inc, sa.inc)))
+    return rc;
+  // media: eps0 wpe^2 and nu per cell, computed in FP64 as meteor-fork.cpp:336-366
+  double numax = 0;
+  {
+    std::vector<T> pw((size_t)ny * nz), pn((size_t)ny * nz);
+    std::vector<double> g(nz), gn(nz);
+    const double msig0 = h.p.metsize[0];
+    for (int k = 0; k < nz; k++) {
+      g[k] = std::exp(-std::pow((h.zcoord[k] - h.p.metloc[2]), 2) / (msig0 * msig0));
+      gn[k] = h.p.nu_scale_height > 0 ? std::exp(-(h.zcoord[k] - h.p.metloc[2]) / h.p.nu_scale_height) : 1.0;
     }
-    // tables
-    auto upload = [&](const std::vector<double>& src, const T** dst) -> int {
-      std::vector<T> tmp(src.begin(), src.end());
-      T* d = nullptr;
-      int r = alloc(&d, tmp.size());
-      if (r) return r;
-      HIPCHK(hipMemcpyAsync(d, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice, stream));
-      HIPCHK(hipStreamSynchronize(stream));
-      *dst = d;
-      return MSC_OK;
-    };
-    if ((rc = upload(h.bh, &sa.bh)) || (rc = upload(h.ah, &sa.ah)) || (rc = upload(h.be, &sa.be)) ||
-        (rc = upload(h.ae, &sa.ae)) || (rc = upload(h.inc, &sa.inc)))
-      return rc;
-    // media: eps0 wpe^2 and nu per cell, computed in FP64 as meteor-fork.cpp:336-366
-    {
-      std::vector<T> pw((size_t)ny * nz), pn((size_t)ny * nz);
-      double numax = 0;
-      std::vector<double> g(nz), gn(nz);
-      const double msig0 = h.p.metsize[0];
-      for (int k = 0; k < nz; k++) {
-        g[k] = std::exp(-std::pow((h.zcoord[k] - h.p.metloc[2]), 2) / (msig0 * msig0));
-        gn[k] = h.p.nu_scale_height > 0 ? std::exp(-(h.zcoord[k] - h.p.metloc[2]) / h.p.nu_scale_height) : 1.0;
-      }
-      for (int i = 0; i < nx; i++) {
-        for (int j = 0; j < ny; j++)
-          for (int k = 0; k < nz; k++) {
-            double wpe = 0, nue = 0;
-            if (!h.p.doplate) {
-              if (h.p.media3d) {
-                const size_t c = ((size_t)i * ny + j) * nz + k;
-                wpe = wp[c];
-                nue = nu[c];
-              } else {
-                wpe = wp[(size_t)i * ny + j] * g[k];
-                nue = nu[(size_t)i * ny + j] * g[k];
-              }
-              if (h.p.nu_scale_height > 0) nue *= gn[k];
+    for (int i = 0; i < nx; i++) {
+      for (int j = 0; j < ny; j++)
+        for (int k = 0; k < nz; k++) {
+          double wpe = 0, nue = 0;
+          if (!h.p.doplate) {
+            if (h.p.media3d) {
+              const size_t c = ((size_t)i * ny + j) * nz + k;
+              wpe = wp[c];
+              nue = nu[c];
+            } else {
+              wpe = wp[(size_t)i * ny + j] * g[k];
+              nue = nu[(size_t)i * ny + j] * g[k];
             }
-            pw[(size_t)j * nz + k] = (T)(kE0 * wpe * wpe);
-            pn[(size_t)j * nz + k] = (T)nue;
-            numax = nue > numax ? nue : numax;
+            if (h.p.nu_scale_height > 0) nue *= gn[k];
           }
-        HIPCHK(hipMemcpyAsync(w2 + (size_t)i * ny * nz, pw.data(), pw.size() * sizeof(T), hipMemcpyHostToDevice, stream));
-        HIPCHK(hipMemcpyAsync(nud + (size_t)i * ny * nz, pn.data(), pn.size() * sizeof(T), hipMemcpyHostToDevice, stream));
-        HIPCHK(hipStreamSynchronize(stream));
-      }
-      // current model: DI, or Lee-Kalluri with polynomial (default) / closed-form coefficients
-      cur = kCurDI;
-      if (h.p.doB0) {
-        const char* force = std::getenv("MSC_LK_CLOSED");
-        const bool want_closed = force && std::atoi(force) != 0;
-        // the FP32 media rounding can lift nu slightly above the FP64 maximum
-        if (!want_closed && plan_lk_poly(h, numax * 1.001 * h.dt, sa.lkdeg, sa.lkc)) cur = kCurLKPoly;
-        else cur = kCurLKClosed;
-      }
-    }
-    // tiles: each axis is cut at the CPML slab edges (L+1 and n-L-1) for x
-    // and y, so that a tile is either inside a slab zone on some axis (the
-    // variant carrying the CPML code) or clear of every slab, halo included
-    // (the interior variant); z keeps full 63-wide tiles from 0, the first
-    // and last of which hold the z slabs.
-    {
-      auto cut = [&](int n, int piece, bool align) {
-        std::vector<int> st;
-        auto span = [&](int lo, int hi) {
-          for (int q = lo; q < hi; q += piece) st.push_back(q);
-        };
-        if (align && h.p.dopml) {
-          span(0, L + 1);
-          span(L + 1, n - L - 1);
-          span(n - L - 1, n);
-        } else {
-          span(0, n);
+          pw[(size_t)j * nz + k] = (T)(kE0 * wpe * wpe);
+          pn[(size_t)j * nz + k] = (T)nue;
+          numax = nue > numax ? nue : numax;
         }
-        st.push_back(n);
-        return st;
-      };
-      const std::vector<int> xs = cut(nx, chunk, true), ys = cut(ny, ty, true), zs = cut(nz, kOwnZ, false);
-      const int nch = (int)xs.size() - 1, tiles_y = (int)ys.size() - 1, tiles_z = (int)zs.size() - 1;
-      if (tiles_z > 255 || tiles_y > 4095 || nch > 4095) return fail(MSC_EINVAL, "grid too large for the tile descriptor");
-      auto zone = [&](int lo, int hi, int n) {  // [lo, hi) intersects p <= L or p >= n-L-1
-        lo = std::max(lo, 0);
-        hi = std::min(hi, n);
-        return lo < hi && (lo <= L || hi - 1 >= n - L - 1);
-      };
-      std::vector<uint32_t> inner, bound;
-      for (int c = 0; c < nch; c++)
-        for (int yy = 0; yy < tiles_y; yy++)
-          for (int zz = 0; zz < tiles_z; zz++) {
-            const bool b = h.p.dopml && (zone(xs[c] - 1, xs[c + 1], nx) || zone(ys[yy] - 1, ys[yy + 1], ny) ||
-                                         zone(zs[zz] - 1, zs[zz + 1], nz));
-            (b ? bound : inner).push_back(pack_tile(zz, yy, c));
-          }
-      n_inner = (int)inner.size();
-      n_bound = (int)bound.size();
-      inner.insert(inner.end(), bound.begin(), bound.end());
-      if ((rc = alloc(&tiles, inner.size()))) return rc;
-      HIPCHK(hipMemcpyAsync(tiles, inner.data(), inner.size() * 4, hipMemcpyHostToDevice, stream));
-      int *dx = nullptr, *dy = nullptr, *dz = nullptr;
-      if ((rc = alloc(&dx, xs.size())) || (rc = alloc(&dy, ys.size())) || (rc = alloc(&dz, zs.size()))) return rc;
-      HIPCHK(hipMemcpyAsync(dx, xs.data(), xs.size() * 4, hipMemcpyHostToDevice, stream));
-      HIPCHK(hipMemcpyAsync(dy, ys.data(), ys.size() * 4, hipMemcpyHostToDevice, stream));
-      HIPCHK(hipMemcpyAsync(dz, zs.data(), zs.size() * 4, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync((void*)(sa.wpe2 + (size_t)i * ny * nz), pw.data(), pw.size() * sizeof(T),
+                            hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync((void*)(sa.nue + (size_t)i * ny * nz), pn.data(), pn.size() * sizeof(T),
+                            hipMemcpyHostToDevice, stream));
       HIPCHK(hipStreamSynchronize(stream));
-      sa.xstart = dx;
-      sa.ystart = dy;
-      sa.zstart = dz;
-      sa.tiles_y = tiles_y;
-      sa.tiles_z = tiles_z;
-      sa.nchunks = nch;
     }
-    if ((rc = alloc(&tdev, 1))) return rc;
-    const size_t fsz = frame_doubles();
-    if (h.p.savefields) {
-      if ((rc = alloc(&eslice, fsz)) || (rc = alloc(&eframes, fsz * h.nframes)) || (rc = alloc(&hframes, fsz * h.nframes)))
-        return rc;
-    }
-    if ((rc = alloc(&emax, h.nframes))) return rc;
-    // NTFF accumulators: 24 arrays of [2M][n1*n2] in file order
-    const size_t M2 = 2 * (size_t)h.M;
-    const size_t fb = (size_t)ny * nz, tb = (size_t)nx * nz, lr = (size_t)nx * ny;
-    size_t off = 0;
-    for (int q = 0; q < 24; q++) {
-      na.off[q] = (long long)off;
-      off += M2 * (q < 8 ? fb : q < 16 ? tb : lr);
-    }
-    acc_n = off;
-    if ((rc = alloc(&acc, acc_n))) return rc;
-    double* scd = nullptr;
-    if ((rc = alloc(&scd, h.sincos.size()))) return rc;
-    HIPCHK(hipMemcpyAsync(scd, h.sincos.data(), h.sincos.size() * 8, hipMemcpyHostToDevice, stream));
-    if ((rc = alloc(&probes, (size_t)6 * h.T * std::max(1, h.p.nprobes)))) return rc;
-    if ((rc = alloc(&pidx, 3 * (size_t)MSC_MAX_PROBES))) return rc;
-    HIPCHK(hipMemcpyAsync(pidx, h.p.probex, 4 * MSC_MAX_PROBES, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(pidx + MSC_MAX_PROBES, h.p.probey, 4 * MSC_MAX_PROBES, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(pidx + 2 * MSC_MAX_PROBES, h.p.probez, 4 * MSC_MAX_PROBES, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-
-    // step arguments
-    sa.nx = nx; sa.ny = ny; sa.nz = nz; sa.L = L;
-    sa.xs = nx - 2 * L - 1; sa.ys = ny - 2 * L - 1; sa.zs = nz - 2 * L - 1;
-    sa.bx1 = L + 10; sa.bx2 = nx - L - 10; sa.by1 = L + 10; sa.by2 = ny - L - 10; sa.bz1 = L + 10; sa.bz2 = nz - L - 10;
-    sa.ssx = (nx + 1) / 2; sa.ssy = (ny + 1) / 2; sa.ssz = (nz + 1) / 2;
-    sa.dopml = h.p.dopml; sa.doplate = h.p.doplate; sa.doB0 = h.p.doB0;
-    sa.plate_lo = (int)(sa.ssy - h.p.platesize / 2 / h.dx);  // meteor-fork.cpp:1339-1340
-    sa.plate_up = (int)(sa.ssy + h.p.platesize / 2 / h.dx);
-    sa.chunk = chunk;
-    sa.cadence = h.cadence;
-    sa.sy = nz; sa.sx = (long long)ny * nz;
-    sa.c2h = (T)h.c2h; sa.c2e0 = (T)h.c2e0; sa.inv_d = (T)(1.0 / h.dx);
-    sa.c1s = (T)h.c1s; sa.c2s = (T)h.c2s;
-    sa.x0 = h.x0; sa.y0 = h.y0; sa.z0 = h.z0; sa.dd = h.dx; sa.rsph = h.p.platesize;
-    sa.dt = h.dt;
-    sa.wx = h.wce[0]; sa.wy = h.wce[1]; sa.wz = h.wce[2]; sa.w0 = h.w0;
-    sa.cth = h.cth; sa.sth = h.sth; sa.Se1 = h.Se1; sa.Ce1 = h.Ce1;
-    for (int q = 0; q < 9; q++) sa.R[q] = h.R[q];
-    sa.eslice = eslice;
-    sa.tdev = tdev;
-
-    na.nx = nx; na.ny = ny; na.nz = nz; na.M = h.M; na.b = L + 5;
-    na.sy = sa.sy; na.sx = sa.sx;
-    for (int q = 0; q < 2; q++)
-      for (int c = 0; c < 3; c++) { na.e[q][c] = sa.e[q][c]; na.h[q][c] = sa.h[q][c]; }
-    na.acc = acc; na.sincos = scd; na.tdev = tdev;
-
-    xa.nx = nx; xa.ny = ny; xa.nz = nz; xa.nprobes = h.p.nprobes; xa.tsteps = h.T;
-    xa.cadence = h.cadence; xa.nframes = h.nframes;
-    xa.sy = sa.sy; xa.sx = sa.sx;
-    for (int q = 0; q < 2; q++)
-      for (int c = 0; c < 3; c++) { xa.e[q][c] = sa.e[q][c]; xa.h[q][c] = sa.h[q][c]; }
-    xa.px = pidx; xa.py = pidx + MSC_MAX_PROBES; xa.pz = pidx + 2 * MSC_MAX_PROBES;
-    xa.probes = probes;
-    xa.ssx = sa.ssx; xa.ssy = sa.ssy; xa.ssz = sa.ssz;
-    xa.eslice = eslice; xa.eframes = eframes; xa.hframes = hframes; xa.emax = emax;
-    xa.tdev = tdev;
-    return MSC_OK;
   }
-
-  int step(int n) override {
-    if (n < 0) return fail(MSC_EINVAL, "nsteps < 0");
-    if (t_host + n > plan.T) return fail(MSC_ESTATE, "stepping past tsteps (%d + %d > %d)", t_host, n, plan.T);
-    if (timing) {
-      for (hipEvent_t e : ev) hipEventDestroy(e);
-      ev.assign(2 * (size_t)n, nullptr);
-      for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-    }
-    for (int s = 0; s < n; s++) {
-      const bool cad = (t_host % plan.cadence) == 0;
-      const bool slice = cad && plan.p.savefields;
-      if (timing) HIPCHK(hipEventRecord(ev[2 * s], stream));
-      HIPCHK(launch_step(sa, ty, cur, false, slice, tiles, n_inner, stream));
-      HIPCHK(launch_step(sa, ty, cur, true, slice, tiles + n_inner, n_bound, stream));
-      if (timing) HIPCHK(hipEventRecord(ev[2 * s + 1], stream));
-      HIPCHK(launch_ntff(na, stream));
-      HIPCHK(launch_aux(xa, cad, stream));
-      ++t_host;
-    }
-    return MSC_OK;
+  // current model: DI, or Lee-Kalluri with polynomial (default) / closed-form coefficients
+  cur = kCurDI;
+  if (h.p.doB0) {
+    const char* force = std::getenv("MSC_LK_CLOSED");
+    const bool want_closed = force && std::atoi(force) != 0;
+    // the FP32 media rounding can lift nu slightly above the FP64 maximum
+    if (!want_closed && plan_lk_poly(h, numax * 1.001 * h.dt, sa.lkdeg, sa.lkc)) cur = kCurLKPoly;
+    else cur = kCurLKClosed;
   }
+  HIPCHK(hipMemcpyAsync((void*)na.sincos, h.sincos.data(), h.sincos.size() * 8, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync((void*)xa.px, h.p.probex, 4 * MSC_MAX_PROBES, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync((void*)xa.py, h.p.probey, 4 * MSC_MAX_PROBES, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync((void*)xa.pz, h.p.probez, 4 * MSC_MAX_PROBES, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(tiles, tiles_host.data(), tiles_host.size() * 4, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync((void*)sa.xstart, xs_.data(), xs_.size() * 4, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync((void*)sa.ystart, ys_.data(), ys_.size() * 4, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync((void*)sa.zstart, zs_.data(), zs_.size() * 4, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(ntff_blocks, ntff_host.data(), ntff_host.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
 
-  int get_field(int comp, double* out) override {
-    if (comp < 0 || comp > 8) return fail(MSC_EINVAL, "component %d out of range", comp);
-    const size_t N = (size_t)plan.nx * plan.ny * plan.nz;
-    std::vector<T> tmp(N);
-    const int par = t_host & 1;
-    const T* src;
-    T* scratch[3] = {nullptr, nullptr, nullptr};
-    if (comp < 3) src = sa.e[par][comp];
-    else if (comp < 6) src = sa.h[par][comp - 3];
-    else {
-      // the device J lags one step (deferred update); catch it up into scratch
-      for (int c = 0; c < 3; c++) HIPCHK(hipMalloc(&scratch[c], N * sizeof(T)));
-      hipError_t e = launch_catchup(sa, cur, scratch[0], scratch[1], scratch[2], stream);
-      if (e != hipSuccess) {
-        for (int c = 0; c < 3; c++) hipFree(scratch[c]);
-        return fail(MSC_EHIP, "catch-up kernel: %s", hipGetErrorString(e));
-      }
-      src = scratch[comp - 6];
-    }
-    hipError_t e = hipMemcpyAsync(tmp.data(), src, N * sizeof(T), hipMemcpyDeviceToHost, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    for (int c = 0; c < 3; c++)
-      if (scratch[c]) hipFree(scratch[c]);
-    if (e != hipSuccess) return fail(MSC_EHIP, "field read-back: %s", hipGetErrorString(e));
-    for (size_t c = 0; c < N; c++) out[c] = (double)tmp[c];
-    return MSC_OK;
-  }
+  // step arguments
+  sa.nx = nx; sa.ny = ny; sa.nz = nz; sa.L = L;
+  sa.xs = nx - 2 * L - 1; sa.ys = ny - 2 * L - 1; sa.zs = nz - 2 * L - 1;
+  sa.bx1 = L + 10; sa.bx2 = nx - L - 10; sa.by1 = L + 10; sa.by2 = ny - L - 10; sa.bz1 = L + 10; sa.bz2 = nz - L - 10;
+  sa.ssx = (nx + 1) / 2; sa.ssy = (ny + 1) / 2; sa.ssz = (nz + 1) / 2;
+  sa.dopml = h.p.dopml; sa.doplate = h.p.doplate; sa.doB0 = h.p.doB0;
+  sa.plate_lo = (int)(sa.ssy - h.p.platesize / 2 / h.dx);  // meteor-fork.cpp:1339-1340
+  sa.plate_up = (int)(sa.ssy + h.p.platesize / 2 / h.dx);
+  sa.chunk = chunk;
+  sa.tiles_y = (int)ys_.size() - 1;
+  sa.tiles_z = (int)zs_.size() - 1;
+  sa.nchunks = (int)xs_.size() - 1;
+  sa.cadence = h.cadence;
+  sa.sy = nz; sa.sx = (long long)ny * nz;
+  sa.c2h = (T)h.c2h; sa.c2e0 = (T)h.c2e0; sa.inv_d = (T)(1.0 / h.dx);
+  sa.c1s = (T)h.c1s; sa.c2s = (T)h.c2s;
+  sa.x0 = h.x0; sa.y0 = h.y0; sa.z0 = h.z0; sa.dd = h.dx; sa.rsph = h.p.platesize;
+  sa.dt = h.dt;
+  sa.wx = h.wce[0]; sa.wy = h.wce[1]; sa.wz = h.wce[2]; sa.w0 = h.w0;
+  sa.cth = h.cth; sa.sth = h.sth; sa.Se1 = h.Se1; sa.Ce1 = h.Ce1;
+  for (int q = 0; q < 9; q++) sa.R[q] = h.R[q];
+  sa.tdev = tdev;
 
-  int get_ntff(double* out) override {
-    const size_t M2 = 2 * (size_t)plan.M;
-    std::vector<double> a(acc_n);
-    HIPCHK(hipMemcpyAsync(a.data(), acc, acc_n * 8, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-    std::memcpy(out, plan.eydft.data(), M2 * 8);
-    std::memcpy(out + M2, plan.ezdft.data(), M2 * 8);
-    double* o = out + 2 * M2;
-    const size_t fb = (size_t)plan.ny * plan.nz, tb = (size_t)plan.nx * plan.nz, lr = (size_t)plan.nx * plan.ny;
-    for (int q = 0; q < 24; q++) {
-      const size_t np = q < 8 ? fb : q < 16 ? tb : lr;
-      const double* src = a.data() + na.off[q];
-      for (size_t pt = 0; pt < np; pt++)
-        for (size_t m = 0; m < M2; m++) o[pt * M2 + m] = src[m * np + pt];
-      o += np * M2;
-    }
-    return MSC_OK;
-  }
+  na.nx = nx; na.ny = ny; na.nz = nz; na.M = h.M; na.b = L + 5;
+  na.sy = sa.sy; na.sx = sa.sx;
+  for (int q = 0; q < 2; q++)
+    for (int c = 0; c < 3; c++) { na.e[q][c] = sa.e[q][c]; na.h[q][c] = sa.h[q][c]; }
+  na.acc = acc; na.tdev = tdev;
 
-  int get_probes(double* out) override {
-    const size_t n = (size_t)6 * plan.T * plan.p.nprobes;
-    if (!n) return MSC_OK;
-    HIPCHK(hipMemcpyAsync(out, probes, n * 8, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-    return MSC_OK;
-  }
+  xa.nx = nx; xa.ny = ny; xa.nz = nz; xa.nprobes = h.p.nprobes; xa.tsteps = h.T;
+  xa.cadence = h.cadence; xa.nframes = h.nframes;
+  xa.sy = sa.sy; xa.sx = sa.sx;
+  for (int q = 0; q < 2; q++)
+    for (int c = 0; c < 3; c++) { xa.e[q][c] = sa.e[q][c]; xa.h[q][c] = sa.h[q][c]; }
+  xa.probes = probes;
+  xa.ssx = sa.ssx; xa.ssy = sa.ssy; xa.ssz = sa.ssz;
+  xa.eslice = sa.eslice; xa.eframes = eframes; xa.hframes = hframes; xa.emax = emax;
+  xa.tdev = tdev;
+  HIPCHK(hipMemcpyAsync(sa_dev, &sa, sizeof(sa), hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(na_dev, &na, sizeof(na), hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(xa_dev, &xa, sizeof(xa), hipMemcpyHostToDevice, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return MSC_OK;
+}
 
-  int get_frame(int f, double* e, double* hh, double* em) override {
-    const int done = t_host == 0 ? 0 : (t_host - 1) / plan.cadence + 1;
-    if (f < 0 || f >= done) return fail(MSC_EINVAL, "frame %d not captured (have %d)", f, done);
-    const size_t fsz = frame_doubles(), nx = plan.nx;
-    if ((e || hh) && !plan.p.savefields) return fail(MSC_ESTATE, "savefields=0: no slice frames");
-    if (e) {
-      HIPCHK(hipMemcpyAsync(e, eframes + (size_t)f * fsz, fsz * 8, hipMemcpyDeviceToHost, stream));
-    }
-    if (hh) {
-      HIPCHK(hipMemcpyAsync(hh, hframes + (size_t)f * fsz, fsz * 8, hipMemcpyDeviceToHost, stream));
-    }
-    unsigned long long bits = 0;
-    if (em) HIPCHK(hipMemcpyAsync(&bits, emax + f, 8, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-    if (e) std::memcpy(e + fsz, plan.eline.data() + (size_t)f * 2 * nx, 2 * nx * 8);
-    if (hh) std::memcpy(hh + fsz, plan.hline.data() + (size_t)f * 2 * nx, 2 * nx * 8);
-    if (em) std::memcpy(em, &bits, 8);
-    return MSC_OK;
+template <typename T>
+int Solver<T>::step(int n) {
+  if (n < 0) return fail(MSC_EINVAL, "nsteps < 0");
+  if (t_host + n > plan.T) return fail(MSC_ESTATE, "stepping past tsteps (%d + %d > %d)", t_host, n, plan.T);
+  if (timing) {
+    for (hipEvent_t e : ev) hipEventDestroy(e);
+    ev.assign(2 * (size_t)n, nullptr);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   }
-};
+  for (int s = 0; s < n; s++) {
+    const bool cad = (t_host % plan.cadence) == 0;
+    const bool slice = cad && plan.p.savefields;
+    if (timing) HIPCHK(hipEventRecord(ev[2 * s], stream));
+    HIPCHK(launch_step<T>(sa_dev, nullptr, ty, cur, false, slice, tiles, n_inner, stream));
+    HIPCHK(launch_step<T>(sa_dev, nullptr, ty, cur, true, slice, tiles + n_inner, n_bound, stream));
+    if (timing) HIPCHK(hipEventRecord(ev[2 * s + 1], stream));
+    HIPCHK(launch_ntff<T>(na_dev, ntff_blocks, (int)ntff_host.size(), stream));
+    if (cad) HIPCHK(launch_frames<T>(xa_dev, stream));
+    HIPCHK(launch_probes<T>(xa_dev, 1, stream));
+    ++t_host;
+  }
+  return MSC_OK;
+}
+
+template <typename T>
+int Solver<T>::get_field(int comp, double* out) {
+  if (comp < 0 || comp > 8) return fail(MSC_EINVAL, "component %d out of range", comp);
+  const size_t N = (size_t)plan.nx * plan.ny * plan.nz;
+  std::vector<T> tmp(N);
+  const int par = t_host & 1;
+  const T* src;
+  T* scratch[3] = {nullptr, nullptr, nullptr};
+  if (comp < 3) src = sa.e[par][comp];
+  else if (comp < 6) src = sa.h[par][comp - 3];
+  else {
+    // the device J lags one step (deferred update); catch it up into scratch
+    for (int c = 0; c < 3; c++) HIPCHK(hipMalloc(&scratch[c], N * sizeof(T)));
+    hipError_t e = launch_catchup<T>(sa_dev, cur, scratch[0], scratch[1], scratch[2], stream);
+    if (e != hipSuccess) {
+      for (int c = 0; c < 3; c++) hipFree(scratch[c]);
+      return fail(MSC_EHIP, "catch-up kernel: %s", hipGetErrorString(e));
+    }
+    src = scratch[comp - 6];
+  }
+  hipError_t e = hipMemcpyAsync(tmp.data(), src, N * sizeof(T), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  for (int c = 0; c < 3; c++)
+    if (scratch[c]) hipFree(scratch[c]);
+  if (e != hipSuccess) return fail(MSC_EHIP, "field read-back: %s", hipGetErrorString(e));
+  for (size_t c = 0; c < N; c++) out[c] = (double)tmp[c];
+  return MSC_OK;
+}
+
+template <typename T>
+int Solver<T>::get_ntff(double* out) {
+  const size_t M2 = 2 * (size_t)plan.M;
+  std::vector<double> a(acc_n);
+  HIPCHK(hipMemcpyAsync(a.data(), acc, acc_n * 8, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  std::memcpy(out, plan.eydft.data(), M2 * 8);
+  std::memcpy(out + M2, plan.ezdft.data(), M2 * 8);
+  double* o = out + 2 * M2;
+  const size_t fb = (size_t)plan.ny * plan.nz, tb = (size_t)plan.nx * plan.nz, lr = (size_t)plan.nx * plan.ny;
+  for (int q = 0; q < 24; q++) {
+    const size_t np = q < 8 ? fb : q < 16 ? tb : lr;
+    const double* src = a.data() + na.off[q];
+    for (size_t pt = 0; pt < np; pt++)
+      for (size_t m = 0; m < M2; m++) o[pt * M2 + m] = src[m * np + pt];
+    o += np * M2;
+  }
+  return MSC_OK;
+}
+
+template <typename T>
+int Solver<T>::get_probes(double* out) {
+  const size_t n = (size_t)6 * plan.T * plan.p.nprobes;
+  if (!n) return MSC_OK;
+  HIPCHK(hipMemcpyAsync(out, probes, n * 8, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return MSC_OK;
+}
+
+template <typename T>
+int Solver<T>::get_frame(int f, double* e, double* hh, double* em) {
+  const int done = t_host == 0 ? 0 : (t_host - 1) / plan.cadence + 1;
+  if (f < 0 || f >= done) return fail(MSC_EINVAL, "frame %d not captured (have %d)", f, done);
+  const size_t fsz = frame_doubles(), nx = plan.nx;
+  if ((e || hh) && !plan.p.savefields) return fail(MSC_ESTATE, "savefields=0: no slice frames");
+  if (e) {
+    HIPCHK(hipMemcpyAsync(e, eframes + (size_t)f * fsz, fsz * 8, hipMemcpyDeviceToHost, stream));
+  }
+  if (hh) {
+    HIPCHK(hipMemcpyAsync(hh, hframes + (size_t)f * fsz, fsz * 8, hipMemcpyDeviceToHost, stream));
+  }
+  unsigned long long bits = 0;
+  if (em) HIPCHK(hipMemcpyAsync(&bits, emax + f, 8, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  if (e) std::memcpy(e + fsz, plan.eline.data() + (size_t)f * 2 * nx, 2 * nx * 8);
+  if (hh) std::memcpy(hh + fsz, plan.hline.data() + (size_t)f * 2 * nx, 2 * nx * 8);
+  if (em) std::memcpy(em, &bits, 8);
+  return MSC_OK;
+}
+
+template struct Solver<float>;
+template struct Solver<double>;
+
+size_t solve_footprint(const msc_params* p, int precision) {
+  std::unique_ptr<SolverBase> s;
+  if (precision == 32) s.reset(new Solver<float>());
+  else if (precision == 64) s.reset(new Solver<double>());
+  else return 0;
+  if (make_plan(p, s->plan) != MSC_OK) return 0;
+  return s->layout();
+}
 
 }  // namespace msc
 
