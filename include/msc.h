@@ -197,6 +197,66 @@ void msc_solver_destroy(msc_solver* s);
 int msc_stream_copy(size_t bytes, int iters, double* gbps);
 
 /*
+ * Multi-solve server.  Replaces the batch layer of the reference: the loops
+ * of setup/meteorbatch.m:26-40 and setup/meteorrandom.m:30-44 that submit
+ * one PBS job per run (setup/meteorrun.m:154-183), each job one process
+ * running the whole time loop.  Here the runs share one GPU:
+ *  - msc_server_create reserves one device arena of `budget_bytes`
+ *    (0: 90% of the free device memory) and starts a worker thread;
+ *  - msc_server_submit queues a solve (inputs copied; savefields must be 0,
+ *    the server returns the NTFF payload and the probes);
+ *  - the worker prepares queued solves in submission order, each as soon
+ *    as its block fits in the arena (msc_solve_footprint gives its size);
+ *  - msc_server_advance runs up to `max_iterations` gang iterations, each
+ *    advancing every resident solve by one time step, and returns early
+ *    when nothing is resident; *ndone counts the solves completed in the
+ *    call.  Call it from one host thread;
+ *  - msc_server_info / _get_ntff / _get_probes read a request's state,
+ *    timestamps and results; msc_server_release forgets a finished request.
+ * A solve's results are the same bits as the same solve run alone through
+ * msc_solver_* (same kernels on the same per-solve data).
+ */
+typedef struct msc_server msc_server;
+
+enum {
+  MSC_REQ_QUEUED = 0,   /* submitted, waiting for preparation / arena space */
+  MSC_REQ_READY = 1,    /* resident, joins the gang at the next iteration */
+  MSC_REQ_RUNNING = 2,  /* stepping */
+  MSC_REQ_DRAINING = 3, /* all steps launched, completing */
+  MSC_REQ_DONE = 4,     /* results available */
+  MSC_REQ_FAILED = 5    /* rejected: msc_server_info sets msc_last_error */
+};
+
+typedef struct msc_request_info {
+  int32_t state;
+  int32_t t;            /* time steps launched */
+  int32_t tsteps;
+  size_t device_bytes;  /* its arena block */
+  double t_submit, t_start, t_done;  /* seconds on the server clock; 0 = not yet */
+} msc_request_info;
+
+typedef struct msc_server_stats {
+  int32_t queued, ready, running, draining;
+  size_t arena_bytes, arena_used;
+  long long iterations;   /* gang iterations launched */
+  long long solve_steps;  /* solve time steps launched */
+} msc_server_stats;
+
+/* Device bytes one solve occupies (0 for invalid parameters). */
+size_t msc_solve_footprint(const msc_params* p, int precision);
+
+int msc_server_create(int precision, size_t budget_bytes, msc_server** out);
+int msc_server_submit(msc_server* s, const msc_params* p, const double* wp, const double* nu, int64_t* id);
+int msc_server_advance(msc_server* s, int max_iterations, int* ndone);
+int msc_server_info(msc_server* s, int64_t id, msc_request_info* info);
+int msc_server_get_ntff(msc_server* s, int64_t id, double* out, size_t n);
+int msc_server_get_probes(msc_server* s, int64_t id, double* out, size_t n);
+int msc_server_release(msc_server* s, int64_t id);
+int msc_server_get_stats(msc_server* s, msc_server_stats* out);
+double msc_server_clock(const msc_server* s);
+void msc_server_destroy(msc_server* s);
+
+/*
  * Whole drop-in run in `dir`: read the files, solve, write output_E.dat,
  * output_H.dat, output_J.dat (when savefields), output_NTF.dat, Probe.dat
  * and log.txt with the reference's layouts.  This is main() of

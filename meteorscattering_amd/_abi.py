@@ -114,6 +114,40 @@ def load_library(path: str | None = None) -> C.CDLL:
     return lib
 
 
+# request states (msc.h)
+REQ_QUEUED, REQ_READY, REQ_RUNNING, REQ_DRAINING, REQ_DONE, REQ_FAILED = range(6)
+REQ_STATES = ("queued", "ready", "running", "draining", "done", "failed")
+
+
+class RequestInfo(C.Structure):
+    """``msc_request_info``."""
+
+    _fields_ = [
+        ("state", C.c_int32),
+        ("t", C.c_int32),
+        ("tsteps", C.c_int32),
+        ("device_bytes", C.c_size_t),
+        ("t_submit", C.c_double),
+        ("t_start", C.c_double),
+        ("t_done", C.c_double),
+    ]
+
+
+class ServerStats(C.Structure):
+    """``msc_server_stats``."""
+
+    _fields_ = [
+        ("queued", C.c_int32),
+        ("ready", C.c_int32),
+        ("running", C.c_int32),
+        ("draining", C.c_int32),
+        ("arena_bytes", C.c_size_t),
+        ("arena_used", C.c_size_t),
+        ("iterations", C.c_longlong),
+        ("solve_steps", C.c_longlong),
+    ]
+
+
 def _declare(lib: C.CDLL) -> None:
     P = C.POINTER
     vp = C.c_void_p
@@ -144,6 +178,17 @@ def _declare(lib: C.CDLL) -> None:
         "msc_solver_destroy": (None, [vp]),
         "msc_run_dir": (C.c_int, [C.c_char_p, C.c_int]),
         "msc_stream_copy": (C.c_int, [C.c_size_t, C.c_int, dp]),
+        "msc_solve_footprint": (C.c_size_t, [P(MscParams), C.c_int]),
+        "msc_server_create": (C.c_int, [C.c_int, C.c_size_t, P(vp)]),
+        "msc_server_submit": (C.c_int, [vp, P(MscParams), dp, dp, P(C.c_int64)]),
+        "msc_server_advance": (C.c_int, [vp, C.c_int, P(C.c_int)]),
+        "msc_server_info": (C.c_int, [vp, C.c_int64, P(RequestInfo)]),
+        "msc_server_get_ntff": (C.c_int, [vp, C.c_int64, dp, C.c_size_t]),
+        "msc_server_get_probes": (C.c_int, [vp, C.c_int64, dp, C.c_size_t]),
+        "msc_server_release": (C.c_int, [vp, C.c_int64]),
+        "msc_server_get_stats": (C.c_int, [vp, P(ServerStats)]),
+        "msc_server_clock": (C.c_double, [vp]),
+        "msc_server_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -22,7 +22,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall"]
 
-HIP_SOURCES = ["msc_kernels.hip", "msc_solver.hip"]
+HIP_SOURCES = ["msc_kernels.hip", "msc_solver.hip", "msc_server.hip"]
 CXX_SOURCES = ["msc_io.cpp", "msc_run.cpp"]
 
 

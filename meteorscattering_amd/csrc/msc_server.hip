@@ -1,0 +1,593 @@
+// Multi-solve server: many independent solves resident on one GPU, advanced
+// together by ganged launches.
+//
+// Reference counterpart: setup/meteorbatch.m and setup/meteorrandom.m submit
+// one PBS job per run (setup/meteorrun.m:154-183), each a separate process
+// running the whole time loop of meteor-fork.cpp on its own.  Here the runs
+// share one device:
+//   - every solve owns one block of a fixed device arena (no hipMalloc or
+//     hipFree while serving); requests are admitted first come, first served
+//     when their block fits;
+//   - a worker thread builds each request's FP64 plan and fills its block on
+//     a set-up stream while the resident solves keep stepping;
+//   - one gang iteration advances every resident solve by one time step with
+//     one fused-step launch per current model and CPML variant, one NTFF
+//     launch and one probe launch, whatever the number of solves; the tile
+//     lists carry a solve id per workgroup, so small grids fill the GPU
+//     together instead of one after another;
+//   - a solve leaves the gang after its last step; its results are copied
+//     out once that step has completed and its block returns to the arena.
+// Each solve's arithmetic is exactly its single-solve arithmetic (the same
+// kernels on the same per-solve argument block), so results do not depend
+// on what else is resident.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "msc_solver.h"
+
+namespace msc {
+
+struct ServerBase {
+  virtual ~ServerBase() = default;
+  virtual int submit(const msc_params* p, const double* wp, const double* nu, int64_t* id) = 0;
+  virtual int advance(int max_iter, int* ndone) = 0;
+  virtual int info(int64_t id, msc_request_info* out) = 0;
+  virtual int get_ntff(int64_t id, double* out, size_t n) = 0;
+  virtual int get_probes(int64_t id, double* out, size_t n) = 0;
+  virtual int release(int64_t id) = 0;
+  virtual int stats(msc_server_stats* out) = 0;
+  virtual double clock() const = 0;
+};
+
+template <typename T>
+struct Server : ServerBase {
+  struct Req {
+    int64_t id = 0;
+    msc_params p{};
+    std::vector<double> wp, nu;
+    std::unique_ptr<Solver<T>> s;
+    int state = MSC_REQ_QUEUED;
+    std::atomic<int> t{0};  // steps launched
+    size_t bytes = 0;
+    hipEvent_t done = nullptr;
+    double t_submit = 0, t_start = 0, t_done = 0;
+    std::vector<double> ntff, probes;
+    std::string err;
+  };
+  // device copies of the gang's argument blocks and launch lists; two sets
+  // so a rebuild never overwrites lists that queued launches still read
+  struct Gang {
+    StepArgs<T>* sa = nullptr;
+    NtffArgs<T>* na = nullptr;
+    AuxArgs<T>* xa = nullptr;
+    uint32_t* tiles = nullptr;
+    uint16_t* sids = nullptr;
+    uint2* nb = nullptr;
+    size_t cap_solves = 0, cap_tiles = 0, cap_nb = 0;
+    // host staging (pinned)
+    char* host = nullptr;
+    size_t host_bytes = 0;
+    hipEvent_t last_use = nullptr;
+    bool used = false;
+  };
+
+  int device = 0;
+  hipStream_t run = nullptr, setup = nullptr;
+  Arena arena;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+  std::mutex mu;
+  std::condition_variable cv_work, cv_ready;
+  std::deque<Req*> queue;    // submitted, not yet prepared (FIFO)
+  std::vector<Req*> ready;   // prepared, waiting to join the gang
+  std::vector<Req*> active;  // stepping
+  std::vector<Req*> draining;  // all steps launched, waiting for completion
+  std::map<int64_t, std::unique_ptr<Req>> reqs;
+  int64_t next_id = 1;
+  bool stop = false;
+  bool worker_busy = false;
+  std::thread worker;
+
+  Gang gang[2];
+  int cur_gang = 0;
+  bool dirty = true;
+  // launch lists of the current gang: per current model, [interior, boundary]
+  int n_tiles[3][2] = {};
+  size_t tile_off[3][2] = {};
+  int n_nb = 0;
+  // run-ahead limit: at most kDepth iterations queued on the run stream
+  static constexpr int kDepth = 4;
+  hipEvent_t iter_ev[kDepth] = {};
+  long long iters = 0;
+  long long steps_done = 0;  // solve-steps launched
+
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  double clock() const override { return now(); }
+
+  int init(size_t budget) {
+    HIPCHK(hipGetDevice(&device));
+    HIPCHK(hipStreamCreateWithFlags(&run, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&setup, hipStreamNonBlocking));
+    for (auto& e : iter_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto& g : gang) HIPCHK(hipEventCreateWithFlags(&g.last_use, hipEventDisableTiming));
+    if (budget == 0) {
+      size_t fr = 0, tot = 0;
+      HIPCHK(hipMemGetInfo(&fr, &tot));
+      budget = fr / 10 * 9;
+    }
+    int rc = arena.init(budget);
+    if (rc) return rc;
+    worker = std::thread([this] { work(); });
+    return MSC_OK;
+  }
+
+  ~Server() override {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      stop = true;
+    }
+    cv_work.notify_all();
+    if (worker.joinable()) worker.join();
+    if (run) hipStreamSynchronize(run);
+    if (setup) hipStreamSynchronize(setup);
+    for (auto& kv : reqs) {
+      if (kv.second->done) hipEventDestroy(kv.second->done);
+      kv.second->s.reset();
+    }
+    for (auto& g : gang) {
+      hipFree(g.sa); hipFree(g.na); hipFree(g.xa); hipFree(g.tiles); hipFree(g.sids); hipFree(g.nb);
+      if (g.host) hipHostFree(g.host);
+      if (g.last_use) hipEventDestroy(g.last_use);
+    }
+    for (auto& e : iter_ev)
+      if (e) hipEventDestroy(e);
+    if (run) hipStreamDestroy(run);
+    if (setup) hipStreamDestroy(setup);
+  }
+
+  // ---- worker: FIFO preparation of queued requests
+  void work() {
+    hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+      if (stop) return;
+      Req* r = queue.front();
+      worker_busy = true;
+      lk.unlock();
+      int rc = prepare(r);
+      lk.lock();
+      worker_busy = false;
+      if (rc == MSC_EBUSY) {
+        // no free range yet: wait for a retirement (or shutdown), keep FIFO order
+        cv_work.wait_for(lk, std::chrono::milliseconds(50));
+        continue;
+      }
+      queue.pop_front();
+      if (rc == MSC_OK) {
+        r->state = MSC_REQ_READY;
+        ready.push_back(r);
+      } else {
+        r->state = MSC_REQ_FAILED;
+        r->err = msc_last_error();
+        r->t_done = now();
+        r->wp.clear(); r->wp.shrink_to_fit();
+        r->nu.clear(); r->nu.shrink_to_fit();
+      }
+      cv_ready.notify_all();
+    }
+  }
+
+  int prepare(Req* r) {
+    if (!r->s) {
+      r->s.reset(new Solver<T>());
+      r->s->tiling_from_env = false;  // the gang launches 7-row tiles
+      int rc = make_plan(&r->p, r->s->plan);
+      if (rc) return rc;
+      r->bytes = r->s->layout();
+      if (r->bytes > arena.capacity())
+        return fail(MSC_ENOMEM, "solve needs %zu device bytes, server arena holds %zu", r->bytes, arena.capacity());
+      r->s->arena = &arena;
+      r->s->stream = setup;
+      r->s->own_stream = false;
+    }
+    int rc = r->s->init(r->wp.data(), r->nu.data());
+    if (rc == MSC_EBUSY) return rc;
+    if (rc) {
+      r->s.reset();
+      return rc;
+    }
+    r->wp.clear(); r->wp.shrink_to_fit();
+    r->nu.clear(); r->nu.shrink_to_fit();
+    return MSC_OK;
+  }
+
+  int submit(const msc_params* p, const double* wp, const double* nu, int64_t* id) override {
+    int rc = msc_params_validate(p);
+    if (rc) return rc;
+    if (p->savefields) return fail(MSC_EINVAL, "server solves produce NTFF and probe results only: set savefields = 0");
+    std::unique_ptr<Req> r(new Req());
+    r->p = *p;
+    const size_t n2 = p->media3d ? (size_t)p->nx * p->ny * p->nz : (size_t)p->nx * p->ny;
+    r->wp.assign(wp, wp + n2);
+    r->nu.assign(nu, nu + n2);
+    r->t_submit = now();
+    std::lock_guard<std::mutex> g(mu);
+    r->id = next_id++;
+    *id = r->id;
+    queue.push_back(r.get());
+    reqs[r->id] = std::move(r);
+    cv_work.notify_all();
+    return MSC_OK;
+  }
+
+  // ---- gang lists
+  // grow a device list (rare: capacities double); the set is idle here
+  template <typename U>
+  int regrow(U*& ptr, size_t n) {
+    if (ptr) hipFree(ptr);
+    ptr = nullptr;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(U));
+    if (e != hipSuccess) return fail(MSC_ENOMEM, "gang list of %zu entries: %s", n, hipGetErrorString(e));
+    ptr = (U*)q;
+    return MSC_OK;
+  }
+
+  int ensure(Gang& g, size_t nsol, size_t ntiles, size_t nnb) {
+    int rc;
+    if (nsol > g.cap_solves) {
+      const size_t c = std::max(nsol, 2 * g.cap_solves);
+      if ((rc = regrow(g.sa, c)) || (rc = regrow(g.na, c)) || (rc = regrow(g.xa, c))) return rc;
+      g.cap_solves = c;
+    }
+    if (ntiles > g.cap_tiles) {
+      const size_t c = std::max(ntiles, 2 * g.cap_tiles);
+      if ((rc = regrow(g.tiles, c)) || (rc = regrow(g.sids, c))) return rc;
+      g.cap_tiles = c;
+    }
+    if (nnb > g.cap_nb) {
+      const size_t c = std::max(nnb, 2 * g.cap_nb);
+      if ((rc = regrow(g.nb, c))) return rc;
+      g.cap_nb = c;
+    }
+    const size_t hb = nsol * (sizeof(StepArgs<T>) + sizeof(NtffArgs<T>) + sizeof(AuxArgs<T>)) + ntiles * 6 +
+                      nnb * sizeof(uint2) + 1024;
+    if (hb > g.host_bytes) {
+      if (g.host) hipHostFree(g.host);
+      g.host = nullptr;
+      void* q = nullptr;
+      HIPCHK(hipHostMalloc(&q, hb * 2));
+      g.host = (char*)q;
+      g.host_bytes = hb * 2;
+    }
+    return MSC_OK;
+  }
+
+  int rebuild() {
+    const int gi = cur_gang ^ 1;  // the set the queued launches are not reading
+    Gang& g = gang[gi];
+    if (g.used) HIPCHK(hipEventSynchronize(g.last_use));
+    const size_t ns = active.size();
+    if (ns > 65535) return fail(MSC_EINVAL, "more than 65535 resident solves");
+    size_t nt = 0, nnb = 0;
+    for (Req* r : active) {
+      nt += r->s->tiles_host.size();
+      nnb += r->s->ntff_host.size();
+    }
+    int rc = ensure(g, std::max<size_t>(ns, 1), std::max<size_t>(nt, 1), std::max<size_t>(nnb, 1));
+    if (rc) return rc;
+    // host image: args, then tiles/sids grouped by (current model, variant), then NTFF blocks
+    char* h = g.host;
+    StepArgs<T>* hsa = (StepArgs<T>*)h;
+    NtffArgs<T>* hna = (NtffArgs<T>*)(hsa + ns);
+    AuxArgs<T>* hxa = (AuxArgs<T>*)(hna + ns);
+    uint32_t* htiles = (uint32_t*)(hxa + ns);
+    uint16_t* hsids = (uint16_t*)(htiles + nt);
+    uint2* hnb = (uint2*)(((uintptr_t)(hsids + nt) + 7) & ~(uintptr_t)7);
+    for (size_t q = 0; q < ns; q++) {
+      hsa[q] = active[q]->s->sa;
+      hna[q] = active[q]->s->na;
+      hxa[q] = active[q]->s->xa;
+    }
+    size_t o = 0;
+    for (int c = 0; c < 3; c++)
+      for (int v = 0; v < 2; v++) {
+        tile_off[c][v] = o;
+        for (size_t q = 0; q < ns; q++) {
+          const Solver<T>& s = *active[q]->s;
+          if (s.cur != c) continue;
+          const size_t b = v ? s.n_inner : 0, e = v ? s.n_inner + s.n_bound : s.n_inner;
+          for (size_t k = b; k < e; k++) {
+            htiles[o] = s.tiles_host[k];
+            hsids[o] = (uint16_t)q;
+            ++o;
+          }
+        }
+        n_tiles[c][v] = (int)(o - tile_off[c][v]);
+      }
+    size_t m = 0;
+    for (size_t q = 0; q < ns; q++)
+      for (const uint2& d : active[q]->s->ntff_host) hnb[m++] = make_uint2((uint32_t)q, d.y);
+    n_nb = (int)m;
+    HIPCHK(hipMemcpyAsync(g.sa, hsa, ns * sizeof(StepArgs<T>), hipMemcpyHostToDevice, run));
+    HIPCHK(hipMemcpyAsync(g.na, hna, ns * sizeof(NtffArgs<T>), hipMemcpyHostToDevice, run));
+    HIPCHK(hipMemcpyAsync(g.xa, hxa, ns * sizeof(AuxArgs<T>), hipMemcpyHostToDevice, run));
+    if (nt) {
+      HIPCHK(hipMemcpyAsync(g.tiles, htiles, nt * 4, hipMemcpyHostToDevice, run));
+      HIPCHK(hipMemcpyAsync(g.sids, hsids, nt * 2, hipMemcpyHostToDevice, run));
+    }
+    if (m) HIPCHK(hipMemcpyAsync(g.nb, hnb, m * sizeof(uint2), hipMemcpyHostToDevice, run));
+    cur_gang = gi;
+    g.used = true;
+    dirty = false;
+    return MSC_OK;
+  }
+
+  // completed solves: copy results out, free their blocks
+  int retire(int* ndone) {
+    for (size_t q = 0; q < draining.size();) {
+      Req* r = draining[q];
+      hipError_t e = hipEventQuery(r->done);
+      if (e == hipErrorNotReady) {
+        ++q;
+        continue;
+      }
+      if (e != hipSuccess) return fail(MSC_EHIP, "solve %lld: %s", (long long)r->id, hipGetErrorString(e));
+      r->ntff.resize(r->s->ntff_doubles());
+      int rc = r->s->get_ntff(r->ntff.data());
+      if (rc) return rc;
+      r->probes.resize((size_t)6 * r->s->plan.T * r->p.nprobes);
+      if (!r->probes.empty() && (rc = r->s->get_probes(r->probes.data()))) return rc;
+      r->s.reset();  // block back to the arena
+      hipEventDestroy(r->done);
+      r->done = nullptr;
+      {
+        std::lock_guard<std::mutex> g(mu);
+        r->state = MSC_REQ_DONE;
+        r->t_done = now();
+      }
+      cv_work.notify_all();
+      draining.erase(draining.begin() + q);
+      if (ndone) ++*ndone;
+    }
+    return MSC_OK;
+  }
+
+  int launch_iteration() {
+    const Gang& g = gang[cur_gang];
+    const int slot = (int)(iters % kDepth);
+    if (iters >= kDepth) HIPCHK(hipEventSynchronize(iter_ev[slot]));
+    for (int c = 0; c < 3; c++)
+      for (int v = 0; v < 2; v++) {
+        const int n = n_tiles[c][v];
+        if (!n) continue;
+        const size_t o = tile_off[c][v];
+        HIPCHK(launch_step<T>(g.sa, g.sids + o, 7, c, v == 1, false, g.tiles + o, n, run));
+      }
+    HIPCHK(launch_ntff<T>(g.na, g.nb, n_nb, run));
+    HIPCHK(launch_probes<T>(g.xa, (int)active.size(), run));
+    HIPCHK(hipEventRecord(iter_ev[slot], run));
+    HIPCHK(hipEventRecord(gang[cur_gang].last_use, run));
+    ++iters;
+    steps_done += (long long)active.size();
+    for (size_t q = 0; q < active.size();) {
+      Req* r = active[q];
+      if (++r->t < r->s->plan.T) {
+        ++q;
+        continue;
+      }
+      HIPCHK(hipEventCreateWithFlags(&r->done, hipEventDisableTiming));
+      HIPCHK(hipEventRecord(r->done, run));
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        r->state = MSC_REQ_DRAINING;
+      }
+      draining.push_back(r);
+      active.erase(active.begin() + q);
+      dirty = true;
+    }
+    return MSC_OK;
+  }
+
+  int advance(int max_iter, int* ndone) override {
+    if (ndone) *ndone = 0;
+    hipSetDevice(device);
+    int it = 0;
+    while (it < max_iter) {
+      int rc = retire(ndone);
+      if (rc) return rc;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        if (!ready.empty()) {
+          for (Req* r : ready) {
+            r->state = MSC_REQ_RUNNING;
+            r->t_start = now();
+            active.push_back(r);
+          }
+          ready.clear();
+          dirty = true;
+        }
+        if (active.empty() && draining.empty()) {
+          // idle, or only preparation pending: give the worker a moment and
+          // return so the caller can submit or poll meanwhile
+          if (!queue.empty()) cv_ready.wait_for(lk, std::chrono::milliseconds(2));
+          if (ready.empty()) break;
+          continue;
+        }
+      }
+      if (active.empty()) {
+        // wait for the draining solves' work
+        HIPCHK(hipStreamSynchronize(run));
+        continue;
+      }
+      if (dirty && (rc = rebuild())) return rc;
+      if ((rc = launch_iteration())) return rc;
+      ++it;
+    }
+    return retire(ndone);
+  }
+
+  Req* find(int64_t id) {
+    auto it = reqs.find(id);
+    return it == reqs.end() ? nullptr : it->second.get();
+  }
+
+  int info(int64_t id, msc_request_info* out) override {
+    std::lock_guard<std::mutex> g(mu);
+    Req* r = find(id);
+    if (!r) return fail(MSC_EINVAL, "unknown request %lld", (long long)id);
+    out->state = r->state;
+    out->t = r->t;
+    out->tsteps = r->p.tsteps;
+    out->device_bytes = r->bytes;
+    out->t_submit = r->t_submit;
+    out->t_start = r->t_start;
+    out->t_done = r->t_done;
+    if (r->state == MSC_REQ_FAILED) fail(MSC_EINVAL, "%s", r->err.c_str());
+    return MSC_OK;
+  }
+
+  int get_ntff(int64_t id, double* out, size_t n) override {
+    std::lock_guard<std::mutex> g(mu);
+    Req* r = find(id);
+    if (!r) return fail(MSC_EINVAL, "unknown request %lld", (long long)id);
+    if (r->state != MSC_REQ_DONE) return fail(MSC_ESTATE, "request %lld not done", (long long)id);
+    if (!out || n < r->ntff.size()) return fail(MSC_EINVAL, "output buffer too small (%zu < %zu)", n, r->ntff.size());
+    std::memcpy(out, r->ntff.data(), r->ntff.size() * 8);
+    return MSC_OK;
+  }
+
+  int get_probes(int64_t id, double* out, size_t n) override {
+    std::lock_guard<std::mutex> g(mu);
+    Req* r = find(id);
+    if (!r) return fail(MSC_EINVAL, "unknown request %lld", (long long)id);
+    if (r->state != MSC_REQ_DONE) return fail(MSC_ESTATE, "request %lld not done", (long long)id);
+    if (n < r->probes.size() || (!out && !r->probes.empty())) return fail(MSC_EINVAL, "output buffer too small");
+    if (!r->probes.empty()) std::memcpy(out, r->probes.data(), r->probes.size() * 8);
+    return MSC_OK;
+  }
+
+  int release(int64_t id) override {
+    std::lock_guard<std::mutex> g(mu);
+    Req* r = find(id);
+    if (!r) return fail(MSC_EINVAL, "unknown request %lld", (long long)id);
+    if (r->state != MSC_REQ_DONE && r->state != MSC_REQ_FAILED)
+      return fail(MSC_ESTATE, "request %lld still in flight", (long long)id);
+    reqs.erase(id);
+    return MSC_OK;
+  }
+
+  int stats(msc_server_stats* out) override {
+    std::lock_guard<std::mutex> g(mu);
+    out->queued = (int)queue.size();
+    out->ready = (int)ready.size();
+    out->running = (int)active.size();
+    out->draining = (int)draining.size();
+    out->arena_bytes = arena.capacity();
+    out->arena_used = arena.used();
+    out->iterations = iters;
+    out->solve_steps = steps_done;
+    return MSC_OK;
+  }
+};
+
+}  // namespace msc
+
+using namespace msc;
+
+struct msc_server {
+  std::unique_ptr<ServerBase> impl;
+};
+
+#define SERVER_OK(s) \
+  if (!(s) || !(s)->impl) return fail(MSC_EINVAL, "NULL server")
+
+extern "C" {
+
+size_t msc_solve_footprint(const msc_params* p, int precision) {
+  if (!p || msc_params_validate(p) != MSC_OK) return 0;
+  return solve_footprint(p, precision);
+}
+
+int msc_server_create(int precision, size_t budget_bytes, msc_server** out) {
+  if (!out) return fail(MSC_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (precision != 32 && precision != 64) return fail(MSC_EINVAL, "precision must be 32 or 64 (got %d)", precision);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MSC_EHIP, "no HIP device visible");
+  std::unique_ptr<ServerBase> s;
+  int rc;
+  if (precision == 32) {
+    auto* q = new Server<float>();
+    s.reset(q);
+    rc = q->init(budget_bytes);
+  } else {
+    auto* q = new Server<double>();
+    s.reset(q);
+    rc = q->init(budget_bytes);
+  }
+  if (rc) return rc;
+  msc_server* h = new msc_server;
+  h->impl = std::move(s);
+  *out = h;
+  return MSC_OK;
+}
+
+int msc_server_submit(msc_server* s, const msc_params* p, const double* wp, const double* nu, int64_t* id) {
+  SERVER_OK(s);
+  if (!p || !wp || !nu || !id) return fail(MSC_EINVAL, "NULL argument");
+  return s->impl->submit(p, wp, nu, id);
+}
+
+int msc_server_advance(msc_server* s, int max_iterations, int* ndone) {
+  SERVER_OK(s);
+  if (max_iterations < 0) return fail(MSC_EINVAL, "max_iterations < 0");
+  return s->impl->advance(max_iterations, ndone);
+}
+
+int msc_server_info(msc_server* s, int64_t id, msc_request_info* info) {
+  SERVER_OK(s);
+  if (!info) return fail(MSC_EINVAL, "NULL argument");
+  return s->impl->info(id, info);
+}
+
+int msc_server_get_ntff(msc_server* s, int64_t id, double* out, size_t n) {
+  SERVER_OK(s);
+  return s->impl->get_ntff(id, out, n);
+}
+
+int msc_server_get_probes(msc_server* s, int64_t id, double* out, size_t n) {
+  SERVER_OK(s);
+  return s->impl->get_probes(id, out, n);
+}
+
+int msc_server_release(msc_server* s, int64_t id) {
+  SERVER_OK(s);
+  return s->impl->release(id);
+}
+
+int msc_server_get_stats(msc_server* s, msc_server_stats* out) {
+  SERVER_OK(s);
+  if (!out) return fail(MSC_EINVAL, "NULL argument");
+  return s->impl->stats(out);
+}
+
+double msc_server_clock(const msc_server* s) { return (s && s->impl) ? s->impl->clock() : -1.0; }
+
+void msc_server_destroy(msc_server* s) { delete s; }
+
+}  // extern "C"

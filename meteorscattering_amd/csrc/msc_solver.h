@@ -71,7 +71,8 @@ struct SolverBase {
   char* blk = nullptr;
   size_t dev_bytes = 0;
   int t_host = 0;
-  int ty = 7, chunk = 16;
+  int ty = 7, chunk = 8;
+  bool tiling_from_env = true;      // MSC_TY / MSC_CHUNK may override ty / chunk
   int cur = 0;                      // kCurDI / kCurLKPoly / kCurLKClosed
   std::vector<uint32_t> tiles_host; // interior tiles, then CPML-boundary tiles
   int n_inner = 0, n_bound = 0;

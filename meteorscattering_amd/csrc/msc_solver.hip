@@ -249,7 +249,7 @@ int make_plan(const msc_params* p, HostPlan& h) {
 // ---------------------------------------------------------------- arena
 
 Arena::~Arena() {
-  if (base_) hipFree(base_);
+  if (base_) (void)hipFree(base_);
 }
 
 int Arena::init(size_t bytes) {
@@ -319,13 +319,13 @@ size_t Arena::largest_free() const {
 // ---------------------------------------------------------------- solver
 
 SolverBase::~SolverBase() {
-  if (stream) hipStreamSynchronize(stream);
+  if (stream) (void)hipStreamSynchronize(stream);
   if (blk) {
     if (arena) arena->release(blk);
-    else hipFree(blk);
+    else (void)hipFree(blk);
   }
-  for (hipEvent_t e : ev) hipEventDestroy(e);
-  if (stream && own_stream) hipStreamDestroy(stream);
+  for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  if (stream && own_stream) (void)hipStreamDestroy(stream);
 }
 
 int SolverBase::sync() {
@@ -352,8 +352,10 @@ template <typename T>
 size_t Solver<T>::layout() {
   const HostPlan& h = plan;
   const int nx = h.nx, ny = h.ny, nz = h.nz, L = h.L;
-  if (const char* s = std::getenv("MSC_TY")) ty = std::atoi(s);
-  if (const char* s = std::getenv("MSC_CHUNK")) chunk = std::max(1, std::atoi(s));
+  if (tiling_from_env) {
+    if (const char* s = std::getenv("MSC_TY")) ty = std::atoi(s);
+    if (const char* s = std::getenv("MSC_CHUNK")) chunk = std::max(1, std::atoi(s));
+  }
   // Tiles: x and y are cut at the CPML slab edges (L+1 and n-L-1), so a tile
   // either meets a slab on some axis (the variant carrying the CPML code) or
   // meets none (the interior variant); z keeps full 63-wide tiles from 0,
@@ -625,7 +627,7 @@ int Solver<T>::step(int n) {
   if (n < 0) return fail(MSC_EINVAL, "nsteps < 0");
   if (t_host + n > plan.T) return fail(MSC_ESTATE, "stepping past tsteps (%d + %d > %d)", t_host, n, plan.T);
   if (timing) {
-    for (hipEvent_t e : ev) hipEventDestroy(e);
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     ev.assign(2 * (size_t)n, nullptr);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   }
@@ -659,7 +661,7 @@ int Solver<T>::get_field(int comp, double* out) {
     for (int c = 0; c < 3; c++) HIPCHK(hipMalloc(&scratch[c], N * sizeof(T)));
     hipError_t e = launch_catchup<T>(sa_dev, cur, scratch[0], scratch[1], scratch[2], stream);
     if (e != hipSuccess) {
-      for (int c = 0; c < 3; c++) hipFree(scratch[c]);
+      for (int c = 0; c < 3; c++) (void)hipFree(scratch[c]);
       return fail(MSC_EHIP, "catch-up kernel: %s", hipGetErrorString(e));
     }
     src = scratch[comp - 6];
@@ -667,7 +669,7 @@ int Solver<T>::get_field(int comp, double* out) {
   hipError_t e = hipMemcpyAsync(tmp.data(), src, N * sizeof(T), hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   for (int c = 0; c < 3; c++)
-    if (scratch[c]) hipFree(scratch[c]);
+    if (scratch[c]) (void)hipFree(scratch[c]);
   if (e != hipSuccess) return fail(MSC_EHIP, "field read-back: %s", hipGetErrorString(e));
   for (size_t c = 0; c < N; c++) out[c] = (double)tmp[c];
   return MSC_OK;
@@ -870,11 +872,11 @@ int msc_stream_copy(size_t bytes, int iters, double* gbps) {
     if (e != hipSuccess) rc = fail(MSC_EHIP, "stream copy: %s", hipGetErrorString(e));
     else *gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
   }
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
-  if (st) hipStreamDestroy(st);
-  if (a) hipFree(a);
-  if (b) hipFree(b);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
   return rc;
 }
 

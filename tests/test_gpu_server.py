@@ -1,0 +1,112 @@
+"""Multi-solve server (msc_server_*): results in a mix equal results alone.
+
+The server advances every resident solve with ganged launches (tiles of
+all solves in one launch, one argument block per solve).  Each solve's
+arithmetic is its single-solve arithmetic, so the NTFF payload and the
+probes must be bit-identical to msc_solver_* on the same inputs, whatever
+else is resident, whichever current model the neighbours use (DI,
+Lee-Kalluri polynomial and closed form are launched as separate groups),
+and whether the arena made the request wait.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import cases
+from meteorscattering_amd.server import Server, footprint
+from meteorscattering_amd.solver import Solver
+
+pytestmark = pytest.mark.gpu
+
+MIX = ["lk_ricker_ez", "di_modgauss_ey", "plate_ricker_45", "lk_nopml_lhcp", "lk_overdense_offset"]
+
+
+def _case(name, **kw):
+    p, wp, nu = cases.case(name)
+    p = dict(p, savefields=0, **kw)
+    return p, wp, nu
+
+
+def _alone(p, wp, nu, precision=32):
+    with Solver(p, wp, nu, precision=precision) as s:
+        s.step(p["tsteps"])
+        s.sync()
+        return s.ntff(), s.probes(), s.device_bytes
+
+
+@pytest.fixture(scope="module")
+def alone():
+    return {n: _alone(*_case(n)) for n in MIX}
+
+
+def test_mix_equals_alone(alone):
+    with Server(precision=32, budget_bytes=4 << 30) as srv:
+        ids = {}
+        for n in MIX:
+            ids[n] = srv.submit(*_case(n))
+        srv.run_until_done()
+        for n in MIX:
+            inf = srv.info(ids[n])
+            assert inf["state_name"] == "done", (n, inf)
+            assert 0 < inf["t_submit"] <= inf["t_start"] <= inf["t_done"]
+            assert inf["t"] == inf["tsteps"]
+            ntff, probes, nbytes = alone[n]
+            assert inf["device_bytes"] >= nbytes
+            np.testing.assert_array_equal(srv.ntff(ids[n]), ntff, err_msg=n)
+            np.testing.assert_array_equal(srv.probes(ids[n]), probes, err_msg=n)
+        st = srv.stats()
+        assert st["arena_used"] == 0
+        # every gang iteration advanced several solves at once
+        assert st["solve_steps"] == sum(cases.case(n)[0]["tsteps"] for n in MIX)
+        assert st["iterations"] < st["solve_steps"]
+
+
+def test_full_arena_queues_in_order_and_rejects_oversize(alone):
+    sizes = {n: footprint(_case(n)[0]) for n in MIX}
+    budget = max(sizes.values()) + min(sizes.values()) // 2  # one resident solve at a time
+    with Server(precision=32, budget_bytes=budget) as srv:
+        big = dict(_case("lk_ricker_ez")[0], nx=120, ny=120, nz=120)
+        bid = srv.submit(big, *_big_maps(big))
+        ids = [srv.submit(*_case(n)) for n in MIX]
+        srv.run_until_done()
+        inf = srv.info(bid)
+        assert inf["state_name"] == "failed" and "arena" in inf["error"]
+        starts = []
+        for n, rid in zip(MIX, ids):
+            inf = srv.info(rid)
+            assert inf["state_name"] == "done"
+            starts.append(inf["t_start"])
+            ntff, probes, _ = alone[n]
+            np.testing.assert_array_equal(srv.ntff(rid), ntff, err_msg=n)
+            np.testing.assert_array_equal(srv.probes(rid), probes, err_msg=n)
+        assert starts == sorted(starts)  # first come, first served
+        # a second round on the same arena reuses the freed blocks
+        rid = srv.submit(*_case("di_modgauss_ey"))
+        srv.run_until_done()
+        np.testing.assert_array_equal(srv.ntff(rid), alone["di_modgauss_ey"][0])
+        srv.release(rid)
+        with pytest.raises(RuntimeError):
+            srv.info(rid)
+
+
+def _big_maps(p):
+    return np.zeros((p["nx"], p["ny"])), np.zeros((p["nx"], p["ny"]))
+
+
+def test_fp64_mix_equals_alone():
+    names = ["lk_ricker_ez", "di_modgauss_ey"]
+    with Server(precision=64, budget_bytes=2 << 30) as srv:
+        ids = [srv.submit(*_case(n)) for n in names]
+        srv.run_until_done()
+        for n, rid in zip(names, ids):
+            ntff, probes, _ = _alone(*_case(n), precision=64)
+            np.testing.assert_array_equal(srv.ntff(rid), ntff, err_msg=n)
+            np.testing.assert_array_equal(srv.probes(rid), probes, err_msg=n)
+
+
+def test_savefields_is_rejected():
+    p, wp, nu = cases.case("lk_ricker_ez")
+    with Server(precision=32, budget_bytes=1 << 30) as srv:
+        with pytest.raises(RuntimeError, match="savefields"):
+            srv.submit(dict(p, savefields=1), wp, nu)
