@@ -92,6 +92,11 @@ typedef struct msc_params {
   int32_t media3d;     /* 1: wp.dat/nu.dat hold [nx][ny][nz], no z-taper */
   double nu_scale_height; /* >0: nu *= exp(-(z - metloc[2]) / H) */
   double src_tau;      /* sourcetype 4 width (s); 0 -> 1/(pi f0) */
+  int32_t j_collocated; /* 1: the Lee-Kalluri current is driven by E at its own
+                          edge; 0 (reference): by the node average of the two
+                          neighbouring edges (meteor-fork.cpp:1577-1579), a
+                          half-cell mismatch that makes the magnetized update
+                          grow without bound */
 } msc_params;
 
 /* Fill `p` with the defaults of setup/meteordefaults.m (plus doB0=0). */
@@ -203,8 +208,10 @@ int msc_stream_copy(size_t bytes, int iters, double* gbps);
  * running the whole time loop.  Here the runs share one GPU:
  *  - msc_server_create reserves one device arena of `budget_bytes`
  *    (0: 90% of the free device memory) and starts a worker thread;
- *  - msc_server_submit queues a solve (inputs copied; savefields must be 0,
- *    the server returns the NTFF payload and the probes);
+ *  - msc_server_submit queues a solve (savefields must be 0: the server
+ *    returns the NTFF payload and the probes).  The parameters are copied;
+ *    wp and nu are read in place by the worker, so they must stay valid
+ *    until the request's state is past MSC_REQ_QUEUED;
  *  - the worker prepares queued solves in submission order, each as soon
  *    as its block fits in the arena (msc_solve_footprint gives its size);
  *  - msc_server_advance runs up to `max_iterations` gang iterations, each

@@ -59,6 +59,7 @@ class MscParams(C.Structure):
         ("media3d", C.c_int32),
         ("nu_scale_height", C.c_double),
         ("src_tau", C.c_double),
+        ("j_collocated", C.c_int32),
     ]
 
 

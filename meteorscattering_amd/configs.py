@@ -96,7 +96,7 @@ def config2(seed: int = 0, tsteps: int = 6000, media3d: bool = True) -> tuple[di
     n0 = 10 ** rng.uniform(11, 14)
     r0 = rng.uniform(0.5, 4.0)
     p = base_params((400, 400, 800), 0.25, tsteps, pmllen=12)
-    p.update(nu_scale_height=7000.0, B0=b_field(60.0, tilt_deg=30.0))
+    p.update(nu_scale_height=7000.0, B0=b_field(60.0, tilt_deg=30.0), j_collocated=1)
     p["_trail"] = dict(n0=n0, r0=r0, nu0=1e5, tilt_deg=30.0)
     if media3d:
         p["media3d"] = 1
@@ -126,8 +126,11 @@ def random_trail(rng: np.random.Generator, nx: int, ny: int, nz: int, dx: float 
     # more steps for bigger grids, as the reference's tfactor rule scales with Nx
     frac = (math.log(ncell) - math.log(64 ** 3)) / (math.log(256 * 256 * 512) - math.log(64 ** 3))
     steps = int(round(steps_range[0] + min(max(frac, 0.0), 1.0) * (steps_range[1] - steps_range[0])))
-    p = base_params((nx, ny, nz), dx, steps)
-    p.update(B0=b_field(bang, tilt_deg=inc), media3d=1, numfiles=1, savefields=0,
+    # dt as setup/meteorrun.m:33-34: Courant 0.8 times the plasma factor below
+    p = base_params((nx, ny, nz), dx, steps, courant=0.8)
+    # j_collocated: the reference's node-averaged Lee-Kalluri drive grows
+    # without bound (see msc_params.j_collocated); these runs use the fix
+    p.update(B0=b_field(bang, tilt_deg=inc), media3d=1, numfiles=1, savefields=0, j_collocated=1,
              nntf=nntf, fntf=list(np.linspace(40e6, 60e6, nntf)) if nntf > 1 else [50e6])
     # keep the explicit plasma coupling stable: the reference's own rule
     # (setup/meteorrun.m:33-34) shrinks dt with the peak plasma frequency
@@ -147,4 +150,18 @@ def config4(seed: int = 0, count: int = 48):
         nx, ny = (int(v) for v in rng.integers(64, 257, size=2))
         nz = int(rng.integers(128, 513))
         out.append(random_trail(rng, nx, ny, nz))
+    return out
+
+
+def config5(seed: int = 0, count: int = 64, nntf: int = 16):
+    """Serving requests: config-4 physics, grids log-uniform in cell count
+    from 64^3 to 256x256x1024 (nx = ny = 64*4^s, nz = 64*16^s, s ~ U[0,1]),
+    16-frequency NTFF (40-60 MHz)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        s = rng.uniform(0.0, 1.0)
+        nxy = int(round(64 * 4 ** s))
+        nz = int(round(64 * 16 ** s))
+        out.append(random_trail(rng, nxy, nxy, nz, nntf=nntf))
     return out

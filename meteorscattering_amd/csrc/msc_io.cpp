@@ -59,7 +59,7 @@ bool spit(const std::string& path, const void* data, size_t n) {
 }
 
 const char kExtMagic[4] = {'M', 'S', 'C', 'X'};
-const int32_t kExtVersion = 1;
+const int32_t kExtVersion = 2;  // v1 lacks j_collocated
 
 }  // namespace
 }  // namespace msc
@@ -118,6 +118,8 @@ int msc_params_validate(const msc_params* p) {
   if (p->nntf < 1 || p->nntf > MSC_MAX_NTF) return fail(MSC_EINVAL, "nntf %d out of range [1,%d]", p->nntf, MSC_MAX_NTF);
   if (p->nprobes < 0 || p->nprobes > MSC_MAX_PROBES)
     return fail(MSC_EINVAL, "nprobes %d out of range [0,%d]", p->nprobes, MSC_MAX_PROBES);
+  if (p->j_collocated != 0 && p->j_collocated != 1)
+    return fail(MSC_EINVAL, "j_collocated must be 0 or 1 (got %d)", p->j_collocated);
   for (int n = 0; n < p->nprobes; ++n)
     if (p->probex[n] < 1 || p->probex[n] >= p->nx || p->probey[n] < 1 || p->probey[n] >= p->ny || p->probez[n] < 1 ||
         p->probez[n] >= p->nz)
@@ -169,8 +171,10 @@ int msc_read_run_dir(const char* dir, msc_params* p, double** wp, double** nu) {
       r.off += 4;
       int32_t ver = 0;
       ok = r.get(&ver, 4);
-      if (!ok || ver != kExtVersion) return fail(MSC_EIO, "inputs.dat: unknown extension record version %d", ver);
+      if (!ok || ver < 1 || ver > kExtVersion)
+        return fail(MSC_EIO, "inputs.dat: unknown extension record version %d", ver);
       ok = r.get(&p->pmllen, 4) && r.get(&p->media3d, 4) && r.get(&p->nu_scale_height, 8) && r.get(&p->src_tau, 8);
+      if (ok && ver >= 2) ok = r.get(&p->j_collocated, 4);
       if (!ok) return fail(MSC_EIO, "inputs.dat: truncated extension record");
     }
   }
@@ -228,11 +232,13 @@ int msc_write_run_dir(const char* dir, const msc_params* p, const double* wp, co
   put(p->probex, 4 * (size_t)p->nprobes); put(p->probey, 4 * (size_t)p->nprobes); put(p->probez, 4 * (size_t)p->nprobes);
   put(&p->nntf, 4); put(p->fntf, 8 * (size_t)p->nntf); put(&p->doplate, 4); put(&p->platesize, 8);
   put(&p->doB0, 4);
-  bool ext = !((p->pmllen == 0 || p->pmllen == 10) && !p->media3d && p->nu_scale_height == 0 && p->src_tau == 0);
+  bool ext = !((p->pmllen == 0 || p->pmllen == 10) && !p->media3d && p->nu_scale_height == 0 && p->src_tau == 0 &&
+               !p->j_collocated);
   if (ext) {
     put(kExtMagic, 4);
     put(&kExtVersion, 4);
     put(&p->pmllen, 4); put(&p->media3d, 4); put(&p->nu_scale_height, 8); put(&p->src_tau, 8);
+    put(&p->j_collocated, 4);
   }
   if (!spit(d + "/inputs.dat", b.data(), b.size())) return fail(MSC_EIO, "cannot write %s/inputs.dat", dir);
   if (!spit(d + "/B0.dat", p->B0, 24)) return fail(MSC_EIO, "cannot write %s/B0.dat", dir);

@@ -34,6 +34,7 @@ struct StepArgs {
   int bx1, bx2, by1, by2, bz1, bz2;     // TF/SF box
   int ssx, ssy, ssz;                    // slice planes
   int dopml, doplate, doB0;
+  int jcol;                             // Lee-Kalluri J driven by collocated E (else node average)
   int plate_lo, plate_up;
   int chunk, tiles_y, tiles_z, nchunks; // launch decomposition
   const int* xstart;  // [nchunks+1] x-chunk starts (aligned to the CPML slab edges)
@@ -67,15 +68,21 @@ struct StepArgs {
   const int* tdev;   // step index
 };
 
-// Per-step accumulation of the near-to-far-field surface DFT.
+// Near-to-far-field surface DFT.  Every step writes the 24 surface
+// sample arrays into slot t % K of a K-step buffer; every K steps (and at
+// the last step) the buffer is contracted with the sin/cos table into the
+// FP64 accumulators: acc[q][2m+c][pt] += sum_k s_k[q][pt] * sc[t0+k][m][c].
 template <typename T>
 struct NtffArgs {
   int nx, ny, nz, M, b;
+  int K, T_steps;           // buffer depth, tsteps
   long long sy, sx;
   T* e[2][3];
   T* h[2][3];
   double* acc;              // 24 arrays, each [M][2][n1*n2]
   long long off[24];        // offsets of the 24 arrays in acc
+  long long uoff[25];       // offsets of the 24 arrays in one sample slot (uoff[24] = U)
+  T* sbuf;                  // [K][U] surface samples
   const double* sincos;     // [tsteps][M][2]
   const int* tdev;
 };
@@ -101,6 +108,8 @@ hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int c
                        const uint32_t* tiles, int n, hipStream_t s);
 template <typename T>
 hipError_t launch_ntff(const NtffArgs<T>* a, const uint2* blocks, int nblocks, hipStream_t s);
+template <typename T>
+hipError_t launch_ntff_contract(const NtffArgs<T>* a, long long U, hipStream_t s);
 template <typename T>
 hipError_t launch_frames(const AuxArgs<T>* a, hipStream_t s);
 template <typename T>

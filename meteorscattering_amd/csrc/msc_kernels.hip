@@ -407,10 +407,14 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         if constexpr (CUR == kCurDI) {
           current_update<T, CUR>(a, qn, qw, jx, jy, jz, ecx, ecy, ecz);
         } else {
-          const T mx = (ecx + exprev) / 2;
-          const T my = (ecy + sE[buf][1][r - 1][lane]) / 2;
-          const T mz = (ecz + sE[buf][2][r][lane - 1]) / 2;
-          current_update<T, CUR>(a, qn, qw, jx, jy, jz, mx, my, mz);
+          if (a.jcol) {
+            current_update<T, CUR>(a, qn, qw, jx, jy, jz, ecx, ecy, ecz);
+          } else {
+            const T mx = (ecx + exprev) / 2;
+            const T my = (ecy + sE[buf][1][r - 1][lane]) / 2;
+            const T mz = (ecz + sE[buf][2][r][lane - 1]) / 2;
+            current_update<T, CUR>(a, qn, qw, jx, jy, jz, mx, my, mz);
+          }
         }
         sto(a.j[0], c, jx);
         sto(a.j[1], c, jy);
@@ -544,7 +548,7 @@ __global__ void current_catchup(const StepArgs<T>* __restrict__ ap, T* ox, T* oy
     const int i = (int)(c / a.sx);
     T jx = a.j[0][c], jy = a.j[1][c], jz = a.j[2][c];
     if (i >= 1 && i < a.nx - 1 && j >= 1 && j < a.ny - 1 && k >= 1 && k < a.nz - 1) {
-      if constexpr (CUR == kCurDI) {
+      if (CUR == kCurDI || a.jcol) {
         current_update<T, CUR>(a, a.nue[c], a.wpe2[c], jx, jy, jz, Ex[c], Ey[c], Ez[c]);
       } else {
         current_update<T, CUR>(a, a.nue[c], a.wpe2[c], jx, jy, jz, (Ex[c] + Ex[c - a.sx]) / 2,
@@ -557,10 +561,10 @@ __global__ void current_catchup(const StepArgs<T>* __restrict__ ap, T* ox, T* oy
   }
 }
 
-// Running DFT of the six NTFF planes (meteor-fork.cpp:971-1023 for the
-// face-averaged H, :1417-1470 for E), FP64 accumulators, both at phase t*dt.
+// Surface samples of the running DFT (meteor-fork.cpp:971-1023 for the
+// face-averaged H, :1417-1470 for E) into slot t % K of the sample buffer.
 template <typename T>
-__global__ void __launch_bounds__(256) ntff_accumulate(const NtffArgs<T>* __restrict__ args,
+__global__ void __launch_bounds__(256) ntff_sample(const NtffArgs<T>* __restrict__ args,
                                                         const uint2* __restrict__ blocks) {
   // blocks[b] = {solve id, face << 29 | block index within the face}
   const uint2 d = blocks[blockIdx.x];
@@ -623,15 +627,60 @@ __global__ void __launch_bounds__(256) ntff_accumulate(const NtffArgs<T>* __rest
       q3 = 0.5 * ((double)Hx[c] + (double)Hx[c - 1]);
       break;
   }
-  const double* sc = a.sincos + (size_t)t * a.M * 2;
-  const double q[4] = {q0, q1, q2, q3};
+  T* slot = a.sbuf + (size_t)(t % a.K) * (size_t)a.uoff[24];
+  slot[a.uoff[4 * face + 0] + pt] = (T)q0;
+  slot[a.uoff[4 * face + 1] + pt] = (T)q1;
+  slot[a.uoff[4 * face + 2] + pt] = (T)q2;
+  slot[a.uoff[4 * face + 3] + pt] = (T)q3;
+}
+
+// FP64 read through the constant address space (uniform table entries go
+// through scalar loads).
+__device__ __forceinline__ double ldc(const double* p, size_t i) {
+  return ((const __attribute__((address_space(4))) double*)p)[i];
+}
+
+// Contraction of the buffered samples at the end of a K-step block (the
+// DFT of meteor-fork.cpp:977-1022, 1417-1470 summed over the block, phase
+// t*dt for both E and H, B2): one thread per sample point, frequencies in
+// groups of 8, FP64 products and sums.  Runs after ntff_sample of step t
+// and before the step counter moves; returns at once unless step t closes
+// a block.
+template <typename T>
+__global__ void __launch_bounds__(256) ntff_contract(const NtffArgs<T>* __restrict__ args) {
+  const ConstArgs<NtffArgs<T>>& a = as_const_space(args);
+  const int t = *a.tdev;
+  if ((t + 1) % a.K != 0 && t + 1 != a.T_steps) return;
+  const long long U = a.uoff[24];
+  const long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= U) return;
+  int q = 0;
+  while (q < 23 && u >= a.uoff[q + 1]) ++q;
+  const long long np = a.uoff[q + 1] - a.uoff[q], pt = u - a.uoff[q];
+  const int kn = t % a.K + 1, t0 = t + 1 - kn, M = a.M;
+  const T* __restrict__ sb = a.sbuf;
+  double* acc = a.acc + a.off[q] + pt;
+  for (int m0 = 0; m0 < M; m0 += 8) {
+    double ss[8], cc[8];
 #pragma unroll
-  for (int n = 0; n < 4; ++n) {
-    double* acc = a.acc + a.off[4 * face + n] + pt;
-    for (int m = 0; m < a.M; ++m) {
-      const double s = sc[2 * m], co = sc[2 * m + 1];
-      acc[(size_t)(2 * m) * npts] += q[n] * s;
-      acc[(size_t)(2 * m + 1) * npts] += q[n] * co;
+    for (int j = 0; j < 8; ++j) ss[j] = cc[j] = 0;
+    for (int k = 0; k < kn; ++k) {
+      const double x = (double)sb[(size_t)k * U + u];
+      const size_t row = (size_t)(t0 + k) * M * 2;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (m0 + j < M) {
+          ss[j] = fma(x, ldc(a.sincos, row + 2 * (m0 + j)), ss[j]);
+          cc[j] = fma(x, ldc(a.sincos, row + 2 * (m0 + j) + 1), cc[j]);
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (m0 + j < M) {
+        acc[(size_t)(2 * (m0 + j)) * np] += ss[j];
+        acc[(size_t)(2 * (m0 + j) + 1) * np] += cc[j];
+      }
     }
   }
 }
@@ -784,7 +833,14 @@ hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int c
 template <typename T>
 hipError_t launch_ntff(const NtffArgs<T>* a, const uint2* blocks, int nblocks, hipStream_t s) {
   if (nblocks <= 0) return hipSuccess;
-  hipLaunchKernelGGL((ntff_accumulate<T>), dim3(nblocks), dim3(kNtffBlock), 0, s, a, blocks);
+  hipLaunchKernelGGL((ntff_sample<T>), dim3(nblocks), dim3(kNtffBlock), 0, s, a, blocks);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ntff_contract(const NtffArgs<T>* a, long long U, hipStream_t s) {
+  if (U <= 0) return hipSuccess;
+  hipLaunchKernelGGL((ntff_contract<T>), dim3((unsigned)((U + 255) / 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -815,6 +871,7 @@ hipError_t launch_catchup(const StepArgs<T>* a, int cur, T* ox, T* oy, T* oz, hi
   template hipError_t launch_step<T>(const StepArgs<T>*, const uint16_t*, int, int, bool, bool, const uint32_t*, int, \
                                      hipStream_t);                                                                 \
   template hipError_t launch_ntff<T>(const NtffArgs<T>*, const uint2*, int, hipStream_t);                          \
+  template hipError_t launch_ntff_contract<T>(const NtffArgs<T>*, long long, hipStream_t);                        \
   template hipError_t launch_frames<T>(const AuxArgs<T>*, hipStream_t);                                            \
   template hipError_t launch_probes<T>(const AuxArgs<T>*, int, hipStream_t);                                       \
   template hipError_t launch_catchup<T>(const StepArgs<T>*, int, T*, T*, T*, hipStream_t);

@@ -55,7 +55,8 @@ struct Server : ServerBase {
   struct Req {
     int64_t id = 0;
     msc_params p{};
-    std::vector<double> wp, nu;
+    const double* wp = nullptr;  // borrowed from the caller until READY
+    const double* nu = nullptr;
     std::unique_ptr<Solver<T>> s;
     int state = MSC_REQ_QUEUED;
     std::atomic<int> t{0};  // steps launched
@@ -184,8 +185,7 @@ struct Server : ServerBase {
         r->state = MSC_REQ_FAILED;
         r->err = msc_last_error();
         r->t_done = now();
-        r->wp.clear(); r->wp.shrink_to_fit();
-        r->nu.clear(); r->nu.shrink_to_fit();
+        r->wp = r->nu = nullptr;
       }
       cv_ready.notify_all();
     }
@@ -204,14 +204,13 @@ struct Server : ServerBase {
       r->s->stream = setup;
       r->s->own_stream = false;
     }
-    int rc = r->s->init(r->wp.data(), r->nu.data());
+    int rc = r->s->init(r->wp, r->nu);
     if (rc == MSC_EBUSY) return rc;
     if (rc) {
       r->s.reset();
       return rc;
     }
-    r->wp.clear(); r->wp.shrink_to_fit();
-    r->nu.clear(); r->nu.shrink_to_fit();
+    r->wp = r->nu = nullptr;
     return MSC_OK;
   }
 
@@ -221,9 +220,8 @@ struct Server : ServerBase {
     if (p->savefields) return fail(MSC_EINVAL, "server solves produce NTFF and probe results only: set savefields = 0");
     std::unique_ptr<Req> r(new Req());
     r->p = *p;
-    const size_t n2 = p->media3d ? (size_t)p->nx * p->ny * p->nz : (size_t)p->nx * p->ny;
-    r->wp.assign(wp, wp + n2);
-    r->nu.assign(nu, nu + n2);
+    r->wp = wp;
+    r->nu = nu;
     r->t_submit = now();
     std::lock_guard<std::mutex> g(mu);
     r->id = next_id++;
@@ -379,6 +377,11 @@ struct Server : ServerBase {
         HIPCHK(launch_step<T>(g.sa, g.sids + o, 7, c, v == 1, false, g.tiles + o, n, run));
       }
     HIPCHK(launch_ntff<T>(g.na, g.nb, n_nb, run));
+    for (size_t q = 0; q < active.size(); q++) {  // solves closing a sample block this step
+      const Solver<T>& s = *active[q]->s;
+      const int t = active[q]->t;
+      if ((t + 1) % s.na.K == 0 || t + 1 == s.plan.T) HIPCHK(launch_ntff_contract<T>(g.na + q, s.na.uoff[24], run));
+    }
     HIPCHK(launch_probes<T>(g.xa, (int)active.size(), run));
     HIPCHK(hipEventRecord(iter_ev[slot], run));
     HIPCHK(hipEventRecord(gang[cur_gang].last_use, run));

@@ -407,12 +407,22 @@ size_t Solver<T>::layout() {
   }
   const size_t M2 = 2 * (size_t)h.M;
   const size_t fb = (size_t)ny * nz, tb = (size_t)nx * nz, lr = (size_t)nx * ny;
-  size_t off = 0;
+  size_t off = 0, uo = 0;
   for (int q = 0; q < 24; q++) {
+    const size_t np = q < 8 ? fb : q < 16 ? tb : lr;
     na.off[q] = (long long)off;
-    off += M2 * (q < 8 ? fb : q < 16 ? tb : lr);
+    na.uoff[q] = (long long)uo;
+    off += M2 * np;
+    uo += np;
   }
+  na.uoff[24] = (long long)uo;
   acc_n = off;
+  // sample-buffer depth: the accumulators' read-modify-write per step
+  // (2 x 2M x 8 B per sample point, once per K steps) stays near a tenth of
+  // the field step's 80 B per cell
+  const double kk = std::ceil(4.0 * h.M * (double)uo / ((double)nx * ny * nz));
+  na.K = (int)std::min(64.0, std::max(4.0, kk));
+  na.T_steps = h.T;
   return carve(nullptr);
 }
 
@@ -468,6 +478,7 @@ size_t Solver<T>::carve(char* base) {
   }
   take(emax, h.nframes);
   take(acc, acc_n);
+  take(na.sbuf, (size_t)na.K * (size_t)na.uoff[24]);
   take(na.sincos, (size_t)h.T * h.M * 2);
   take(probes, (size_t)6 * h.T * std::max(1, h.p.nprobes));
   take(xa.px, MSC_MAX_PROBES);
@@ -583,6 +594,7 @@ int Solver<T>::init(const double* wp, const double* nu) {
   sa.bx1 = L + 10; sa.bx2 = nx - L - 10; sa.by1 = L + 10; sa.by2 = ny - L - 10; sa.bz1 = L + 10; sa.bz2 = nz - L - 10;
   sa.ssx = (nx + 1) / 2; sa.ssy = (ny + 1) / 2; sa.ssz = (nz + 1) / 2;
   sa.dopml = h.p.dopml; sa.doplate = h.p.doplate; sa.doB0 = h.p.doB0;
+  sa.jcol = h.p.j_collocated;
   sa.plate_lo = (int)(sa.ssy - h.p.platesize / 2 / h.dx);  // meteor-fork.cpp:1339-1340
   sa.plate_up = (int)(sa.ssy + h.p.platesize / 2 / h.dx);
   sa.chunk = chunk;
@@ -639,6 +651,7 @@ int Solver<T>::step(int n) {
     HIPCHK(launch_step<T>(sa_dev, nullptr, ty, cur, true, slice, tiles + n_inner, n_bound, stream));
     if (timing) HIPCHK(hipEventRecord(ev[2 * s + 1], stream));
     HIPCHK(launch_ntff<T>(na_dev, ntff_blocks, (int)ntff_host.size(), stream));
+    if ((t_host + 1) % na.K == 0 || t_host + 1 == plan.T) HIPCHK(launch_ntff_contract<T>(na_dev, na.uoff[24], stream));
     if (cad) HIPCHK(launch_frames<T>(xa_dev, stream));
     HIPCHK(launch_probes<T>(xa_dev, 1, stream));
     ++t_host;

@@ -29,7 +29,7 @@ ME = 9.10939e-31
 C_LIGHT = 3.0e8
 
 EXT_MAGIC = b"MSCX"
-EXT_VERSION = 1
+EXT_VERSION = 2  # v1 lacks j_collocated
 
 
 def default_params() -> dict:
@@ -44,7 +44,7 @@ def default_params() -> dict:
         probex=[(nx + 1) // 2], probey=[(ny + 1) // 2], probez=[(nz + 1) // 2],
         nntf=15, fntf=list(np.linspace(0.9, 1.1, 15) * f0), doplate=0, platesize=1.0,
         doB0=0, B0=[0.0, 0.0, 5e-5],
-        pmllen=0, media3d=0, nu_scale_height=0.0, src_tau=0.0,
+        pmllen=0, media3d=0, nu_scale_height=0.0, src_tau=0.0, j_collocated=0,
     )
 
 
@@ -66,7 +66,8 @@ def meteorrun_tsteps(nx: int, dx: float, dt: float, tfactor: float) -> int:
 
 def ext_is_default(p: dict) -> bool:
     return (int(p.get("pmllen", 0)) in (0, 10) and not p.get("media3d", 0)
-            and not p.get("nu_scale_height", 0.0) and not p.get("src_tau", 0.0))
+            and not p.get("nu_scale_height", 0.0) and not p.get("src_tau", 0.0)
+            and not p.get("j_collocated", 0))
 
 
 def encode_inputs_dat(p: dict, with_doB0: bool = True, with_ext: bool | None = None) -> bytes:
@@ -104,6 +105,7 @@ def encode_inputs_dat(p: dict, with_doB0: bool = True, with_ext: bool | None = N
             out.extend(EXT_MAGIC)
             i32(EXT_VERSION); i32(p.get("pmllen", 0)); i32(p.get("media3d", 0))
             f64(p.get("nu_scale_height", 0.0)); f64(p.get("src_tau", 0.0))
+            i32(p.get("j_collocated", 0))
     return bytes(out)
 
 
@@ -133,16 +135,18 @@ def decode_inputs_dat(buf: bytes) -> dict:
     p["fntf"] = list(take(f"<{p['nntf']}d"))
     p["doplate"], = take("<i"); p["platesize"], = take("<d")
     p["doB0"] = 0
-    p.update(pmllen=0, media3d=0, nu_scale_height=0.0, src_tau=0.0)
+    p.update(pmllen=0, media3d=0, nu_scale_height=0.0, src_tau=0.0, j_collocated=0)
     if off + 4 <= len(buf):
         p["doB0"], = take("<i")
         if buf[off:off + 4] == EXT_MAGIC:
             off += 4
             ver, = take("<i")
-            if ver != EXT_VERSION:
+            if ver not in (1, EXT_VERSION):
                 raise ValueError(f"unknown extension record version {ver}")
             p["pmllen"], p["media3d"] = take("<2i")
             p["nu_scale_height"], p["src_tau"] = take("<2d")
+            if ver >= 2:
+                p["j_collocated"], = take("<i")
     return p
 
 

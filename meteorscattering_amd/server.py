@@ -43,6 +43,7 @@ class Server:
         self._h = h
         self.precision = precision
         self.params: dict[int, dict] = {}
+        self._maps: dict[int, tuple] = {}
 
     def submit(self, p: dict, wp: np.ndarray, nu: np.ndarray) -> int:
         s = _abi.params_to_struct(p)
@@ -51,13 +52,22 @@ class Server:
         rid = C.c_int64()
         _check(self.lib.msc_server_submit(self._h, C.byref(s), _abi.dptr(wp), _abi.dptr(nu), C.byref(rid)), self.lib)
         self.params[rid.value] = dict(p)
+        # the worker reads the maps in place until the request is prepared
+        self._maps[rid.value] = (wp, nu)
         return rid.value
 
     def advance(self, max_iterations: int = 1) -> int:
         """Run up to `max_iterations` gang iterations; returns solves completed."""
         n = C.c_int()
         _check(self.lib.msc_server_advance(self._h, int(max_iterations), C.byref(n)), self.lib)
+        for rid in [r for r in self._maps if self._state(r) > _abi.REQ_QUEUED]:
+            del self._maps[rid]
         return n.value
+
+    def _state(self, rid: int) -> int:
+        inf = _abi.RequestInfo()
+        _check(self.lib.msc_server_info(self._h, rid, C.byref(inf)), self.lib)
+        return inf.state
 
     def info(self, rid: int) -> dict:
         inf = _abi.RequestInfo()
@@ -105,6 +115,7 @@ class Server:
         if getattr(self, "_h", None):
             self.lib.msc_server_destroy(self._h)
             self._h = None
+        self._maps = {}
 
     def __enter__(self):
         return self

@@ -820,9 +820,16 @@ static void update_j(orc* o) {
         double A[9], K[9], Em[3];
         double wp = o->wpe[c];
         orc_lk_matrices(o->nue[c], dt, o->wce, A, K);
-        Em[0] = (o->ex[c] + F(o->ex, i - 1, j, k)) / 2;
-        Em[1] = (o->ey[c] + F(o->ey, i, j - 1, k)) / 2;
-        Em[2] = (o->ez[c] + F(o->ez, i, j, k - 1)) / 2;
+        if (o->p.j_collocated) {
+          /* extension: E at J's own edge (removes the half-cell shift) */
+          Em[0] = o->ex[c];
+          Em[1] = o->ey[c];
+          Em[2] = o->ez[c];
+        } else {
+          Em[0] = (o->ex[c] + F(o->ex, i - 1, j, k)) / 2;
+          Em[1] = (o->ey[c] + F(o->ey, i, j - 1, k)) / 2;
+          Em[2] = (o->ez[c] + F(o->ez, i, j, k - 1)) / 2;
+        }
         double jx0 = A[0] * o->jx[c] + A[1] * o->jy[c] + A[2] * o->jz[c] + E0 * wp * wp * (K[0] * Em[0] + K[1] * Em[1] + K[2] * Em[2]);
         double jy0 = A[3] * o->jx[c] + A[4] * o->jy[c] + A[5] * o->jz[c] + E0 * wp * wp * (K[3] * Em[0] + K[4] * Em[1] + K[5] * Em[2]);
         o->jz[c] = A[6] * o->jx[c] + A[7] * o->jy[c] + A[8] * o->jz[c] + E0 * wp * wp * (K[6] * Em[0] + K[7] * Em[1] + K[8] * Em[2]);

@@ -77,7 +77,7 @@ def test_c_reader_reads_python_writer(lib, tmp_path):
 
 def test_c_writer_bytes_equal_python_writer(lib, tmp_path):
     p, wp, nu = _run_case()
-    p = dict(p, pmllen=11, nu_scale_height=7000.0)
+    p = dict(p, pmllen=11, nu_scale_height=7000.0, j_collocated=1)
     d1, d2 = tmp_path / "c", tmp_path / "py"
     d1.mkdir()
     runio.write_run_dir(str(d2), p, wp, nu)
@@ -88,7 +88,22 @@ def test_c_writer_bytes_equal_python_writer(lib, tmp_path):
     for f in ("inputs.dat", "B0.dat", "wp.dat", "nu.dat"):
         assert (d1 / f).read_bytes() == (d2 / f).read_bytes(), f
     q = runio.decode_inputs_dat((d1 / "inputs.dat").read_bytes())
-    assert q["pmllen"] == 11 and q["nu_scale_height"] == 7000.0
+    assert q["pmllen"] == 11 and q["nu_scale_height"] == 7000.0 and q["j_collocated"] == 1
+    # the C reader takes the record back, and still reads a version-1 record
+    s2 = _abi.MscParams()
+    a, b = C.POINTER(C.c_double)(), C.POINTER(C.c_double)()
+    assert lib.msc_read_run_dir(str(d1).encode(), C.byref(s2), C.byref(a), C.byref(b)) == 0
+    assert s2.j_collocated == 1 and s2.pmllen == 11
+    lib.msc_free(a)
+    lib.msc_free(b)
+    v1 = bytearray((d1 / "inputs.dat").read_bytes()[:-4])
+    k = v1.find(runio.EXT_MAGIC)
+    v1[k + 4:k + 8] = struct.pack("<i", 1)
+    (d1 / "inputs.dat").write_bytes(bytes(v1))
+    assert lib.msc_read_run_dir(str(d1).encode(), C.byref(s2), C.byref(a), C.byref(b)) == 0
+    assert s2.j_collocated == 0 and s2.pmllen == 11 and runio.decode_inputs_dat(bytes(v1))["j_collocated"] == 0
+    lib.msc_free(a)
+    lib.msc_free(b)
 
 
 def test_missing_doB0_reads_as_zero(lib, tmp_path):

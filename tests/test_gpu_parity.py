@@ -177,6 +177,19 @@ def test_lee_kalluri_closed_form_variant_matches_oracle(msc, oracle_lib, monkeyp
     assert not rep, "\n".join(rep[:12])
 
 
+@pytest.mark.parametrize("precision,tol", [(64, 1e-10), (32, 2e-5)])
+def test_collocated_current_extension_matches_oracle(msc, oracle_lib, precision, tol):
+    """j_collocated = 1 (E at J's own edge; not in the reference) on a grid
+    with interior tiles, against the oracle's same extension."""
+    Solver, _ = msc
+    p, _, _ = cases.case("lk_ricker_ez")
+    p.update(nx=60, ny=48, nz=140, tsteps=120, j_collocated=1, nprobes=1, probex=[30], probey=[24], probez=[70])
+    wp, nu = cases.trail_maps(p, n0=5e13, r0=1.5, nu0=1e5, centre=(0.5, -0.25))
+    g, o, rep = run_pair(Solver, oracle_lib, p, wp, nu, precision, p["tsteps"], 30, tol, f"jcol{precision}")
+    assert not rep, "\n".join(rep[:12])
+    assert relerr(g.ntff(), o.ntff()) < tol
+
+
 def test_lee_kalluri_zero_collision_and_zero_field_limits(msc, oracle_lib):
     """nu = 0 in part of the grid and B = 0: finite limits (the reference
     divides by zero there), GPU polynomial path against the oracle's limits."""

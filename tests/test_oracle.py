@@ -229,3 +229,34 @@ def test_lk_reduces_to_collisional_decay_without_field(oracle_lib):
     A, K = oracle_lib.lk_matrices(2e6, 1e-9, w)
     np.testing.assert_allclose(A, math.exp(-2e-3) * np.eye(3), rtol=1e-15)
     np.testing.assert_allclose(K, (1 - math.exp(-2e-3)) / 2e6 * np.eye(3), rtol=1e-14)
+
+
+def _trail_run(oracle_lib, j_collocated, doB0, n0=1e13, steps=600):
+    from meteorscattering_amd import configs
+
+    p = configs.base_params((48, 48, 48), 0.5, steps)
+    p.update(numfiles=1, savefields=0, doB0=doB0, media3d=1, j_collocated=j_collocated,
+             B0=configs.b_field(30.0) if doB0 else [0.0, 0.0, 0.0])
+    wp, nu = configs.trail_3d(p, n0, 1.5, 1e4, 20.0)
+    o = oracle_lib.Oracle(p, wp, nu)
+    peaks = []
+    for _ in range(3):
+        o.step(steps // 3)
+        peaks.append(float(np.abs(o.field(0)).max()))
+    o.close()
+    return peaks
+
+
+def test_node_averaged_lee_kalluri_grows_and_collocated_does_not(oracle_lib):
+    """The reference drives the edge-located J with the node average of two
+    edges (meteor-fork.cpp:1577-1579) and feeds it back unaveraged
+    (:1168,1228,1286).  That half-cell shift makes the magnetized update grow
+    without bound (here x1e5 and more per 200 steps at n0 = 1e13), while the
+    same update driven by the collocated E (j_collocated = 1) stays bounded
+    like Young's DI scheme, whose amplitudes it then tracks."""
+    ref = _trail_run(oracle_lib, 0, 1)
+    col = _trail_run(oracle_lib, 1, 1)
+    di = _trail_run(oracle_lib, 0, 0)
+    assert ref[2] > 1e4 * ref[0] and ref[2] > 1e6
+    assert max(col) < 10 * max(di) and col[2] < 3 * col[0]
+    assert max(di) < 10
