@@ -11,6 +11,7 @@ namespace msc {
 constexpr int kLanes = 64;
 constexpr int kOwnZ = kLanes - 1;
 constexpr int kMaxLkDeg = 24;
+constexpr int kLkPolyDeg = 12;  // highest degree the in-kernel Horner takes
 constexpr int kNtffBlock = 256;  // surface points per NTFF workgroup
 
 // current models selected per launch

@@ -88,15 +88,27 @@ __device__ __forceinline__ void current_update(const A& a, T nu_t, T w_t, T& jx,
   } else {
     double Ee1, kP, kI, kX;
     if constexpr (CUR == kCurLKPoly) {
+      // Horner from degree lkdeg (<= kMaxLkDeg): a uniform jump into a
+      // straight run of FMAs (fma(0, x, c) = c starts each chain exactly)
       const double x = nu * a.dt;
-      const int d = a.lkdeg;
-      double pa = a.lkc[0][d], pi = a.lkc[1][d], px = a.lkc[2][d], pp = a.lkc[3][d];
-      for (int n = d - 1; n >= 0; --n) {
-        pa = fma(pa, x, a.lkc[0][n]);
-        pi = fma(pi, x, a.lkc[1][n]);
-        px = fma(px, x, a.lkc[2][n]);
-        pp = fma(pp, x, a.lkc[3][n]);
+      double pa = 0, pi = 0, px = 0, pp = 0;
+#define MSC_HORNER(n)                  \
+  case n:                              \
+    pa = fma(pa, x, a.lkc[0][n]);      \
+    pi = fma(pi, x, a.lkc[1][n]);      \
+    px = fma(px, x, a.lkc[2][n]);      \
+    pp = fma(pp, x, a.lkc[3][n]);      \
+    [[fallthrough]];
+      switch (a.lkdeg) {
+        MSC_HORNER(12) MSC_HORNER(11) MSC_HORNER(10) MSC_HORNER(9) MSC_HORNER(8) MSC_HORNER(7)
+        MSC_HORNER(6) MSC_HORNER(5) MSC_HORNER(4) MSC_HORNER(3) MSC_HORNER(2) MSC_HORNER(1)
+        case 0:
+          pa = fma(pa, x, a.lkc[0][0]);
+          pi = fma(pi, x, a.lkc[1][0]);
+          px = fma(px, x, a.lkc[2][0]);
+          pp = fma(pp, x, a.lkc[3][0]);
       }
+#undef MSC_HORNER
       Ee1 = pa; kI = pi; kX = px; kP = pp;
     } else {
       if (nu > 0 && a.w0 > 0) {
@@ -198,6 +210,16 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   const int istart = ib > 0 ? ib - 1 : 0;
   // owned cells inside the current's domain [1,N-1)^3 (the x test is per plane)
   const bool jdom = own && j >= 1 && j < ny - 1 && k >= 1 && k < nz - 1;
+  // TF/SF face signs of this lane's (j, k), fixed along the x march
+  // (meteor-fork.cpp:808-822, 868-875, 927-934, 1175-1188, 1236-1243,
+  // 1295-1302); the x conditions are uniform per plane.  Each correction is
+  // fma(sign, term, field): exactly field +/- term, or field for sign 0.
+  const bool jin = j >= a.by1 && j < a.by2, kin = k >= a.bz1 && k < a.bz2;
+  const T sHx0 = kin ? (T)((int)(j == a.by1 - 1) - (int)(j == a.by2 - 1)) : (T)0;
+  const T sHx1 = jin ? (T)((int)(k == a.bz2 - 1) - (int)(k == a.bz1 - 1)) : (T)0;
+  const T sEx2 = jin ? (T)((int)(k == a.bz1) - (int)(k == a.bz2)) : (T)0;
+  const T sEx3 = kin ? (T)((int)(j == a.by2) - (int)(j == a.by1)) : (T)0;
+  const T mJK = (jin && kin) ? (T)1 : (T)0;
 
   const int t = *a.tdev;
   const int par = t & 1;
@@ -309,14 +331,8 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         const T dEy = (eykp - ecy) * inv;
         hx = hx - c2h * (dEz - dEy);
         if (i >= a.bx1 && i < a.bx2) {  // TF/SF (:808-822)
-          if (k >= a.bz1 && k < a.bz2) {
-            if (j == a.by1 - 1) hx = hx + tf0;
-            if (j == a.by2 - 1) hx = hx - tf0;
-          }
-          if (j >= a.by1 && j < a.by2) {
-            if (k == a.bz1 - 1) hx = hx - tf1;
-            if (k == a.bz2 - 1) hx = hx + tf1;
-          }
+          hx = fma(sHx0, tf0, hx);
+          hx = fma(sHx1, tf1, hx);
         }
         if (PML) {  // (:690-714, :824-850)
           const int jj = slab_h(j, ny, L);
@@ -341,10 +357,8 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         const T dEx = (exkp - ecx) * inv;
         const T dEz = (enz - ecz) * inv;
         hy = hy - c2h * (dEx - dEz);
-        if (j >= a.by1 && j < a.by2 && k >= a.bz1 && k < a.bz2) {  // (:868-875)
-          if (i == a.bx1 - 1) hy = hy - tfEzB1;
-          if (i == a.bx2 - 1) hy = hy + tfEzB2;
-        }
+        if (i == a.bx1 - 1) hy = fma(-mJK, tfEzB1, hy);  // (:868-875)
+        if (i == a.bx2 - 1) hy = fma(mJK, tfEzB2, hy);
         if (PML) {  // (:718-744, :877-910)
           const int ii = slab_h(i, nx, L);
           if (ii >= 0) {
@@ -368,10 +382,8 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         const T dEy = (eny - ecy) * inv;
         const T dEx = (exjp - ecx) * inv;
         hz = hz - c2h * (dEy - dEx);
-        if (j >= a.by1 && j < a.by2 && k >= a.bz1 && k < a.bz2) {  // (:927-934)
-          if (i == a.bx1 - 1) hz = hz + tfEyB1;
-          if (i == a.bx2 - 1) hz = hz - tfEyB2;
-        }
+        if (i == a.bx1 - 1) hz = fma(mJK, tfEyB1, hz);  // (:927-934)
+        if (i == a.bx2 - 1) hz = fma(-mJK, tfEyB2, hz);
         if (PML) {  // (:747-791, :936-968)
           const int ii = slab_h(i, nx, L);
           if (ii >= 0) {
@@ -428,14 +440,8 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         const T dHy = (hy - sH[1][r][lane - 1]) * inv;
         T ex = c1 * ecx + c2 * (dHz - dHy) - c2 * jx;
         if (i >= a.bx1 && i < a.bx2 - 1) {
-          if (j >= a.by1 && j < a.by2) {
-            if (k == a.bz1) ex = ex + tf2;
-            if (k == a.bz2) ex = ex - tf2;
-          }
-          if (k >= a.bz1 && k < a.bz2) {
-            if (j == a.by1) ex = ex - tf3;
-            if (j == a.by2) ex = ex + tf3;
-          }
+          ex = fma(sEx2, tf2, ex);
+          ex = fma(sEx3, tf3, ex);
         }
         if (PML) {
           const int jj = slab_e(j, ny, L);
@@ -461,10 +467,8 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         const T dHx = (hx - sH[0][r][lane - 1]) * inv;
         const T dHz = (hz - hzprev) * inv;
         ey = c1 * ecy + c2 * (dHx - dHz) - c2 * jy;
-        if (j >= a.by1 && j < a.by2 && k >= a.bz1 && k < a.bz2) {
-          if (i == a.bx1) ey = ey + tfHzB1;
-          if (i == a.bx2 - 1) ey = ey - tfHzB2;
-        }
+        if (i == a.bx1) ey = fma(mJK, tfHzB1, ey);
+        if (i == a.bx2 - 1) ey = fma(-mJK, tfHzB2, ey);
         if (PML) {
           const int ii = slab_e(i, nx, L);
           if (ii >= 0 && j < ny - 1 && k < nz - 1) {
@@ -491,10 +495,8 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         const T dHy = (hy - hyprev) * inv;
         const T dHx = (hx - sH[0][r - 1][lane]) * inv;
         ez = c1 * ecz + c2 * (dHy - dHx) - c2 * jz;
-        if (j >= a.by1 && j < a.by2 && k >= a.bz1 && k < a.bz2) {
-          if (i == a.bx1) ez = ez - tfHyB1;
-          if (i == a.bx2 - 1) ez = ez + tfHyB2;
-        }
+        if (i == a.bx1) ez = fma(-mJK, tfHyB1, ez);
+        if (i == a.bx2 - 1) ez = fma(mJK, tfHyB2, ez);
         if (a.doplate == 1 && i == a.ssx && j >= a.plate_lo && j < a.plate_up && k >= a.plate_lo && k < a.plate_up - 1)
           ez = 0;  // (:1344-1350)
         ezpre = ez;
@@ -826,6 +828,8 @@ hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int c
   switch (ty) {
     case 3: return launch_c<T, 3>(a, sids, cur, pml, slice, tiles, n, s);
     case 7: return launch_c<T, 7>(a, sids, cur, pml, slice, tiles, n, s);
+    case 11: return launch_c<T, 11>(a, sids, cur, pml, slice, tiles, n, s);
+    case 15: return launch_c<T, 15>(a, sids, cur, pml, slice, tiles, n, s);
   }
   return hipErrorInvalidValue;
 }

@@ -170,7 +170,7 @@ static bool plan_lk_poly(const HostPlan& h, double xmax, int& deg, double c[4][k
     term *= xmax / (d + 1);
     if (term < 1e-18) { deg = d; break; }
   }
-  if (deg < 0) return false;
+  if (deg < 0 || deg > kLkPolyDeg) return false;
   if (deg < 1) deg = 1;
   double fact = 1;  // n!
   for (int n = 0; n <= kMaxLkDeg; ++n) {
@@ -500,7 +500,7 @@ int Solver<T>::init(const double* wp, const double* nu) {
   const HostPlan& h = plan;
   const int nx = h.nx, ny = h.ny, nz = h.nz, L = h.L;
   const size_t N = (size_t)nx * ny * nz;
-  if (ty != 3 && ty != 7) return fail(MSC_EINVAL, "MSC_TY must be 3 or 7");
+  if (ty != 3 && ty != 7 && ty != 11 && ty != 15) return fail(MSC_EINVAL, "MSC_TY must be 3, 7, 11 or 15");
   if ((double)N * sizeof(T) >= 4294967296.0)
     return fail(MSC_EINVAL, "grid of %zu cells exceeds the 4 GiB-per-array limit of the 32-bit offset kernel", N);
   const size_t bytes = layout();
