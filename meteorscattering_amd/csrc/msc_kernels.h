@@ -39,7 +39,8 @@ struct StepArgs {
   int plate_lo, plate_up;
   int chunk, tiles_y, tiles_z, nchunks; // launch decomposition
   const int* xstart;  // [nchunks+1] x-chunk starts (aligned to the CPML slab edges)
-  const int* ystart;  // [tiles_y+1] row-tile starts
+  const int* ystart;  // [tiles_y+1] row-tile starts of interior tiles (ty rows)
+  const int* ystart_b;  // row-tile starts of CPML-boundary tiles (ty_b rows)
   const int* zstart;  // [tiles_z+1] z-tile starts
   int cadence;                          // tsteps/numfiles
   long long sy, sx;                     // strides: sy = nz, sx = ny*nz

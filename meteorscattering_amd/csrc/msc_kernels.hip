@@ -199,7 +199,8 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   const int ty = (tile >> 8) & 0xfff;
   const int chunk = tile >> 20;
   const int k0 = a.zstart[tz], k1 = a.zstart[tz + 1];
-  const int j0 = a.ystart[ty], j1 = a.ystart[ty + 1];
+  const int* __restrict__ yst = PML ? a.ystart_b : a.ystart;
+  const int j0 = yst[ty], j1 = yst[ty + 1];
   const int ib = a.xstart[chunk], ie = a.xstart[chunk + 1];
 
   const int nx = a.nx, ny = a.ny, nz = a.nz, L = a.L;

@@ -92,7 +92,10 @@ struct Server : ServerBase {
   std::condition_variable cv_work, cv_ready;
   std::deque<Req*> queue;    // submitted, not yet prepared (FIFO)
   std::vector<Req*> ready;   // prepared, waiting to join the gang
-  std::vector<Req*> active;  // stepping
+  std::vector<Req*> active;  // resident and not finished
+  std::vector<Req*> sel;     // the subset the current gang lists step
+  static constexpr int kTy = 7, kTyB = 7;  // tile rows of every gang solve
+  int fill_tiles = 3072;     // tiles that keep every CU busy (4 rounds of 3 workgroups x 256 CUs)
   std::vector<Req*> draining;  // all steps launched, waiting for completion
   std::map<int64_t, std::unique_ptr<Req>> reqs;
   int64_t next_id = 1;
@@ -131,6 +134,7 @@ struct Server : ServerBase {
     }
     int rc = arena.init(budget);
     if (rc) return rc;
+    if (const char* f = std::getenv("MSC_FILL_TILES")) fill_tiles = std::atoi(f);
     worker = std::thread([this] { work(); });
     return MSC_OK;
   }
@@ -194,7 +198,9 @@ struct Server : ServerBase {
   int prepare(Req* r) {
     if (!r->s) {
       r->s.reset(new Solver<T>());
-      r->s->tiling_from_env = false;  // the gang launches 7-row tiles
+      r->s->tiling_from_env = false;  // the gang launches kTy / kTyB-row tiles
+      r->s->ty = kTy;
+      r->s->ty_b = kTyB;
       int rc = make_plan(&r->p, r->s->plan);
       if (rc) return rc;
       r->bytes = r->s->layout();
@@ -275,10 +281,35 @@ struct Server : ServerBase {
     return MSC_OK;
   }
 
+  // Which resident solves the gang steps: the oldest always (no request
+  // starves), then the others by least remaining work (cells x steps left),
+  // until their tiles fill the device; the rest wait resident.  The GPU
+  // stays full either way, and short requests are not slowed to the pace
+  // of the largest grid in the mix.  fill_tiles <= 0 steps every solve.
+  void select() {
+    sel = active;
+    if (fill_tiles <= 0 || sel.size() <= 1) return;
+    auto left = [](const Req* r) {
+      const HostPlan& h = r->s->plan;
+      return (double)h.nx * h.ny * h.nz * (double)(h.T - r->t);
+    };
+    auto oldest = std::min_element(sel.begin(), sel.end(), [](const Req* a, const Req* b) { return a->id < b->id; });
+    std::iter_swap(sel.begin(), oldest);
+    std::sort(sel.begin() + 1, sel.end(), [&](const Req* a, const Req* b) {
+      const double la = left(a), lb = left(b);
+      return la != lb ? la < lb : a->id < b->id;
+    });
+    size_t tiles = 0, q = 0;
+    for (; q < sel.size() && (q == 0 || tiles < (size_t)fill_tiles); q++) tiles += sel[q]->s->tiles_host.size();
+    sel.resize(q);
+  }
+
   int rebuild() {
     const int gi = cur_gang ^ 1;  // the set the queued launches are not reading
     Gang& g = gang[gi];
     if (g.used) HIPCHK(hipEventSynchronize(g.last_use));
+    select();
+    const std::vector<Req*>& active = sel;
     const size_t ns = active.size();
     if (ns > 65535) return fail(MSC_EINVAL, "more than 65535 resident solves");
     size_t nt = 0, nnb = 0;
@@ -374,21 +405,21 @@ struct Server : ServerBase {
         const int n = n_tiles[c][v];
         if (!n) continue;
         const size_t o = tile_off[c][v];
-        HIPCHK(launch_step<T>(g.sa, g.sids + o, 7, c, v == 1, false, g.tiles + o, n, run));
+        HIPCHK(launch_step<T>(g.sa, g.sids + o, v == 1 ? kTyB : kTy, c, v == 1, false, g.tiles + o, n, run));
       }
     HIPCHK(launch_ntff<T>(g.na, g.nb, n_nb, run));
-    for (size_t q = 0; q < active.size(); q++) {  // solves closing a sample block this step
-      const Solver<T>& s = *active[q]->s;
-      const int t = active[q]->t;
+    for (size_t q = 0; q < sel.size(); q++) {  // solves closing a sample block this step
+      const Solver<T>& s = *sel[q]->s;
+      const int t = sel[q]->t;
       if ((t + 1) % s.na.K == 0 || t + 1 == s.plan.T) HIPCHK(launch_ntff_contract<T>(g.na + q, s.na.uoff[24], run));
     }
-    HIPCHK(launch_probes<T>(g.xa, (int)active.size(), run));
+    HIPCHK(launch_probes<T>(g.xa, (int)sel.size(), run));
     HIPCHK(hipEventRecord(iter_ev[slot], run));
     HIPCHK(hipEventRecord(gang[cur_gang].last_use, run));
     ++iters;
-    steps_done += (long long)active.size();
-    for (size_t q = 0; q < active.size();) {
-      Req* r = active[q];
+    steps_done += (long long)sel.size();
+    for (size_t q = 0; q < sel.size();) {
+      Req* r = sel[q];
       if (++r->t < r->s->plan.T) {
         ++q;
         continue;
@@ -400,7 +431,8 @@ struct Server : ServerBase {
         r->state = MSC_REQ_DRAINING;
       }
       draining.push_back(r);
-      active.erase(active.begin() + q);
+      sel.erase(sel.begin() + q);
+      active.erase(std::find(active.begin(), active.end(), r));
       dirty = true;
     }
     return MSC_OK;

@@ -71,7 +71,7 @@ struct SolverBase {
   char* blk = nullptr;
   size_t dev_bytes = 0;
   int t_host = 0;
-  int ty = 7, chunk = 8;
+  int ty = 7, ty_b = 7, chunk = 8;  // interior / CPML-boundary tile rows, x-chunk planes
   bool tiling_from_env = true;      // MSC_TY / MSC_CHUNK may override ty / chunk
   int cur = 0;                      // kCurDI / kCurLKPoly / kCurLKClosed
   std::vector<uint32_t> tiles_host; // interior tiles, then CPML-boundary tiles
@@ -127,7 +127,7 @@ struct Solver : SolverBase {
 
  private:
   // CPML-aligned tile starts per axis, computed by layout()
-  std::vector<int> xs_, ys_, zs_;
+  std::vector<int> xs_, ys_, ysb_, zs_;
   size_t carve(char* base);
 };
 

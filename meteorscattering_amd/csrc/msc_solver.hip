@@ -354,6 +354,7 @@ size_t Solver<T>::layout() {
   const int nx = h.nx, ny = h.ny, nz = h.nz, L = h.L;
   if (tiling_from_env) {
     if (const char* s = std::getenv("MSC_TY")) ty = std::atoi(s);
+    if (const char* s = std::getenv("MSC_TY_B")) ty_b = std::atoi(s);
     if (const char* s = std::getenv("MSC_CHUNK")) chunk = std::max(1, std::atoi(s));
   }
   // Tiles: x and y are cut at the CPML slab edges (L+1 and n-L-1), so a tile
@@ -378,22 +379,33 @@ size_t Solver<T>::layout() {
     return st;
   };
   xs_ = cut(nx, chunk, true);
+  // interior and boundary tiles cut y differently; both cuttings align at
+  // L+1 and n-L-1, so a row is in a slab zone in both or in neither
   ys_ = cut(ny, ty, true);
+  ysb_ = cut(ny, h.p.dopml ? ty_b : ty, true);
   zs_ = cut(nz, kOwnZ, false);
   const int nch = (int)xs_.size() - 1, tiles_y = (int)ys_.size() - 1, tiles_z = (int)zs_.size() - 1;
+  const int tiles_yb = (int)ysb_.size() - 1;
   auto zone = [&](int lo, int hi, int n) {  // owned range [lo, hi) meets p <= L or p >= n-L-1
     return lo < hi && (lo <= L || hi - 1 >= n - L - 1);
   };
   std::vector<uint32_t> bound;
   tiles_host.clear();
-  if (tiles_z <= 255 && tiles_y <= 4095 && nch <= 4095) {
-    for (int c = 0; c < nch; c++)
+  if (tiles_z <= 255 && tiles_y <= 4095 && tiles_yb <= 4095 && nch <= 4095) {
+    for (int c = 0; c < nch; c++) {
+      const bool xz = zone(xs_[c], xs_[c + 1], nx);
       for (int yy = 0; yy < tiles_y; yy++)
         for (int zz = 0; zz < tiles_z; zz++) {
-          const bool b = h.p.dopml && (zone(xs_[c], xs_[c + 1], nx) || zone(ys_[yy], ys_[yy + 1], ny) ||
-                                       zone(zs_[zz], zs_[zz + 1], nz));
-          (b ? bound : tiles_host).push_back(pack_tile(zz, yy, c));
+          const bool b = h.p.dopml && (xz || zone(ys_[yy], ys_[yy + 1], ny) || zone(zs_[zz], zs_[zz + 1], nz));
+          if (!b) tiles_host.push_back(pack_tile(zz, yy, c));
         }
+      if (!h.p.dopml) continue;
+      for (int yy = 0; yy < tiles_yb; yy++)
+        for (int zz = 0; zz < tiles_z; zz++) {
+          const bool b = xz || zone(ysb_[yy], ysb_[yy + 1], ny) || zone(zs_[zz], zs_[zz + 1], nz);
+          if (b) bound.push_back(pack_tile(zz, yy, c));
+        }
+    }
   }
   n_inner = (int)tiles_host.size();
   n_bound = (int)bound.size();
@@ -418,9 +430,9 @@ size_t Solver<T>::layout() {
   na.uoff[24] = (long long)uo;
   acc_n = off;
   // sample-buffer depth: the accumulators' read-modify-write per step
-  // (2 x 2M x 8 B per sample point, once per K steps) stays near a tenth of
-  // the field step's 80 B per cell
-  const double kk = std::ceil(4.0 * h.M * (double)uo / ((double)nx * ny * nz));
+  // (2 x 2M x 8 B per sample point, once per K steps) stays near 2.5% of the
+  // field step's 80 B per cell
+  const double kk = std::ceil(16.0 * h.M * (double)uo / ((double)nx * ny * nz));
   na.K = (int)std::min(64.0, std::max(4.0, kk));
   na.T_steps = h.T;
   return carve(nullptr);
@@ -487,6 +499,7 @@ size_t Solver<T>::carve(char* base) {
   take(tiles, tiles_host.size());
   take(sa.xstart, xs_.size());
   take(sa.ystart, ys_.size());
+  take(sa.ystart_b, ysb_.size());
   take(sa.zstart, zs_.size());
   take(ntff_blocks, ntff_host.size());
   take(sa_dev, 1);
@@ -500,11 +513,13 @@ int Solver<T>::init(const double* wp, const double* nu) {
   const HostPlan& h = plan;
   const int nx = h.nx, ny = h.ny, nz = h.nz, L = h.L;
   const size_t N = (size_t)nx * ny * nz;
-  if (ty != 3 && ty != 7 && ty != 11 && ty != 15) return fail(MSC_EINVAL, "MSC_TY must be 3, 7, 11 or 15");
+  auto tyok = [](int v) { return v == 3 || v == 7 || v == 11 || v == 15; };
+  if (!tyok(ty) || !tyok(ty_b)) return fail(MSC_EINVAL, "MSC_TY and MSC_TY_B must be 3, 7, 11 or 15");
   if ((double)N * sizeof(T) >= 4294967296.0)
     return fail(MSC_EINVAL, "grid of %zu cells exceeds the 4 GiB-per-array limit of the 32-bit offset kernel", N);
   const size_t bytes = layout();
-  if ((int)zs_.size() - 1 > 255 || (int)ys_.size() - 1 > 4095 || (int)xs_.size() - 1 > 4095)
+  if ((int)zs_.size() - 1 > 255 || (int)ys_.size() - 1 > 4095 || (int)ysb_.size() - 1 > 4095 ||
+      (int)xs_.size() - 1 > 4095)
     return fail(MSC_EINVAL, "grid too large for the tile descriptor");
   if (!stream) {
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -585,6 +600,7 @@ int Solver<T>::init(const double* wp, const double* nu) {
   HIPCHK(hipMemcpyAsync(tiles, tiles_host.data(), tiles_host.size() * 4, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync((void*)sa.xstart, xs_.data(), xs_.size() * 4, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync((void*)sa.ystart, ys_.data(), ys_.size() * 4, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync((void*)sa.ystart_b, ysb_.data(), ysb_.size() * 4, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync((void*)sa.zstart, zs_.data(), zs_.size() * 4, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync(ntff_blocks, ntff_host.data(), ntff_host.size() * sizeof(uint2), hipMemcpyHostToDevice, stream));
 
@@ -648,7 +664,7 @@ int Solver<T>::step(int n) {
     const bool slice = cad && plan.p.savefields;
     if (timing) HIPCHK(hipEventRecord(ev[2 * s], stream));
     HIPCHK(launch_step<T>(sa_dev, nullptr, ty, cur, false, slice, tiles, n_inner, stream));
-    HIPCHK(launch_step<T>(sa_dev, nullptr, ty, cur, true, slice, tiles + n_inner, n_bound, stream));
+    HIPCHK(launch_step<T>(sa_dev, nullptr, ty_b, cur, true, slice, tiles + n_inner, n_bound, stream));
     if (timing) HIPCHK(hipEventRecord(ev[2 * s + 1], stream));
     HIPCHK(launch_ntff<T>(na_dev, ntff_blocks, (int)ntff_host.size(), stream));
     if ((t_host + 1) % na.K == 0 || t_host + 1 == plan.T) HIPCHK(launch_ntff_contract<T>(na_dev, na.uoff[24], stream));

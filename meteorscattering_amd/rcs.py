@@ -118,11 +118,22 @@ def bistatic(ntf: dict, p: dict, theta, phi) -> dict:
     nd, M = theta.size, f.size
     L = np.zeros((nd, M, 3), dtype=np.complex128)
     N = np.zeros((nd, M, 3), dtype=np.complex128)
+    # evenly spaced frequencies: exp(-i k_m p) = exp(-i k_0 p) exp(-i dk p)^m,
+    # one complex product per sample and frequency instead of an exp
+    even = M > 2 and np.allclose(np.diff(k), k[1] - k[0], rtol=1e-12, atol=0)
     for kind, comp, z, w, pos in surface_samples(ntf, p):
         proj = rhat @ pos.T  # [nd, S]
-        ph = np.exp(-1j * k[None, :, None] * proj[:, None, :])  # [nd, M, S]
-        s = np.einsum("dms,ms,s->dm", ph, z, w)
-        (L if kind == "M" else N)[:, :, comp] += s
+        zw = z * w[None, :]  # [M, S]
+        acc = (L if kind == "M" else N)[:, :, comp]
+        if even:
+            ph = np.exp(-1j * k[0] * proj)
+            step = np.exp(-1j * (k[1] - k[0]) * proj)
+            for m in range(M):
+                acc[:, m] += ph @ zw[m]
+                ph *= step
+        else:
+            for m in range(M):
+                acc[:, m] += np.exp(-1j * k[m] * proj) @ zw[m]
     Lth = np.einsum("dmc,dc->dm", L, thhat)
     Lph = np.einsum("dmc,dc->dm", L, phhat)
     Nth = np.einsum("dmc,dc->dm", N, thhat)

@@ -147,15 +147,18 @@ def test_interior_tiles_with_cpml_match_oracle(msc, oracle_lib, monkeypatch):
 
 
 def test_tilings_are_bit_identical(msc, oracle_lib, monkeypatch):
-    """Every cell is computed by the same code whatever the tile shape or
-    x-chunk length (halo cells are recomputed from read-only data)."""
+    """Every cell is computed by the same code whatever the tile heights of
+    the interior and CPML-boundary launches or the x-chunk length (halo
+    cells are recomputed from read-only data).  The grid has interior
+    tiles (z = 160 leaves a middle z-tile clear of both z slabs)."""
     Solver, _ = msc
     p, wp, nu = cases.case("lk_ricker_ez")
-    p.update(tsteps=80, nx=47, ny=45, nz=70)
+    p.update(tsteps=80, nx=58, ny=61, nz=160, probex=[29], probey=[30], probez=[80], nprobes=1)
     wp, nu = cases.trail_maps(p)
     outs = []
-    for ty, chunk in ((7, 32), (3, 5), (3, 1), (7, 47)):
+    for ty, tyb, chunk in ((7, 3, 8), (3, 7, 5), (3, 3, 1), (15, 11, 47), (7, 15, 16)):
         monkeypatch.setenv("MSC_TY", str(ty))
+        monkeypatch.setenv("MSC_TY_B", str(tyb))
         monkeypatch.setenv("MSC_CHUNK", str(chunk))
         g = Solver(p, wp, nu, precision=32)
         g.step(80)

@@ -130,3 +130,24 @@ def test_vacuum_has_no_scattered_field(oracle_lib, sphere_ntf):
     ntf = runio.parse_ntf(runio.encode_ntf(2, 15, p["fntf"], o.ntff()), n, n, n)
     s = rcs.monostatic(ntf, p)["sig"]
     assert np.all(s < 1e-20)
+
+
+def test_even_frequency_recurrence_equals_direct_phases():
+    """bistatic() builds exp(-i k_m r.r') by recurrence for evenly spaced
+    frequencies; one frequency at a time takes the direct exp."""
+    p = configs.base_params((48, 50, 52), 0.5, 10)
+    p.update(nntf=5, fntf=list(np.linspace(40e6, 60e6, 5)))
+    ntf = _fake_ntf(p)
+    rng = np.random.default_rng(3)
+    for name in runio.ntf_face_names():
+        ntf[name] = rng.standard_normal(ntf[name].shape)
+    ntf["Ezdft"] = rng.standard_normal(10)
+    th = np.linspace(0.1, 3.0, 7)
+    ph = np.linspace(-3.0, 3.0, 7)
+    many = rcs.bistatic(ntf, p, th, ph)["sig"]
+    for m in range(5):
+        one = dict(ntf, fntf=ntf["fntf"][m:m + 1], Ezdft=ntf["Ezdft"][2 * m:2 * m + 2],
+                   Eydft=ntf["Eydft"][2 * m:2 * m + 2])
+        for name in runio.ntf_face_names():
+            one[name] = ntf[name][..., 2 * m:2 * m + 2]
+        np.testing.assert_allclose(rcs.bistatic(one, p, th, ph)["sig"][:, 0], many[:, m], rtol=1e-10)
