@@ -261,6 +261,12 @@ int msc_server_get_probes(msc_server* s, int64_t id, double* out, size_t n);
 int msc_server_release(msc_server* s, int64_t id);
 int msc_server_get_stats(msc_server* s, msc_server_stats* out);
 double msc_server_clock(const msc_server* s);
+/* Scheduling knobs.  fill_tiles: each gang iteration steps the oldest
+ * resident solve, then the others by least remaining work (cells x steps
+ * left), until their workgroup tiles reach fill_tiles (<= 0: every resident
+ * solve, the batch setting); default 16384.  use_graphs: replay each gang's
+ * fixed launch sequence as a captured hipGraph (default 1). */
+int msc_server_configure(msc_server* s, int fill_tiles, int use_graphs);
 void msc_server_destroy(msc_server* s);
 
 /*

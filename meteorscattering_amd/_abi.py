@@ -189,6 +189,7 @@ def _declare(lib: C.CDLL) -> None:
         "msc_server_release": (C.c_int, [vp, C.c_int64]),
         "msc_server_get_stats": (C.c_int, [vp, P(ServerStats)]),
         "msc_server_clock": (C.c_double, [vp]),
+        "msc_server_configure": (C.c_int, [vp, C.c_int, C.c_int]),
         "msc_server_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():

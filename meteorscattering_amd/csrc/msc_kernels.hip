@@ -646,13 +646,13 @@ __device__ __forceinline__ double ldc(const double* p, size_t i) {
 // Contraction of the buffered samples at the end of a K-step block (the
 // DFT of meteor-fork.cpp:977-1022, 1417-1470 summed over the block, phase
 // t*dt for both E and H, B2): one thread per sample point, frequencies in
-// groups of 8, FP64 products and sums.  Runs after ntff_sample of step t
-// and before the step counter moves; returns at once unless step t closes
-// a block.
+// groups of 8, FP64 products and sums.  Runs after the step counter moved
+// past step t (so it can follow a captured step sequence); returns at once
+// unless step t closes a block.
 template <typename T>
 __global__ void __launch_bounds__(256) ntff_contract(const NtffArgs<T>* __restrict__ args) {
   const ConstArgs<NtffArgs<T>>& a = as_const_space(args);
-  const int t = *a.tdev;
+  const int t = *a.tdev - 1;
   if ((t + 1) % a.K != 0 && t + 1 != a.T_steps) return;
   const long long U = a.uoff[24];
   const long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -48,6 +48,7 @@ struct ServerBase {
   virtual int release(int64_t id) = 0;
   virtual int stats(msc_server_stats* out) = 0;
   virtual double clock() const = 0;
+  virtual int configure(int fill_tiles, int use_graphs) = 0;
 };
 
 template <typename T>
@@ -81,6 +82,9 @@ struct Server : ServerBase {
     size_t host_bytes = 0;
     hipEvent_t last_use = nullptr;
     bool used = false;
+    // one captured gang iteration (fused steps, NTFF samples, probes/tick)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
   };
 
   int device = 0;
@@ -95,7 +99,8 @@ struct Server : ServerBase {
   std::vector<Req*> active;  // resident and not finished
   std::vector<Req*> sel;     // the subset the current gang lists step
   static constexpr int kTy = 7, kTyB = 7;  // tile rows of every gang solve
-  int fill_tiles = 3072;     // tiles that keep every CU busy (4 rounds of 3 workgroups x 256 CUs)
+  int fill_tiles = 16384;   // ~20 rounds of 3 workgroups on each of the 256 CUs
+  bool use_graphs = true;     // tiles that keep every CU busy (4 rounds of 3 workgroups x 256 CUs)
   std::vector<Req*> draining;  // all steps launched, waiting for completion
   std::map<int64_t, std::unique_ptr<Req>> reqs;
   int64_t next_id = 1;
@@ -120,6 +125,12 @@ struct Server : ServerBase {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   double clock() const override { return now(); }
+  int configure(int fill, int graphs) override {
+    fill_tiles = fill;
+    use_graphs = graphs != 0;
+    dirty = true;
+    return MSC_OK;
+  }
 
   int init(size_t budget) {
     HIPCHK(hipGetDevice(&device));
@@ -135,6 +146,7 @@ struct Server : ServerBase {
     int rc = arena.init(budget);
     if (rc) return rc;
     if (const char* f = std::getenv("MSC_FILL_TILES")) fill_tiles = std::atoi(f);
+    if (const char* f = std::getenv("MSC_GRAPHS")) use_graphs = std::atoi(f) != 0;
     worker = std::thread([this] { work(); });
     return MSC_OK;
   }
@@ -156,6 +168,8 @@ struct Server : ServerBase {
       hipFree(g.sa); hipFree(g.na); hipFree(g.xa); hipFree(g.tiles); hipFree(g.sids); hipFree(g.nb);
       if (g.host) hipHostFree(g.host);
       if (g.last_use) hipEventDestroy(g.last_use);
+      if (g.exec) hipGraphExecDestroy(g.exec);
+      if (g.graph) hipGraphDestroy(g.graph);
     }
     for (auto& e : iter_ev)
       if (e) hipEventDestroy(e);
@@ -363,6 +377,43 @@ struct Server : ServerBase {
     cur_gang = gi;
     g.used = true;
     dirty = false;
+    return use_graphs ? capture(g) : MSC_OK;
+  }
+
+  // The fixed part of an iteration: the fused-step launches per current
+  // model and variant, the NTFF samples and the probe/tick launch.  The
+  // kernels read each solve's step from device memory, so the same launch
+  // sequence (or its captured graph) serves every iteration until the gang
+  // changes.
+  hipError_t launch_fixed(const Gang& g) {
+    hipError_t e = hipSuccess;
+    for (int c = 0; c < 3 && e == hipSuccess; c++)
+      for (int v = 0; v < 2 && e == hipSuccess; v++) {
+        const int n = n_tiles[c][v];
+        if (!n) continue;
+        const size_t o = tile_off[c][v];
+        e = launch_step<T>(g.sa, g.sids + o, v == 1 ? kTyB : kTy, c, v == 1, false, g.tiles + o, n, run);
+      }
+    if (e == hipSuccess) e = launch_ntff<T>(g.na, g.nb, n_nb, run);
+    if (e == hipSuccess) e = launch_probes<T>(g.xa, (int)sel.size(), run);
+    return e;
+  }
+
+  int capture(Gang& g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    g.exec = nullptr;
+    g.graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(run, hipStreamCaptureModeThreadLocal));
+    const hipError_t e = launch_fixed(g);
+    hipGraph_t gr = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(run, &gr);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return fail(MSC_EHIP, "gang capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    g.graph = gr;
+    HIPCHK(hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0));
     return MSC_OK;
   }
 
@@ -400,20 +451,13 @@ struct Server : ServerBase {
     const Gang& g = gang[cur_gang];
     const int slot = (int)(iters % kDepth);
     if (iters >= kDepth) HIPCHK(hipEventSynchronize(iter_ev[slot]));
-    for (int c = 0; c < 3; c++)
-      for (int v = 0; v < 2; v++) {
-        const int n = n_tiles[c][v];
-        if (!n) continue;
-        const size_t o = tile_off[c][v];
-        HIPCHK(launch_step<T>(g.sa, g.sids + o, v == 1 ? kTyB : kTy, c, v == 1, false, g.tiles + o, n, run));
-      }
-    HIPCHK(launch_ntff<T>(g.na, g.nb, n_nb, run));
-    for (size_t q = 0; q < sel.size(); q++) {  // solves closing a sample block this step
+    if (g.exec) HIPCHK(hipGraphLaunch(g.exec, run));
+    else HIPCHK(launch_fixed(g));
+    for (size_t q = 0; q < sel.size(); q++) {  // solves whose step closes a sample block
       const Solver<T>& s = *sel[q]->s;
       const int t = sel[q]->t;
       if ((t + 1) % s.na.K == 0 || t + 1 == s.plan.T) HIPCHK(launch_ntff_contract<T>(g.na + q, s.na.uoff[24], run));
     }
-    HIPCHK(launch_probes<T>(g.xa, (int)sel.size(), run));
     HIPCHK(hipEventRecord(iter_ev[slot], run));
     HIPCHK(hipEventRecord(gang[cur_gang].last_use, run));
     ++iters;
@@ -622,6 +666,11 @@ int msc_server_get_stats(msc_server* s, msc_server_stats* out) {
 }
 
 double msc_server_clock(const msc_server* s) { return (s && s->impl) ? s->impl->clock() : -1.0; }
+
+int msc_server_configure(msc_server* s, int fill_tiles, int use_graphs) {
+  SERVER_OK(s);
+  return s->impl->configure(fill_tiles, use_graphs);
+}
 
 void msc_server_destroy(msc_server* s) { delete s; }
 

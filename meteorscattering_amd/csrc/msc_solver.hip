@@ -667,9 +667,9 @@ int Solver<T>::step(int n) {
     HIPCHK(launch_step<T>(sa_dev, nullptr, ty_b, cur, true, slice, tiles + n_inner, n_bound, stream));
     if (timing) HIPCHK(hipEventRecord(ev[2 * s + 1], stream));
     HIPCHK(launch_ntff<T>(na_dev, ntff_blocks, (int)ntff_host.size(), stream));
-    if ((t_host + 1) % na.K == 0 || t_host + 1 == plan.T) HIPCHK(launch_ntff_contract<T>(na_dev, na.uoff[24], stream));
     if (cad) HIPCHK(launch_frames<T>(xa_dev, stream));
     HIPCHK(launch_probes<T>(xa_dev, 1, stream));
+    if ((t_host + 1) % na.K == 0 || t_host + 1 == plan.T) HIPCHK(launch_ntff_contract<T>(na_dev, na.uoff[24], stream));
     ++t_host;
   }
   return MSC_OK;

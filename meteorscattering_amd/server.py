@@ -36,11 +36,15 @@ def ntff_doubles(p: dict) -> int:
 class Server:
     """Many solves on one GPU; see include/msc.h for the contract."""
 
-    def __init__(self, precision: int = 32, budget_bytes: int = 0):
+    def __init__(self, precision: int = 32, budget_bytes: int = 0, fill_tiles: int | None = None,
+                 graphs: bool = True):
         self.lib = _abi.load_library()
         h = C.c_void_p()
         _check(self.lib.msc_server_create(precision, int(budget_bytes), C.byref(h)), self.lib)
         self._h = h
+        if fill_tiles is not None or not graphs:
+            _check(self.lib.msc_server_configure(h, 16384 if fill_tiles is None else int(fill_tiles), int(graphs)),
+                   self.lib)
         self.precision = precision
         self.params: dict[int, dict] = {}
         self._maps: dict[int, tuple] = {}

@@ -34,9 +34,10 @@ def run_serial(trails, precision: int = 32) -> dict:
 
 
 def run_batch(trails, precision: int = 32, budget: int = 0, chunk: int = 4) -> dict:
-    """Every trail submitted at once to one server; wall time to the last result."""
+    """Every trail submitted at once to one server that steps every resident
+    solve each iteration (throughput setting); wall time to the last result."""
     t0 = time.perf_counter()
-    with Server(precision=precision, budget_bytes=budget) as srv:
+    with Server(precision=precision, budget_bytes=budget, fill_tiles=0) as srv:
         ids = [srv.submit(p, wp, nu) for p, wp, nu in trails]
         srv.run_until_done(chunk)
         wall = time.perf_counter() - t0

@@ -110,3 +110,19 @@ def test_savefields_is_rejected():
     with Server(precision=32, budget_bytes=1 << 30) as srv:
         with pytest.raises(RuntimeError, match="savefields"):
             srv.submit(dict(p, savefields=1), wp, nu)
+
+
+@pytest.mark.parametrize("fill,graphs", [(1, True), (0, False), (400, True)])
+def test_scheduling_settings_do_not_change_results(alone, fill, graphs):
+    """fill 1 steps one solve at a time (the oldest), fill 0 every resident
+    solve, fill 400 a few small ones at a time with pauses and resumptions;
+    with and without captured graphs the results are the single-solve bits."""
+    with Server(precision=32, budget_bytes=4 << 30, fill_tiles=fill, graphs=graphs) as srv:
+        ids = {n: srv.submit(*_case(n)) for n in MIX}
+        srv.run_until_done()
+        for n in MIX:
+            np.testing.assert_array_equal(srv.ntff(ids[n]), alone[n][0], err_msg=n)
+            np.testing.assert_array_equal(srv.probes(ids[n]), alone[n][1], err_msg=n)
+        if fill == 1:
+            st = srv.stats()
+            assert st["iterations"] == st["solve_steps"]
