@@ -17,6 +17,16 @@ constexpr int kNtffBlock = 256;  // surface points per NTFF workgroup
 // current models selected per launch
 enum { kCurDI = 0, kCurLKPoly = 1, kCurLKClosed = 2 };
 
+// CPML slab axes a fused-step variant carries code for: a tile whose owned
+// cells meet the slabs of one axis only takes that axis's variant, a tile
+// meeting several takes the full one
+enum { kPmlX = 1, kPmlY = 2, kPmlZ = 4, kPmlAll = 7 };
+// the tile lists of one solve, in this order, and the variant of each
+enum { kListInterior = 0, kListX, kListY, kListZ, kListAll, kNumLists };
+inline int list_pm(int l) {
+  return l == kListX ? kPmlX : l == kListY ? kPmlY : l == kListZ ? kPmlZ : l == kListAll ? kPmlAll : 0;
+}
+
 // packed tile descriptor: z-tile (8 bits) | y-tile (12 bits) | x-chunk (12 bits),
 // indices into StepArgs::zstart / ystart / xstart
 __host__ __device__ inline uint32_t pack_tile(int tz, int ty, int ch) {
@@ -106,7 +116,7 @@ struct AuxArgs {
 };
 
 template <typename T>
-hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int cur, bool pml, bool slice,
+hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int cur, int pm, bool slice,
                        const uint32_t* tiles, int n, hipStream_t s);
 template <typename T>
 hipError_t launch_ntff(const NtffArgs<T>* a, const uint2* blocks, int nblocks, hipStream_t s);

@@ -88,27 +88,17 @@ __device__ __forceinline__ void current_update(const A& a, T nu_t, T w_t, T& jx,
   } else {
     double Ee1, kP, kI, kX;
     if constexpr (CUR == kCurLKPoly) {
-      // Horner from degree lkdeg (<= kMaxLkDeg): a uniform jump into a
-      // straight run of FMAs (fma(0, x, c) = c starts each chain exactly)
+      // Horner from degree lkdeg (a loop: unrolling it would hoist every
+      // coefficient of every degree into registers across the x march)
       const double x = nu * a.dt;
-      double pa = 0, pi = 0, px = 0, pp = 0;
-#define MSC_HORNER(n)                  \
-  case n:                              \
-    pa = fma(pa, x, a.lkc[0][n]);      \
-    pi = fma(pi, x, a.lkc[1][n]);      \
-    px = fma(px, x, a.lkc[2][n]);      \
-    pp = fma(pp, x, a.lkc[3][n]);      \
-    [[fallthrough]];
-      switch (a.lkdeg) {
-        MSC_HORNER(12) MSC_HORNER(11) MSC_HORNER(10) MSC_HORNER(9) MSC_HORNER(8) MSC_HORNER(7)
-        MSC_HORNER(6) MSC_HORNER(5) MSC_HORNER(4) MSC_HORNER(3) MSC_HORNER(2) MSC_HORNER(1)
-        case 0:
-          pa = fma(pa, x, a.lkc[0][0]);
-          pi = fma(pi, x, a.lkc[1][0]);
-          px = fma(px, x, a.lkc[2][0]);
-          pp = fma(pp, x, a.lkc[3][0]);
+      const int d = a.lkdeg;
+      double pa = a.lkc[0][d], pi = a.lkc[1][d], px = a.lkc[2][d], pp = a.lkc[3][d];
+      for (int n = d - 1; n >= 0; --n) {
+        pa = fma(pa, x, a.lkc[0][n]);
+        pi = fma(pi, x, a.lkc[1][n]);
+        px = fma(px, x, a.lkc[2][n]);
+        pp = fma(pp, x, a.lkc[3][n]);
       }
-#undef MSC_HORNER
       Ee1 = pa; kI = pi; kX = px; kP = pp;
     } else {
       if (nu > 0 && a.w0 > 0) {
@@ -177,7 +167,7 @@ __device__ __forceinline__ bool in_sphere(const A& a, int i, int j, int k) {
   return sqrt(__dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z))) < a.rsph;
 }
 
-template <typename T, int TY, int CUR, bool PML, bool SLICE>
+template <typename T, int TY, int CUR, int PM, bool SLICE>
 __global__ void __launch_bounds__((TY + 1) * kLanes)
 fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ tiles,
           const uint16_t* __restrict__ sids) {
@@ -199,7 +189,7 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   const int ty = (tile >> 8) & 0xfff;
   const int chunk = tile >> 20;
   const int k0 = a.zstart[tz], k1 = a.zstart[tz + 1];
-  const int* __restrict__ yst = PML ? a.ystart_b : a.ystart;
+  const int* __restrict__ yst = PM ? a.ystart_b : a.ystart;
   const int j0 = yst[ty], j1 = yst[ty + 1];
   const int ib = a.xstart[chunk], ie = a.xstart[chunk + 1];
 
@@ -335,20 +325,24 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
           hx = fma(sHx0, tf0, hx);
           hx = fma(sHx1, tf1, hx);
         }
-        if (PML) {  // (:690-714, :824-850)
-          const int jj = slab_h(j, ny, L);
-          if (jj >= 0) {
-            const uint32_t q = (((uint32_t)i * 2 * L + jj) * nz + k) * W;
-            const T p = a.bh[jj] * ldo(a.psih[par][0], q) + a.ah[jj] * dEz;
-            if (wr) sto(a.psih[par ^ 1][0], q, p);
-            hx = hx - c2h * p;
+        if constexpr (PM != 0) {  // (:690-714, :824-850)
+          if constexpr ((PM & 2) != 0) {
+            const int jj = slab_h(j, ny, L);
+            if (jj >= 0) {
+              const uint32_t q = (((uint32_t)i * 2 * L + jj) * nz + k) * W;
+              const T p = a.bh[jj] * ldo(a.psih[par][0], q) + a.ah[jj] * dEz;
+              if (wr) sto(a.psih[par ^ 1][0], q, p);
+              hx = hx - c2h * p;
+            }
           }
-          const int kk = slab_h(k, nz, L);
-          if (kk >= 0) {
-            const uint32_t q = (((uint32_t)i * ny + j) * 2 * L + kk) * W;
-            const T p = a.bh[kk] * ldo(a.psih[par][1], q) + a.ah[kk] * dEy;
-            if (wr) sto(a.psih[par ^ 1][1], q, p);
-            hx = hx + c2h * p;
+          if constexpr ((PM & 4) != 0) {
+            const int kk = slab_h(k, nz, L);
+            if (kk >= 0) {
+              const uint32_t q = (((uint32_t)i * ny + j) * 2 * L + kk) * W;
+              const T p = a.bh[kk] * ldo(a.psih[par][1], q) + a.ah[kk] * dEy;
+              if (wr) sto(a.psih[par ^ 1][1], q, p);
+              hx = hx + c2h * p;
+            }
           }
         }
         if (wr) sto(Hxn, c, hx);
@@ -360,20 +354,24 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         hy = hy - c2h * (dEx - dEz);
         if (i == a.bx1 - 1) hy = fma(-mJK, tfEzB1, hy);  // (:868-875)
         if (i == a.bx2 - 1) hy = fma(mJK, tfEzB2, hy);
-        if (PML) {  // (:718-744, :877-910)
-          const int ii = slab_h(i, nx, L);
-          if (ii >= 0) {
-            const uint32_t q = (((uint32_t)ii * ny + j) * nz + k) * W;
-            const T p = a.bh[ii] * ldo(a.psih[par][2], q) + a.ah[ii] * dEz;
-            if (wr) sto(a.psih[par ^ 1][2], q, p);
-            hy = hy + c2h * p;
+        if constexpr (PM != 0) {  // (:718-744, :877-910)
+          if constexpr ((PM & 1) != 0) {
+            const int ii = slab_h(i, nx, L);
+            if (ii >= 0) {
+              const uint32_t q = (((uint32_t)ii * ny + j) * nz + k) * W;
+              const T p = a.bh[ii] * ldo(a.psih[par][2], q) + a.ah[ii] * dEz;
+              if (wr) sto(a.psih[par ^ 1][2], q, p);
+              hy = hy + c2h * p;
+            }
           }
-          const int kk = slab_h(k, nz, L);
-          if (kk >= 0) {
-            const uint32_t q = (((uint32_t)i * ny + j) * 2 * L + kk) * W;
-            const T p = a.bh[kk] * ldo(a.psih[par][3], q) + a.ah[kk] * dEx;
-            if (wr) sto(a.psih[par ^ 1][3], q, p);
-            hy = hy - c2h * p;
+          if constexpr ((PM & 4) != 0) {
+            const int kk = slab_h(k, nz, L);
+            if (kk >= 0) {
+              const uint32_t q = (((uint32_t)i * ny + j) * 2 * L + kk) * W;
+              const T p = a.bh[kk] * ldo(a.psih[par][3], q) + a.ah[kk] * dEx;
+              if (wr) sto(a.psih[par ^ 1][3], q, p);
+              hy = hy - c2h * p;
+            }
           }
         }
         if (wr) sto(Hyn, c, hy);
@@ -385,20 +383,24 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         hz = hz - c2h * (dEy - dEx);
         if (i == a.bx1 - 1) hz = fma(mJK, tfEyB1, hz);  // (:927-934)
         if (i == a.bx2 - 1) hz = fma(-mJK, tfEyB2, hz);
-        if (PML) {  // (:747-791, :936-968)
-          const int ii = slab_h(i, nx, L);
-          if (ii >= 0) {
-            const uint32_t q = (((uint32_t)ii * ny + j) * nz + k) * W;
-            const T p = a.bh[ii] * ldo(a.psih[par][4], q) + a.ah[ii] * dEy;
-            if (wr) sto(a.psih[par ^ 1][4], q, p);
-            hz = hz - c2h * p;
+        if constexpr (PM != 0) {  // (:747-791, :936-968)
+          if constexpr ((PM & 1) != 0) {
+            const int ii = slab_h(i, nx, L);
+            if (ii >= 0) {
+              const uint32_t q = (((uint32_t)ii * ny + j) * nz + k) * W;
+              const T p = a.bh[ii] * ldo(a.psih[par][4], q) + a.ah[ii] * dEy;
+              if (wr) sto(a.psih[par ^ 1][4], q, p);
+              hz = hz - c2h * p;
+            }
           }
-          const int jj = slab_h(j, ny, L);
-          if (jj >= 0) {
-            const uint32_t q = (((uint32_t)i * 2 * L + jj) * nz + k) * W;
-            const T p = a.bh[jj] * ldo(a.psih[par][5], q) + a.ah[jj] * dEx;
-            if (wr) sto(a.psih[par ^ 1][5], q, p);
-            hz = hz + c2h * p;
+          if constexpr ((PM & 2) != 0) {
+            const int jj = slab_h(j, ny, L);
+            if (jj >= 0) {
+              const uint32_t q = (((uint32_t)i * 2 * L + jj) * nz + k) * W;
+              const T p = a.bh[jj] * ldo(a.psih[par][5], q) + a.ah[jj] * dEx;
+              if (wr) sto(a.psih[par ^ 1][5], q, p);
+              hz = hz + c2h * p;
+            }
           }
         }
         if (wr) sto(Hzn, c, hz);
@@ -444,20 +446,24 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
           ex = fma(sEx2, tf2, ex);
           ex = fma(sEx3, tf3, ex);
         }
-        if (PML) {
-          const int jj = slab_e(j, ny, L);
-          if (jj >= 0) {
-            const uint32_t q = (((uint32_t)i * 2 * L + jj) * nz + k) * W;
-            const T p = a.be[jj] * ldo(a.psie[0], q) + a.ae[jj] * dHz;
-            sto(a.psie[0], q, p);
-            ex = ex + c2e0 * p;
+        if constexpr (PM != 0) {
+          if constexpr ((PM & 2) != 0) {
+            const int jj = slab_e(j, ny, L);
+            if (jj >= 0) {
+              const uint32_t q = (((uint32_t)i * 2 * L + jj) * nz + k) * W;
+              const T p = a.be[jj] * ldo(a.psie[0], q) + a.ae[jj] * dHz;
+              sto(a.psie[0], q, p);
+              ex = ex + c2e0 * p;
+            }
           }
-          const int kk = slab_e(k, nz, L);
-          if (kk >= 0) {
-            const uint32_t q = (((uint32_t)i * ny + j) * 2 * L + kk) * W;
-            const T p = a.be[kk] * ldo(a.psie[1], q) + a.ae[kk] * dHy;
-            sto(a.psie[1], q, p);
-            ex = ex - c2e0 * p;
+          if constexpr ((PM & 4) != 0) {
+            const int kk = slab_e(k, nz, L);
+            if (kk >= 0) {
+              const uint32_t q = (((uint32_t)i * ny + j) * 2 * L + kk) * W;
+              const T p = a.be[kk] * ldo(a.psie[1], q) + a.ae[kk] * dHy;
+              sto(a.psie[1], q, p);
+              ex = ex - c2e0 * p;
+            }
           }
         }
         sto(Exn, c, ex);
@@ -470,20 +476,24 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         ey = c1 * ecy + c2 * (dHx - dHz) - c2 * jy;
         if (i == a.bx1) ey = fma(mJK, tfHzB1, ey);
         if (i == a.bx2 - 1) ey = fma(-mJK, tfHzB2, ey);
-        if (PML) {
-          const int ii = slab_e(i, nx, L);
-          if (ii >= 0 && j < ny - 1 && k < nz - 1) {
-            const uint32_t q = (((uint32_t)ii * ny + j) * nz + k) * W;
-            const T p = a.be[ii] * ldo(a.psie[2], q) + a.ae[ii] * dHz;
-            sto(a.psie[2], q, p);
-            ey = ey - c2e0 * p;
+        if constexpr (PM != 0) {
+          if constexpr ((PM & 1) != 0) {
+            const int ii = slab_e(i, nx, L);
+            if (ii >= 0 && j < ny - 1 && k < nz - 1) {
+              const uint32_t q = (((uint32_t)ii * ny + j) * nz + k) * W;
+              const T p = a.be[ii] * ldo(a.psie[2], q) + a.ae[ii] * dHz;
+              sto(a.psie[2], q, p);
+              ey = ey - c2e0 * p;
+            }
           }
-          const int kk = slab_e(k, nz, L);
-          if (kk >= 0 && j < ny - 1) {
-            const uint32_t q = (((uint32_t)i * ny + j) * 2 * L + kk) * W;
-            const T p = a.be[kk] * ldo(a.psie[3], q) + a.ae[kk] * dHx;
-            sto(a.psie[3], q, p);
-            ey = ey + c2e0 * p;
+          if constexpr ((PM & 4) != 0) {
+            const int kk = slab_e(k, nz, L);
+            if (kk >= 0 && j < ny - 1) {
+              const uint32_t q = (((uint32_t)i * ny + j) * 2 * L + kk) * W;
+              const T p = a.be[kk] * ldo(a.psie[3], q) + a.ae[kk] * dHx;
+              sto(a.psie[3], q, p);
+              ey = ey + c2e0 * p;
+            }
           }
         }
         if (a.doplate == 1 && i == a.ssx && j >= a.plate_lo && j < a.plate_up - 1 && k >= a.plate_lo && k < a.plate_up)
@@ -501,20 +511,24 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         if (a.doplate == 1 && i == a.ssx && j >= a.plate_lo && j < a.plate_up && k >= a.plate_lo && k < a.plate_up - 1)
           ez = 0;  // (:1344-1350)
         ezpre = ez;
-        if (PML) {
-          const int ii = slab_e(i, nx, L);
-          if (ii >= 0 && j < ny - 1 && k < nz - 1) {
-            const uint32_t q = (((uint32_t)ii * ny + j) * nz + k) * W;
-            const T p = a.be[ii] * ldo(a.psie[4], q) + a.ae[ii] * dHy;
-            sto(a.psie[4], q, p);
-            ez = ez + c2e0 * p;
+        if constexpr (PM != 0) {
+          if constexpr ((PM & 1) != 0) {
+            const int ii = slab_e(i, nx, L);
+            if (ii >= 0 && j < ny - 1 && k < nz - 1) {
+              const uint32_t q = (((uint32_t)ii * ny + j) * nz + k) * W;
+              const T p = a.be[ii] * ldo(a.psie[4], q) + a.ae[ii] * dHy;
+              sto(a.psie[4], q, p);
+              ez = ez + c2e0 * p;
+            }
           }
-          const int jj = slab_e(j, ny, L);
-          if (jj >= 0 && k < nz - 1) {
-            const uint32_t q = (((uint32_t)i * 2 * L + jj) * nz + k) * W;
-            const T p = a.be[jj] * ldo(a.psie[5], q) + a.ae[jj] * dHx;
-            sto(a.psie[5], q, p);
-            ez = ez - c2e0 * p;
+          if constexpr ((PM & 2) != 0) {
+            const int jj = slab_e(j, ny, L);
+            if (jj >= 0 && k < nz - 1) {
+              const uint32_t q = (((uint32_t)i * 2 * L + jj) * nz + k) * W;
+              const T p = a.be[jj] * ldo(a.psie[5], q) + a.ae[jj] * dHx;
+              sto(a.psie[5], q, p);
+              ez = ez - c2e0 * p;
+            }
           }
         }
         sto(Ezn, c, ez);
@@ -793,44 +807,55 @@ hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStrea
 
 // ---------------------------------------------------------------- launchers
 
-template <typename T, int TY, int CUR, bool PML, bool SLICE>
+template <typename T, int TY, int CUR, int PM, bool SLICE>
 static hipError_t launch_v(const StepArgs<T>* a, const uint16_t* sids, const uint32_t* tiles, int n, hipStream_t s) {
-  hipLaunchKernelGGL((fdtd_step<T, TY, CUR, PML, SLICE>), dim3(n), dim3((TY + 1) * kLanes), 0, s, a, tiles, sids);
+  hipLaunchKernelGGL((fdtd_step<T, TY, CUR, PM, SLICE>), dim3(n), dim3((TY + 1) * kLanes), 0, s, a, tiles, sids);
   return hipGetLastError();
 }
 
+template <typename T, int TY, int CUR, int PM>
+static hipError_t launch_s(const StepArgs<T>* a, const uint16_t* sids, bool slice, const uint32_t* tiles, int n,
+                           hipStream_t s) {
+  return slice ? launch_v<T, TY, CUR, PM, true>(a, sids, tiles, n, s) : launch_v<T, TY, CUR, PM, false>(a, sids, tiles, n, s);
+}
+
 template <typename T, int TY, int CUR>
-static hipError_t launch_ps(const StepArgs<T>* a, const uint16_t* sids, bool pml, bool slice, const uint32_t* tiles,
+static hipError_t launch_ps(const StepArgs<T>* a, const uint16_t* sids, int pm, bool slice, const uint32_t* tiles,
                             int n, hipStream_t s) {
-  if (pml)
-    return slice ? launch_v<T, TY, CUR, true, true>(a, sids, tiles, n, s) : launch_v<T, TY, CUR, true, false>(a, sids, tiles, n, s);
-  return slice ? launch_v<T, TY, CUR, false, true>(a, sids, tiles, n, s) : launch_v<T, TY, CUR, false, false>(a, sids, tiles, n, s);
+  switch (pm) {
+    case 0: return launch_s<T, TY, CUR, 0>(a, sids, slice, tiles, n, s);
+    case kPmlX: return launch_s<T, TY, CUR, kPmlX>(a, sids, slice, tiles, n, s);
+    case kPmlY: return launch_s<T, TY, CUR, kPmlY>(a, sids, slice, tiles, n, s);
+    case kPmlZ: return launch_s<T, TY, CUR, kPmlZ>(a, sids, slice, tiles, n, s);
+    case kPmlAll: return launch_s<T, TY, CUR, kPmlAll>(a, sids, slice, tiles, n, s);
+  }
+  return hipErrorInvalidValue;
 }
 
 template <typename T, int TY>
-static hipError_t launch_c(const StepArgs<T>* a, const uint16_t* sids, int cur, bool pml, bool slice,
+static hipError_t launch_c(const StepArgs<T>* a, const uint16_t* sids, int cur, int pm, bool slice,
                            const uint32_t* tiles, int n, hipStream_t s) {
   switch (cur) {
-    case kCurDI: return launch_ps<T, TY, kCurDI>(a, sids, pml, slice, tiles, n, s);
-    case kCurLKPoly: return launch_ps<T, TY, kCurLKPoly>(a, sids, pml, slice, tiles, n, s);
-    case kCurLKClosed: return launch_ps<T, TY, kCurLKClosed>(a, sids, pml, slice, tiles, n, s);
+    case kCurDI: return launch_ps<T, TY, kCurDI>(a, sids, pm, slice, tiles, n, s);
+    case kCurLKPoly: return launch_ps<T, TY, kCurLKPoly>(a, sids, pm, slice, tiles, n, s);
+    case kCurLKClosed: return launch_ps<T, TY, kCurLKClosed>(a, sids, pm, slice, tiles, n, s);
   }
   return hipErrorInvalidValue;
 }
 
 // One launch of the fused step over `n` tiles of the list `tiles`, tile b
 // belonging to solve sids[b] of the argument array `a` (sids == nullptr:
-// every tile is solve 0); `pml` selects the variant carrying the CPML code
-// (tiles that touch a slab), `slice` the one capturing E slices.
+// every tile is solve 0); `pm` is the set of CPML slab axes the variant
+// carries code for (kPmlX | kPmlY | kPmlZ, 0 = interior), `slice` selects
+// the variant capturing E slices.
 template <typename T>
-hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int cur, bool pml, bool slice,
+hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int cur, int pm, bool slice,
                        const uint32_t* tiles, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   switch (ty) {
-    case 3: return launch_c<T, 3>(a, sids, cur, pml, slice, tiles, n, s);
-    case 7: return launch_c<T, 7>(a, sids, cur, pml, slice, tiles, n, s);
-    case 11: return launch_c<T, 11>(a, sids, cur, pml, slice, tiles, n, s);
-    case 15: return launch_c<T, 15>(a, sids, cur, pml, slice, tiles, n, s);
+    case 3: return launch_c<T, 3>(a, sids, cur, pm, slice, tiles, n, s);
+    case 7: return launch_c<T, 7>(a, sids, cur, pm, slice, tiles, n, s);
+    case 15: return launch_c<T, 15>(a, sids, cur, pm, slice, tiles, n, s);
   }
   return hipErrorInvalidValue;
 }
@@ -873,7 +898,7 @@ hipError_t launch_catchup(const StepArgs<T>* a, int cur, T* ox, T* oy, T* oz, hi
 }
 
 #define MSC_INST(T)                                                                                                \
-  template hipError_t launch_step<T>(const StepArgs<T>*, const uint16_t*, int, int, bool, bool, const uint32_t*, int, \
+  template hipError_t launch_step<T>(const StepArgs<T>*, const uint16_t*, int, int, int, bool, const uint32_t*, int, \
                                      hipStream_t);                                                                 \
   template hipError_t launch_ntff<T>(const NtffArgs<T>*, const uint2*, int, hipStream_t);                          \
   template hipError_t launch_ntff_contract<T>(const NtffArgs<T>*, long long, hipStream_t);                        \

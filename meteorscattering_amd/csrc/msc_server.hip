@@ -112,8 +112,8 @@ struct Server : ServerBase {
   int cur_gang = 0;
   bool dirty = true;
   // launch lists of the current gang: per current model, [interior, boundary]
-  int n_tiles[3][2] = {};
-  size_t tile_off[3][2] = {};
+  int n_tiles[3][kNumLists] = {};
+  size_t tile_off[3][kNumLists] = {};
   int n_nb = 0;
   // run-ahead limit: at most kDepth iterations queued on the run stream
   static constexpr int kDepth = 4;
@@ -348,12 +348,12 @@ struct Server : ServerBase {
     }
     size_t o = 0;
     for (int c = 0; c < 3; c++)
-      for (int v = 0; v < 2; v++) {
+      for (int v = 0; v < kNumLists; v++) {
         tile_off[c][v] = o;
         for (size_t q = 0; q < ns; q++) {
           const Solver<T>& s = *active[q]->s;
           if (s.cur != c) continue;
-          const size_t b = v ? s.n_inner : 0, e = v ? s.n_inner + s.n_bound : s.n_inner;
+          const size_t b = s.list_off[v], e = s.list_off[v + 1];
           for (size_t k = b; k < e; k++) {
             htiles[o] = s.tiles_host[k];
             hsids[o] = (uint16_t)q;
@@ -388,11 +388,11 @@ struct Server : ServerBase {
   hipError_t launch_fixed(const Gang& g) {
     hipError_t e = hipSuccess;
     for (int c = 0; c < 3 && e == hipSuccess; c++)
-      for (int v = 0; v < 2 && e == hipSuccess; v++) {
+      for (int v = 0; v < kNumLists && e == hipSuccess; v++) {
         const int n = n_tiles[c][v];
         if (!n) continue;
         const size_t o = tile_off[c][v];
-        e = launch_step<T>(g.sa, g.sids + o, v == 1 ? kTyB : kTy, c, v == 1, false, g.tiles + o, n, run);
+        e = launch_step<T>(g.sa, g.sids + o, v == kListInterior ? kTy : kTyB, c, list_pm(v), false, g.tiles + o, n, run);
       }
     if (e == hipSuccess) e = launch_ntff<T>(g.na, g.nb, n_nb, run);
     if (e == hipSuccess) e = launch_probes<T>(g.xa, (int)sel.size(), run);

@@ -74,8 +74,8 @@ struct SolverBase {
   int ty = 7, ty_b = 7, chunk = 8;  // interior / CPML-boundary tile rows, x-chunk planes
   bool tiling_from_env = true;      // MSC_TY / MSC_CHUNK may override ty / chunk
   int cur = 0;                      // kCurDI / kCurLKPoly / kCurLKClosed
-  std::vector<uint32_t> tiles_host; // interior tiles, then CPML-boundary tiles
-  int n_inner = 0, n_bound = 0;
+  std::vector<uint32_t> tiles_host; // the kNumLists tile lists, one after another
+  int list_off[kNumLists + 1] = {};  // list l is tiles_host[list_off[l], list_off[l+1])
   std::vector<uint2> ntff_host;     // NTFF workgroup descriptors (solve id 0)
   bool timing = false;
   std::vector<hipEvent_t> ev;       // pairs around the step kernels of the last step() call

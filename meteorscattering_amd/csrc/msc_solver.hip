@@ -389,7 +389,9 @@ size_t Solver<T>::layout() {
   auto zone = [&](int lo, int hi, int n) {  // owned range [lo, hi) meets p <= L or p >= n-L-1
     return lo < hi && (lo <= L || hi - 1 >= n - L - 1);
   };
-  std::vector<uint32_t> bound;
+  // interior tiles (ty rows) and CPML tiles (ty_b rows), the latter by the
+  // set of slab axes their owned cells meet
+  std::vector<uint32_t> lists[kNumLists];
   tiles_host.clear();
   if (tiles_z <= 255 && tiles_y <= 4095 && tiles_yb <= 4095 && nch <= 4095) {
     for (int c = 0; c < nch; c++) {
@@ -397,19 +399,24 @@ size_t Solver<T>::layout() {
       for (int yy = 0; yy < tiles_y; yy++)
         for (int zz = 0; zz < tiles_z; zz++) {
           const bool b = h.p.dopml && (xz || zone(ys_[yy], ys_[yy + 1], ny) || zone(zs_[zz], zs_[zz + 1], nz));
-          if (!b) tiles_host.push_back(pack_tile(zz, yy, c));
+          if (!b) lists[kListInterior].push_back(pack_tile(zz, yy, c));
         }
       if (!h.p.dopml) continue;
       for (int yy = 0; yy < tiles_yb; yy++)
         for (int zz = 0; zz < tiles_z; zz++) {
-          const bool b = xz || zone(ysb_[yy], ysb_[yy + 1], ny) || zone(zs_[zz], zs_[zz + 1], nz);
-          if (b) bound.push_back(pack_tile(zz, yy, c));
+          const int m = (xz ? kPmlX : 0) | (zone(ysb_[yy], ysb_[yy + 1], ny) ? kPmlY : 0) |
+                        (zone(zs_[zz], zs_[zz + 1], nz) ? kPmlZ : 0);
+          if (!m) continue;
+          const int l = m == kPmlX ? kListX : m == kPmlY ? kListY : m == kPmlZ ? kListZ : kListAll;
+          lists[l].push_back(pack_tile(zz, yy, c));
         }
     }
   }
-  n_inner = (int)tiles_host.size();
-  n_bound = (int)bound.size();
-  tiles_host.insert(tiles_host.end(), bound.begin(), bound.end());
+  for (int l = 0; l < kNumLists; l++) {
+    list_off[l] = (int)tiles_host.size();
+    tiles_host.insert(tiles_host.end(), lists[l].begin(), lists[l].end());
+  }
+  list_off[kNumLists] = (int)tiles_host.size();
   // NTFF: faces x (ny*nz points), y (nx*nz), z (nx*ny), two each
   ntff_host.clear();
   for (int face = 0; face < 6; face++) {
@@ -513,8 +520,8 @@ int Solver<T>::init(const double* wp, const double* nu) {
   const HostPlan& h = plan;
   const int nx = h.nx, ny = h.ny, nz = h.nz, L = h.L;
   const size_t N = (size_t)nx * ny * nz;
-  auto tyok = [](int v) { return v == 3 || v == 7 || v == 11 || v == 15; };
-  if (!tyok(ty) || !tyok(ty_b)) return fail(MSC_EINVAL, "MSC_TY and MSC_TY_B must be 3, 7, 11 or 15");
+  auto tyok = [](int v) { return v == 3 || v == 7 || v == 15; };
+  if (!tyok(ty) || !tyok(ty_b)) return fail(MSC_EINVAL, "MSC_TY and MSC_TY_B must be 3, 7 or 15");
   if ((double)N * sizeof(T) >= 4294967296.0)
     return fail(MSC_EINVAL, "grid of %zu cells exceeds the 4 GiB-per-array limit of the 32-bit offset kernel", N);
   const size_t bytes = layout();
@@ -663,8 +670,9 @@ int Solver<T>::step(int n) {
     const bool cad = (t_host % plan.cadence) == 0;
     const bool slice = cad && plan.p.savefields;
     if (timing) HIPCHK(hipEventRecord(ev[2 * s], stream));
-    HIPCHK(launch_step<T>(sa_dev, nullptr, ty, cur, false, slice, tiles, n_inner, stream));
-    HIPCHK(launch_step<T>(sa_dev, nullptr, ty_b, cur, true, slice, tiles + n_inner, n_bound, stream));
+    for (int l = 0; l < kNumLists; l++)
+      HIPCHK(launch_step<T>(sa_dev, nullptr, l == kListInterior ? ty : ty_b, cur, list_pm(l), slice,
+                            tiles + list_off[l], list_off[l + 1] - list_off[l], stream));
     if (timing) HIPCHK(hipEventRecord(ev[2 * s + 1], stream));
     HIPCHK(launch_ntff<T>(na_dev, ntff_blocks, (int)ntff_host.size(), stream));
     if (cad) HIPCHK(launch_frames<T>(xa_dev, stream));
