@@ -67,9 +67,13 @@ struct StepArgs {
   int lkdeg;
   double lkc[4][kMaxLkDeg + 1];
   // device pointers
-  T* e[2][3];        // ping-pong E: [parity][x,y,z]
-  T* h[2][3];        // ping-pong H
-  T* j[3];           // J, updated in place (deferred one step)
+  // Field sets rotate through three buffers: after step t, E and H are in
+  // set t % 3 and the (deferred) J^(t-1) in set (t - 1) % 3; step t reads
+  // set t % 3 and writes set (t + 1) % 3, so a kernel advancing two steps
+  // at once can write step t + 2 while step t is still being read.
+  T* e[3][3];        // E: [set][x,y,z]
+  T* h[3][3];        // H
+  T* j[3][3];        // J
   const T* wpe2;     // eps0 * wpe^2 per cell
   const T* nue;      // collision frequency per cell
   T* psih[2][6];     // ping-pong psi for H: hxy hxz hyx hyz hzx hzy
@@ -89,8 +93,8 @@ struct NtffArgs {
   int nx, ny, nz, M, b;
   int K, T_steps;           // buffer depth, tsteps
   long long sy, sx;
-  T* e[2][3];
-  T* h[2][3];
+  T* e[3][3];
+  T* h[3][3];
   double* acc;              // 24 arrays, each [M][2][n1*n2]
   long long off[24];        // offsets of the 24 arrays in acc
   long long uoff[25];       // offsets of the 24 arrays in one sample slot (uoff[24] = U)
@@ -103,8 +107,8 @@ template <typename T>
 struct AuxArgs {
   int nx, ny, nz, nprobes, tsteps, cadence, nframes;
   long long sy, sx;
-  T* e[2][3];
-  T* h[2][3];
+  T* e[3][3];
+  T* h[3][3];
   const int* px; const int* py; const int* pz;
   double* probes;           // [6][tsteps][nprobes]
   int ssx, ssy, ssz;

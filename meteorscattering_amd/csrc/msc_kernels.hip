@@ -213,19 +213,26 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   const T mJK = (jin && kin) ? (T)1 : (T)0;
 
   const int t = *a.tdev;
-  const int par = t & 1;
-  const T* __restrict__ Exo = a.e[par][0];
-  const T* __restrict__ Eyo = a.e[par][1];
-  const T* __restrict__ Ezo = a.e[par][2];
-  const T* __restrict__ Hxo = a.h[par][0];
-  const T* __restrict__ Hyo = a.h[par][1];
-  const T* __restrict__ Hzo = a.h[par][2];
-  T* __restrict__ Exn = a.e[par ^ 1][0];
-  T* __restrict__ Eyn = a.e[par ^ 1][1];
-  T* __restrict__ Ezn = a.e[par ^ 1][2];
-  T* __restrict__ Hxn = a.h[par ^ 1][0];
-  T* __restrict__ Hyn = a.h[par ^ 1][1];
-  T* __restrict__ Hzn = a.h[par ^ 1][2];
+  const int par = t & 1;  // psi_H ping-pong
+  const int si = t % 3, so = (t + 1) % 3, sj = (t + 2) % 3;  // field sets: read, write, J^(t-1)
+  const T* __restrict__ Exo = a.e[si][0];
+  const T* __restrict__ Eyo = a.e[si][1];
+  const T* __restrict__ Ezo = a.e[si][2];
+  const T* __restrict__ Hxo = a.h[si][0];
+  const T* __restrict__ Hyo = a.h[si][1];
+  const T* __restrict__ Hzo = a.h[si][2];
+  T* __restrict__ Exn = a.e[so][0];
+  T* __restrict__ Eyn = a.e[so][1];
+  T* __restrict__ Ezn = a.e[so][2];
+  T* __restrict__ Hxn = a.h[so][0];
+  T* __restrict__ Hyn = a.h[so][1];
+  T* __restrict__ Hzn = a.h[so][2];
+  const T* __restrict__ Jxo = a.j[sj][0];
+  const T* __restrict__ Jyo = a.j[sj][1];
+  const T* __restrict__ Jzo = a.j[sj][2];
+  T* __restrict__ Jxn = a.j[si][0];
+  T* __restrict__ Jyn = a.j[si][1];
+  T* __restrict__ Jzn = a.j[si][2];
   const T* __restrict__ inc = a.inc + (size_t)t * 4 * nx;
   const T c2h = a.c2h, c2e0 = a.c2e0, inv = a.inv_d;
   // TF/SF values at fixed planes (meteor-fork.cpp:870-872, 929-931, 1238-1240, 1297-1299)
@@ -299,7 +306,7 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
       }
       if (valid) { nhx = ldo(Hxo, cn); nhy = ldo(Hyo, cn); nhz = ldo(Hzo, cn); }
       if (jdom && i + 1 >= 1 && i + 1 < nx - 1) {
-        nqx = ldo(a.j[0], cn); nqy = ldo(a.j[1], cn); nqz = ldo(a.j[2], cn);
+        nqx = ldo(Jxo, cn); nqy = ldo(Jyo, cn); nqz = ldo(Jzo, cn);
         nqw = ldo(a.wpe2, cn); nqn = ldo(a.nue, cn);
       }
       nt0 = inc[i + 1]; nt1 = inc[nx + i + 1]; nt2 = inc[2 * nx + i + 1]; nt3 = inc[3 * nx + i + 1];
@@ -431,9 +438,9 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
             current_update<T, CUR>(a, qn, qw, jx, jy, jz, mx, my, mz);
           }
         }
-        sto(a.j[0], c, jx);
-        sto(a.j[1], c, jy);
-        sto(a.j[2], c, jz);
+        sto(Jxn, c, jx);
+        sto(Jyn, c, jy);
+        sto(Jzn, c, jz);
       }
       T c1 = 1, c2 = c2e0;
       if (a.doplate == 2 && in_sphere(a, i, j, k)) { c1 = a.c1s; c2 = a.c2s; }
@@ -556,14 +563,15 @@ __global__ void current_catchup(const StepArgs<T>* __restrict__ ap, T* ox, T* oy
   const ConstArgs<StepArgs<T>>& a = as_const_space(ap);
   const long long n = (long long)a.nx * a.ny * a.nz;
   const int t = *a.tdev;
-  const T* Ex = a.e[t & 1][0];
-  const T* Ey = a.e[t & 1][1];
-  const T* Ez = a.e[t & 1][2];
+  const T* Ex = a.e[t % 3][0];
+  const T* Ey = a.e[t % 3][1];
+  const T* Ez = a.e[t % 3][2];
+  const int sj = (t + 2) % 3;  // J^(t-1)
   for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
     const int k = (int)(c % a.nz);
     const int j = (int)((c / a.nz) % a.ny);
     const int i = (int)(c / a.sx);
-    T jx = a.j[0][c], jy = a.j[1][c], jz = a.j[2][c];
+    T jx = a.j[sj][0][c], jy = a.j[sj][1][c], jz = a.j[sj][2][c];
     if (i >= 1 && i < a.nx - 1 && j >= 1 && j < a.ny - 1 && k >= 1 && k < a.nz - 1) {
       if (CUR == kCurDI || a.jcol) {
         current_update<T, CUR>(a, a.nue[c], a.wpe2[c], jx, jy, jz, Ex[c], Ey[c], Ez[c]);
@@ -597,7 +605,7 @@ __global__ void __launch_bounds__(256) ntff_sample(const NtffArgs<T>* __restrict
   if (pt >= npts) return;
   const int u = (int)(pt / n2), v = (int)(pt % n2);
   const int t = *a.tdev;
-  const int par = (t & 1) ^ 1;  // buffers this step wrote
+  const int par = (t + 1) % 3;  // the field set this step wrote
   const T* Ex = a.e[par][0];
   const T* Ey = a.e[par][1];
   const T* Ez = a.e[par][2];
@@ -711,7 +719,7 @@ __global__ void __launch_bounds__(256) cadence_frames(const AuxArgs<T>* __restri
   if (t % a.cadence) return;
   const int f = t / a.cadence;
   if (f >= a.nframes) return;
-  const int par = (t & 1) ^ 1;
+  const int par = (t + 1) % 3;  // the field set this step wrote
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const long long gsz = (long long)gridDim.x * blockDim.x;
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -770,7 +778,7 @@ template <typename T>
 __global__ void __launch_bounds__(64) probes_and_tick(const AuxArgs<T>* __restrict__ args) {
   const ConstArgs<AuxArgs<T>>& a = as_const_space(args + blockIdx.x);
   const int t = *a.tdev;
-  const int par = (t & 1) ^ 1;
+  const int par = (t + 1) % 3;  // the field set this step wrote
   const T* Ex = a.e[par][0];
   const T* Ey = a.e[par][1];
   const T* Ez = a.e[par][2];

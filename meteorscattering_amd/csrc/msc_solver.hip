@@ -460,12 +460,12 @@ size_t Solver<T>::carve(char* base) {
     total = o + std::max<size_t>(n, 1) * sizeof(U);
     ptr = base ? reinterpret_cast<P>(base + o) : nullptr;
   };
-  for (int q = 0; q < 2; q++)
+  for (int q = 0; q < 3; q++)
     for (int c = 0; c < 3; c++) {
       take(sa.e[q][c], N);
       take(sa.h[q][c], N);
+      take(sa.j[q][c], N);
     }
-  for (int c = 0; c < 3; c++) take(sa.j[c], N);
   take(sa.wpe2, N);
   take(sa.nue, N);
   // psi slabs: H ping-pong, E in place
@@ -637,14 +637,14 @@ int Solver<T>::init(const double* wp, const double* nu) {
 
   na.nx = nx; na.ny = ny; na.nz = nz; na.M = h.M; na.b = L + 5;
   na.sy = sa.sy; na.sx = sa.sx;
-  for (int q = 0; q < 2; q++)
+  for (int q = 0; q < 3; q++)
     for (int c = 0; c < 3; c++) { na.e[q][c] = sa.e[q][c]; na.h[q][c] = sa.h[q][c]; }
   na.acc = acc; na.tdev = tdev;
 
   xa.nx = nx; xa.ny = ny; xa.nz = nz; xa.nprobes = h.p.nprobes; xa.tsteps = h.T;
   xa.cadence = h.cadence; xa.nframes = h.nframes;
   xa.sy = sa.sy; xa.sx = sa.sx;
-  for (int q = 0; q < 2; q++)
+  for (int q = 0; q < 3; q++)
     for (int c = 0; c < 3; c++) { xa.e[q][c] = sa.e[q][c]; xa.h[q][c] = sa.h[q][c]; }
   xa.probes = probes;
   xa.ssx = sa.ssx; xa.ssy = sa.ssy; xa.ssz = sa.ssz;
@@ -688,7 +688,7 @@ int Solver<T>::get_field(int comp, double* out) {
   if (comp < 0 || comp > 8) return fail(MSC_EINVAL, "component %d out of range", comp);
   const size_t N = (size_t)plan.nx * plan.ny * plan.nz;
   std::vector<T> tmp(N);
-  const int par = t_host & 1;
+  const int par = t_host % 3;
   const T* src;
   T* scratch[3] = {nullptr, nullptr, nullptr};
   if (comp < 3) src = sa.e[par][comp];
