@@ -82,7 +82,23 @@ struct StepArgs {
   const T* inc;      // [tsteps][4][nx]: c2h Ezinc/dx, c2h Eyinc/dx, c2e0 Hyinc/dx, c2e0 Hzinc/dx
   T* eslice;         // staging of E slices: [2][nx][ny] [2][nx][nz] [2][ny][nz]
   const int* tdev;   // step index
+  // Two-step tiles: the region [x2lo,x2hi) x [y2lo,y2hi) x [z2lo,z2hi), at
+  // least 6 cells clear of the grid faces on every side (so clear of the
+  // CPML slabs, the NTFF planes and their H averages), advanced two steps
+  // per launch by fdtd_pair; its x-chunk, row and z starts.
+  int x2lo, x2hi, y2lo, y2hi, z2lo, z2hi;
+  const int* xstart2;
+  const int* ystart2;
+  const int* zstart2;
+  // probes, whose neighbourhoods need step t+1 values in pair mode
+  int nprobes;
+  const int* px; const int* py; const int* pz;
 };
+
+// Rows of a two-step workgroup (2 halo rows below and above the owned ones)
+// and owned z cells per tile (64 lanes less 2 low and 2 high halo lanes).
+constexpr int kPairRows = 12;
+constexpr int kPairOwnZ = kLanes - 4;
 
 // Near-to-far-field surface DFT.  Every step writes the 24 surface
 // sample arrays into slot t % K of a K-step buffer; every K steps (and at
@@ -122,6 +138,8 @@ struct AuxArgs {
 template <typename T>
 hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int cur, int pm, bool slice,
                        const uint32_t* tiles, int n, hipStream_t s);
+template <typename T>
+hipError_t launch_pair(const StepArgs<T>* a, int cur, const uint32_t* tiles, int n, hipStream_t s);
 template <typename T>
 hipError_t launch_ntff(const NtffArgs<T>* a, const uint2* blocks, int nblocks, hipStream_t s);
 template <typename T>

@@ -556,6 +556,348 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   }
 }
 
+// Two time steps per launch on the tiles of the two-step region (no CPML
+// terms, no slice capture: the region is clear of both).  Step t is
+// computed on the owned cells plus a halo of two cells on every side in y
+// and z, step t+1 on the owned cells, so every value step t+1 needs is
+// produced inside the workgroup and tiles stay independent.  The march
+// along x is skewed by one plane: iteration i computes step t at plane i
+// and step t+1 at plane i-1, with the step-t planes i-1 and i of E and the
+// step-t plane i of H held in LDS.  Per two steps each of E, H, J and the
+// media is read once (set t % 3, J^(t-1)) and E, H, J written once (set
+// (t+2) % 3, J^(t+1)).  Step-t values are written (set (t+1) % 3) only for
+// owned cells the one-step launches around the region read as halo (the
+// two outermost layers of the region) and for probe neighbourhoods.  The
+// arithmetic of every cell and step is the one-step kernel's, term by term.
+template <typename T, int CUR>
+__global__ void __launch_bounds__(kPairRows * kLanes)
+fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ tiles) {
+  constexpr int R = kPairRows;
+  constexpr uint32_t W = sizeof(T);
+  __shared__ T s0[2][3][R][kLanes];   // E^t of planes i and i+1 (by parity)
+  __shared__ T s1h[3][R][kLanes];     // H^(t+1/2) of plane i
+  __shared__ T s1e[2][3][R][kLanes];  // E^(t+1) of planes i-1 and i (by parity)
+  __shared__ T s2h[3][R][kLanes];     // H^(t+3/2) of plane i-1
+
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int r = threadIdx.x / kLanes;
+  const uint32_t tile = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+  const ConstArgs<StepArgs<T>>& a = as_const_space(args);
+  const int k0 = a.zstart2[tile & 0xff], k1 = a.zstart2[(tile & 0xff) + 1];
+  const int j0 = a.ystart2[(tile >> 8) & 0xfff], j1 = a.ystart2[((tile >> 8) & 0xfff) + 1];
+  const int ib = a.xstart2[tile >> 20], ie = a.xstart2[(tile >> 20) + 1];
+
+  const int nx = a.nx, ny = a.ny, nz = a.nz;
+  const int j = j0 - 2 + r;
+  const int k = k0 - 2 + lane;
+  // positions inside the grid away from its faces: every domain condition
+  // of the one-step update holds there
+  const bool inside = j >= 1 && j < ny - 1 && k >= 1 && k < nz - 1;
+  const bool valid = j >= 0 && j < ny && k >= 0 && k < nz;
+  const bool h1ok = inside && r < R - 1 && lane < kLanes - 1;  // H^(t+1/2): needs the +1 neighbours
+  const bool e1ok = h1ok && r >= 1 && lane >= 1;                // E^(t+1): needs the -1 H neighbours
+  const bool h2ok = e1ok && r < R - 2 && lane < kLanes - 2;     // H^(t+3/2)
+  const bool own = inside && r >= 2 && r < R - 2 && j < j1 && lane >= 2 && lane < kLanes - 2 && k < k1;
+  // owned cells whose step-t values the one-step launches or the probes read
+  bool rimjk = j < a.y2lo + 2 || j >= a.y2hi - 2 || k < a.z2lo + 2 || k >= a.z2hi - 2;
+  unsigned long long pmask = 0;  // probes whose (j, k) box holds this cell
+  for (int p = 0; p < a.nprobes; ++p)
+    if (abs(j - a.py[p]) <= 1 && abs(k - a.pz[p]) <= 1) pmask |= 1ull << p;
+
+  const bool jin = j >= a.by1 && j < a.by2, kin = k >= a.bz1 && k < a.bz2;
+  const T sHx0 = kin ? (T)((int)(j == a.by1 - 1) - (int)(j == a.by2 - 1)) : (T)0;
+  const T sHx1 = jin ? (T)((int)(k == a.bz2 - 1) - (int)(k == a.bz1 - 1)) : (T)0;
+  const T sEx2 = jin ? (T)((int)(k == a.bz1) - (int)(k == a.bz2)) : (T)0;
+  const T sEx3 = kin ? (T)((int)(j == a.by2) - (int)(j == a.by1)) : (T)0;
+  const T mJK = (jin && kin) ? (T)1 : (T)0;
+
+  const int t = *a.tdev;
+  const int s0i = t % 3, s1o = (t + 1) % 3, s2o = (t + 2) % 3;  // step t, t+1, t+2 field sets
+  const T* __restrict__ Exo = a.e[s0i][0];
+  const T* __restrict__ Eyo = a.e[s0i][1];
+  const T* __restrict__ Ezo = a.e[s0i][2];
+  const T* __restrict__ Hxo = a.h[s0i][0];
+  const T* __restrict__ Hyo = a.h[s0i][1];
+  const T* __restrict__ Hzo = a.h[s0i][2];
+  const T* __restrict__ Jxo = a.j[s2o][0];  // J^(t-1)
+  const T* __restrict__ Jyo = a.j[s2o][1];
+  const T* __restrict__ Jzo = a.j[s2o][2];
+  const T* __restrict__ inc1 = a.inc + (size_t)t * 4 * nx;        // step t
+  const T* __restrict__ inc2 = a.inc + (size_t)(t + 1) * 4 * nx;  // step t+1
+  const T c2h = a.c2h, c2e0 = a.c2e0, inv = a.inv_d;
+  const T tfEzB1 = inc1[a.bx1], tfEzB2 = inc1[a.bx2 - 1];
+  const T tfEyB1 = inc1[nx + a.bx1], tfEyB2 = inc1[nx + a.bx2 - 1];
+  const T tfHyB1 = inc1[2 * nx + a.bx1 - 1], tfHyB2 = inc1[2 * nx + a.bx2 - 1];
+  const T tfHzB1 = inc1[3 * nx + a.bx1 - 1], tfHzB2 = inc1[3 * nx + a.bx2 - 1];
+  const T ufEzB1 = inc2[a.bx1], ufEzB2 = inc2[a.bx2 - 1];
+  const T ufEyB1 = inc2[nx + a.bx1], ufEyB2 = inc2[nx + a.bx2 - 1];
+  const T ufHyB1 = inc2[2 * nx + a.bx1 - 1], ufHyB2 = inc2[2 * nx + a.bx2 - 1];
+  const T ufHzB1 = inc2[3 * nx + a.bx1 - 1], ufHzB2 = inc2[3 * nx + a.bx2 - 1];
+  const uint32_t sxb = (uint32_t)a.sx * W;
+  const uint32_t ob = valid ? ((uint32_t)j * (uint32_t)a.sy + (uint32_t)k) * W : 0u;
+
+  // batch of plane i+1 (issued at plane i): E^t(i+2), H^(t-1/2)(i+1),
+  // J^(t-1)(i+1), media(i+1), incident terms of step t at i+1 and step t+1 at i
+  T px = 0, py = 0, pz = 0;
+  T nhx = 0, nhy = 0, nhz = 0;
+  T nqx = 0, nqy = 0, nqz = 0, nqw = 0, nqn = 0;
+  T nt0 = 0, nt1 = 0, nt2 = 0, nt3 = 0, mt0 = 0, mt1 = 0, mt2 = 0, mt3 = 0;
+  const int i0 = ib - 2;
+  {
+    const uint32_t c = (uint32_t)i0 * sxb + ob;
+    T ex = 0, ey = 0, ez = 0;
+    if (valid) {
+      ex = ldo(Exo, c); ey = ldo(Eyo, c); ez = ldo(Ezo, c);
+      nhx = ldo(Hxo, c); nhy = ldo(Hyo, c); nhz = ldo(Hzo, c);
+      px = ldo(Exo, c + sxb); py = ldo(Eyo, c + sxb); pz = ldo(Ezo, c + sxb);
+    }
+    s0[i0 & 1][0][r][lane] = ex; s0[i0 & 1][1][r][lane] = ey; s0[i0 & 1][2][r][lane] = ez;
+    nt0 = inc1[i0]; nt1 = inc1[nx + i0]; nt2 = inc1[2 * nx + i0]; nt3 = inc1[3 * nx + i0];
+  }
+  // carried along x: step-t H and E, step-t+1 E, J^t and media of plane i-1,
+  // step-t+1 E x of plane i-2, step-t+1 H of plane i-2
+  T h1px = 0, h1py = 0, h1pz = 0, e0xp = 0;
+  T e1px = 0, e1py = 0, e1pz = 0, e1xpp = 0;
+  T j1px = 0, j1py = 0, j1pz = 0, w1p = 0, n1p = 0;
+  T h2py = 0, h2pz = 0;
+
+  for (int i = i0; i <= ie; ++i) {
+    const uint32_t c = (uint32_t)i * sxb + ob;
+    const int b0 = i & 1, bn = b0 ^ 1;
+    // ---- A: take the batch of plane i, stage E^t(i+1), issue the batch of plane i+1
+    const T hx0 = nhx, hy0 = nhy, hz0 = nhz;
+    const T qx = nqx, qy = nqy, qz = nqz, qw = nqw, qn = nqn;
+    const T tf0 = nt0, tf1 = nt1, tf2 = nt2, tf3 = nt3;  // step t, plane i
+    const T uf0 = mt0, uf1 = mt1, uf2 = mt2, uf3 = mt3;  // step t+1, plane i-1
+    s0[bn][0][r][lane] = px; s0[bn][1][r][lane] = py; s0[bn][2][r][lane] = pz;
+    if (i + 1 <= ie) {
+      const uint32_t cn = c + sxb;
+      if (valid) {
+        px = ldo(Exo, cn + sxb); py = ldo(Eyo, cn + sxb); pz = ldo(Ezo, cn + sxb);
+        nhx = ldo(Hxo, cn); nhy = ldo(Hyo, cn); nhz = ldo(Hzo, cn);
+      }
+      if (e1ok) {
+        nqx = ldo(Jxo, cn); nqy = ldo(Jyo, cn); nqz = ldo(Jzo, cn);
+        nqw = ldo(a.wpe2, cn); nqn = ldo(a.nue, cn);
+      }
+      nt0 = inc1[i + 1]; nt1 = inc1[nx + i + 1]; nt2 = inc1[2 * nx + i + 1]; nt3 = inc1[3 * nx + i + 1];
+      mt0 = inc2[i]; mt1 = inc2[nx + i]; mt2 = inc2[2 * nx + i]; mt3 = inc2[3 * nx + i];
+    }
+    const bool rim = own && i >= ib && i < ie &&
+                     (rimjk || i < a.x2lo + 2 || i >= a.x2hi - 2 ||
+                      (pmask && [&] {
+                        for (int p = 0; p < a.nprobes; ++p)
+                          if (((pmask >> p) & 1) && abs(i - a.px[p]) <= 1) return true;
+                        return false;
+                      }()));
+    lds_barrier();
+
+    // ---- B1: H^(t+1/2) at plane i   (:797-806, :854-865, :913-924 and TF/SF)
+    T h1x = hx0, h1y = hy0, h1z = hz0;
+    const T ecx = s0[b0][0][r][lane], ecy = s0[b0][1][r][lane], ecz = s0[b0][2][r][lane];
+    if (h1ok) {
+      const T ezjp = s0[b0][2][r + 1][lane];
+      const T eykp = s0[b0][1][r][lane + 1];
+      const T exkp = s0[b0][0][r][lane + 1];
+      const T exjp = s0[b0][0][r + 1][lane];
+      const T eny = s0[bn][1][r][lane], enz = s0[bn][2][r][lane];
+      {
+        const T dEz = (ezjp - ecz) * inv;
+        const T dEy = (eykp - ecy) * inv;
+        h1x = h1x - c2h * (dEz - dEy);
+        if (i >= a.bx1 && i < a.bx2) {
+          h1x = fma(sHx0, tf0, h1x);
+          h1x = fma(sHx1, tf1, h1x);
+        }
+      }
+      {
+        const T dEx = (exkp - ecx) * inv;
+        const T dEz = (enz - ecz) * inv;
+        h1y = h1y - c2h * (dEx - dEz);
+        if (i == a.bx1 - 1) h1y = fma(-mJK, tfEzB1, h1y);
+        if (i == a.bx2 - 1) h1y = fma(mJK, tfEzB2, h1y);
+      }
+      {
+        const T dEy = (eny - ecy) * inv;
+        const T dEx = (exjp - ecx) * inv;
+        h1z = h1z - c2h * (dEy - dEx);
+        if (i == a.bx1 - 1) h1z = fma(mJK, tfEyB1, h1z);
+        if (i == a.bx2 - 1) h1z = fma(-mJK, tfEyB2, h1z);
+      }
+      if (rim) {
+        sto(a.h[s1o][0], c, h1x);
+        sto(a.h[s1o][1], c, h1y);
+        sto(a.h[s1o][2], c, h1z);
+      }
+    }
+    s1h[0][r][lane] = h1x;
+    s1h[1][r][lane] = h1y;
+    s1h[2][r][lane] = h1z;
+    lds_barrier();
+
+    // ---- C1: J^t (deferred) and E^(t+1) at plane i
+    T e1x = 0, e1y = 0, e1z = 0, j1x = 0, j1y = 0, j1z = 0;
+    if (e1ok && i >= ib - 1) {
+      j1x = qx;
+      j1y = qy;
+      j1z = qz;
+      if constexpr (CUR == kCurDI) {
+        current_update<T, CUR>(a, qn, qw, j1x, j1y, j1z, ecx, ecy, ecz);
+      } else {
+        if (a.jcol) {
+          current_update<T, CUR>(a, qn, qw, j1x, j1y, j1z, ecx, ecy, ecz);
+        } else {
+          const T mx = (ecx + e0xp) / 2;
+          const T my = (ecy + s0[b0][1][r - 1][lane]) / 2;
+          const T mz = (ecz + s0[b0][2][r][lane - 1]) / 2;
+          current_update<T, CUR>(a, qn, qw, j1x, j1y, j1z, mx, my, mz);
+        }
+      }
+      T c1 = 1, c2 = c2e0;
+      if (a.doplate == 2 && in_sphere(a, i, j, k)) { c1 = a.c1s; c2 = a.c2s; }
+      {
+        const T dHz = (h1z - s1h[2][r - 1][lane]) * inv;
+        const T dHy = (h1y - s1h[1][r][lane - 1]) * inv;
+        e1x = c1 * ecx + c2 * (dHz - dHy) - c2 * j1x;
+        if (i >= a.bx1 && i < a.bx2 - 1) {
+          e1x = fma(sEx2, tf2, e1x);
+          e1x = fma(sEx3, tf3, e1x);
+        }
+      }
+      {
+        const T dHx = (h1x - s1h[0][r][lane - 1]) * inv;
+        const T dHz = (h1z - h1pz) * inv;
+        e1y = c1 * ecy + c2 * (dHx - dHz) - c2 * j1y;
+        if (i == a.bx1) e1y = fma(mJK, tfHzB1, e1y);
+        if (i == a.bx2 - 1) e1y = fma(-mJK, tfHzB2, e1y);
+        if (a.doplate == 1 && i == a.ssx && j >= a.plate_lo && j < a.plate_up - 1 && k >= a.plate_lo && k < a.plate_up)
+          e1y = 0;
+      }
+      {
+        const T dHy = (h1y - h1py) * inv;
+        const T dHx = (h1x - s1h[0][r - 1][lane]) * inv;
+        e1z = c1 * ecz + c2 * (dHy - dHx) - c2 * j1z;
+        if (i == a.bx1) e1z = fma(-mJK, tfHyB1, e1z);
+        if (i == a.bx2 - 1) e1z = fma(mJK, tfHyB2, e1z);
+        if (a.doplate == 1 && i == a.ssx && j >= a.plate_lo && j < a.plate_up && k >= a.plate_lo && k < a.plate_up - 1)
+          e1z = 0;
+      }
+      if (rim) {
+        sto(a.e[s1o][0], c, e1x);
+        sto(a.e[s1o][1], c, e1y);
+        sto(a.e[s1o][2], c, e1z);
+      }
+    }
+    s1e[b0][0][r][lane] = e1x;
+    s1e[b0][1][r][lane] = e1y;
+    s1e[b0][2][r][lane] = e1z;
+    lds_barrier();
+
+    // ---- B2: H^(t+3/2) at plane i-1
+    const int ip = i - 1;
+    const uint32_t cp = c - sxb;
+    T h2x = h1px, h2y = h1py, h2z = h1pz;
+    if (h2ok && i >= ib) {
+      const T ezjp = s1e[bn][2][r + 1][lane];
+      const T eykp = s1e[bn][1][r][lane + 1];
+      const T exkp = s1e[bn][0][r][lane + 1];
+      const T exjp = s1e[bn][0][r + 1][lane];
+      {
+        const T dEz = (ezjp - e1pz) * inv;
+        const T dEy = (eykp - e1py) * inv;
+        h2x = h2x - c2h * (dEz - dEy);
+        if (ip >= a.bx1 && ip < a.bx2) {
+          h2x = fma(sHx0, uf0, h2x);
+          h2x = fma(sHx1, uf1, h2x);
+        }
+      }
+      {
+        const T dEx = (exkp - e1px) * inv;
+        const T dEz = (e1z - e1pz) * inv;
+        h2y = h2y - c2h * (dEx - dEz);
+        if (ip == a.bx1 - 1) h2y = fma(-mJK, ufEzB1, h2y);
+        if (ip == a.bx2 - 1) h2y = fma(mJK, ufEzB2, h2y);
+      }
+      {
+        const T dEy = (e1y - e1py) * inv;
+        const T dEx = (exjp - e1px) * inv;
+        h2z = h2z - c2h * (dEy - dEx);
+        if (ip == a.bx1 - 1) h2z = fma(mJK, ufEyB1, h2z);
+        if (ip == a.bx2 - 1) h2z = fma(-mJK, ufEyB2, h2z);
+      }
+      if (own && ip >= ib) {
+        sto(a.h[s2o][0], cp, h2x);
+        sto(a.h[s2o][1], cp, h2y);
+        sto(a.h[s2o][2], cp, h2z);
+      }
+    }
+    s2h[0][r][lane] = h2x;
+    s2h[1][r][lane] = h2y;
+    s2h[2][r][lane] = h2z;
+    lds_barrier();
+
+    // ---- C2: J^(t+1) and E^(t+2) at plane i-1 (owned cells)
+    if (own && ip >= ib) {
+      T j2x = j1px, j2y = j1py, j2z = j1pz;
+      if constexpr (CUR == kCurDI) {
+        current_update<T, CUR>(a, n1p, w1p, j2x, j2y, j2z, e1px, e1py, e1pz);
+      } else {
+        if (a.jcol) {
+          current_update<T, CUR>(a, n1p, w1p, j2x, j2y, j2z, e1px, e1py, e1pz);
+        } else {
+          const T mx = (e1px + e1xpp) / 2;
+          const T my = (e1py + s1e[bn][1][r - 1][lane]) / 2;
+          const T mz = (e1pz + s1e[bn][2][r][lane - 1]) / 2;
+          current_update<T, CUR>(a, n1p, w1p, j2x, j2y, j2z, mx, my, mz);
+        }
+      }
+      sto(a.j[s1o][0], cp, j2x);
+      sto(a.j[s1o][1], cp, j2y);
+      sto(a.j[s1o][2], cp, j2z);
+      T c1 = 1, c2 = c2e0;
+      if (a.doplate == 2 && in_sphere(a, ip, j, k)) { c1 = a.c1s; c2 = a.c2s; }
+      {
+        const T dHz = (h2z - s2h[2][r - 1][lane]) * inv;
+        const T dHy = (h2y - s2h[1][r][lane - 1]) * inv;
+        T ex = c1 * e1px + c2 * (dHz - dHy) - c2 * j2x;
+        if (ip >= a.bx1 && ip < a.bx2 - 1) {
+          ex = fma(sEx2, uf2, ex);
+          ex = fma(sEx3, uf3, ex);
+        }
+        sto(a.e[s2o][0], cp, ex);
+      }
+      {
+        const T dHx = (h2x - s2h[0][r][lane - 1]) * inv;
+        const T dHz = (h2z - h2pz) * inv;
+        T ey = c1 * e1py + c2 * (dHx - dHz) - c2 * j2y;
+        if (ip == a.bx1) ey = fma(mJK, ufHzB1, ey);
+        if (ip == a.bx2 - 1) ey = fma(-mJK, ufHzB2, ey);
+        if (a.doplate == 1 && ip == a.ssx && j >= a.plate_lo && j < a.plate_up - 1 && k >= a.plate_lo && k < a.plate_up)
+          ey = 0;
+        sto(a.e[s2o][1], cp, ey);
+      }
+      {
+        const T dHy = (h2y - h2py) * inv;
+        const T dHx = (h2x - s2h[0][r - 1][lane]) * inv;
+        T ez = c1 * e1pz + c2 * (dHy - dHx) - c2 * j2z;
+        if (ip == a.bx1) ez = fma(-mJK, ufHyB1, ez);
+        if (ip == a.bx2 - 1) ez = fma(mJK, ufHyB2, ez);
+        if (a.doplate == 1 && ip == a.ssx && j >= a.plate_lo && j < a.plate_up && k >= a.plate_lo && k < a.plate_up - 1)
+          ez = 0;
+        sto(a.e[s2o][2], cp, ez);
+      }
+    }
+    // ---- carry along x
+    e1xpp = e1px;
+    e1px = e1x; e1py = e1y; e1pz = e1z;
+    h1px = h1x; h1py = h1y; h1pz = h1z;
+    e0xp = ecx;
+    j1px = j1x; j1py = j1y; j1pz = j1z;
+    w1p = qw; n1p = qn;
+    h2py = h2y; h2pz = h2z;
+  }
+}
+
 // J^(t+1) from J^t and E^(t+1) into `out` (what the reference holds after
 // the same number of steps); used only to read J back.
 template <typename T, int CUR>
@@ -868,6 +1210,19 @@ hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int c
   return hipErrorInvalidValue;
 }
 
+// Two steps of the tiles of the two-step region (fdtd_pair).
+template <typename T>
+hipError_t launch_pair(const StepArgs<T>* a, int cur, const uint32_t* tiles, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const dim3 g(n), b(kPairRows * kLanes);
+  switch (cur) {
+    case kCurDI: hipLaunchKernelGGL((fdtd_pair<T, kCurDI>), g, b, 0, s, a, tiles); break;
+    case kCurLKPoly: hipLaunchKernelGGL((fdtd_pair<T, kCurLKPoly>), g, b, 0, s, a, tiles); break;
+    default: hipLaunchKernelGGL((fdtd_pair<T, kCurLKClosed>), g, b, 0, s, a, tiles); break;
+  }
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_ntff(const NtffArgs<T>* a, const uint2* blocks, int nblocks, hipStream_t s) {
   if (nblocks <= 0) return hipSuccess;
@@ -908,6 +1263,7 @@ hipError_t launch_catchup(const StepArgs<T>* a, int cur, T* ox, T* oy, T* oz, hi
 #define MSC_INST(T)                                                                                                \
   template hipError_t launch_step<T>(const StepArgs<T>*, const uint16_t*, int, int, int, bool, const uint32_t*, int, \
                                      hipStream_t);                                                                 \
+  template hipError_t launch_pair<T>(const StepArgs<T>*, int, const uint32_t*, int, hipStream_t);                 \
   template hipError_t launch_ntff<T>(const NtffArgs<T>*, const uint2*, int, hipStream_t);                          \
   template hipError_t launch_ntff_contract<T>(const NtffArgs<T>*, long long, hipStream_t);                        \
   template hipError_t launch_frames<T>(const AuxArgs<T>*, hipStream_t);                                            \

@@ -77,6 +77,14 @@ struct SolverBase {
   std::vector<uint32_t> tiles_host; // the kNumLists tile lists, one after another
   int list_off[kNumLists + 1] = {};  // list l is tiles_host[list_off[l], list_off[l+1])
   std::vector<uint2> ntff_host;     // NTFF workgroup descriptors (solve id 0)
+  // Two-step mode: steps t, t+1 advance the two-step region with fdtd_pair
+  // and the rest of the grid with the one-step lists restricted to tiles
+  // outside it.  pair_host holds those outer lists (offsets outer_off) and
+  // then the fdtd_pair tiles (from outer_off[kNumLists] to the end).
+  bool pair = false;
+  int chunk2 = 16;                  // x planes per fdtd_pair tile
+  std::vector<uint32_t> pair_host;
+  int outer_off[kNumLists + 1] = {};
   bool timing = false;
   std::vector<hipEvent_t> ev;       // pairs around the step kernels of the last step() call
   double kernel_ms = 0;
@@ -108,6 +116,7 @@ struct Solver : SolverBase {
   NtffArgs<T>* na_dev = nullptr;
   AuxArgs<T>* xa_dev = nullptr;
   uint32_t* tiles = nullptr;
+  uint32_t* tiles2 = nullptr;  // pair_host on the device
   uint2* ntff_blocks = nullptr;
   size_t acc_n = 0;
   double* acc = nullptr;
@@ -128,6 +137,9 @@ struct Solver : SolverBase {
  private:
   // CPML-aligned tile starts per axis, computed by layout()
   std::vector<int> xs_, ys_, ysb_, zs_;
+  // two-step region bounds and fdtd_pair tile starts
+  int x2lo = 0, x2hi = 0, y2lo = 0, y2hi = 0, z2lo = 0, z2hi = 0;
+  std::vector<int> xs2_, ys2_, zs2_;
   size_t carve(char* base);
 };
 

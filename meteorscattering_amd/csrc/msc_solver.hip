@@ -356,34 +356,60 @@ size_t Solver<T>::layout() {
     if (const char* s = std::getenv("MSC_TY")) ty = std::atoi(s);
     if (const char* s = std::getenv("MSC_TY_B")) ty_b = std::atoi(s);
     if (const char* s = std::getenv("MSC_CHUNK")) chunk = std::max(1, std::atoi(s));
+    if (const char* s = std::getenv("MSC_PAIR")) pair = std::atoi(s) != 0;
+    if (const char* s = std::getenv("MSC_CHUNK2")) chunk2 = std::max(1, std::atoi(s));
   }
+  // Two-step region: 6 cells clear of every face (L+6: clear of the CPML
+  // slabs [0, L] and [n-L-1, n), of the NTFF planes at L+5 and n-L-5 with
+  // their one-cell H averages, and far enough in that a one-step tile's
+  // one-cell halo and the two rim layers never reach a slab).
+  x2lo = x2hi = y2lo = y2hi = z2lo = z2hi = 0;
+  if (pair) {
+    const int m = L + 6;
+    if (nx - m > m && ny - m > m && nz - m > m) {
+      x2lo = m; x2hi = nx - m; y2lo = m; y2hi = ny - m; z2lo = m; z2hi = nz - m;
+    }
+  }
+  const bool region = x2hi > x2lo;
   // Tiles: x and y are cut at the CPML slab edges (L+1 and n-L-1), so a tile
   // either meets a slab on some axis (the variant carrying the CPML code) or
   // meets none (the interior variant); z keeps full 63-wide tiles from 0,
   // the first and last of which hold the z slabs.  A tile's one-cell low
   // halo (row/column/plane L when the tile starts at L+1) needs no CPML
   // term: slab_h(L) is outside both H slabs, and no E is computed there.
-  auto cut = [&](int n, int piece, bool align) {
+  // (all three axes are also cut at the two-step region's bounds, so a
+  // one-step tile is inside the region or outside it)
+  auto cut = [&](int n, int piece, bool align, int rlo, int rhi) {
+    std::vector<int> br;
+    if (align && h.p.dopml) br = {L + 1, n - L - 1};
+    if (rhi > rlo) { br.push_back(rlo); br.push_back(rhi); }
+    std::sort(br.begin(), br.end());
+    br.push_back(n);
     std::vector<int> st;
-    auto span = [&](int lo, int hi) {
-      for (int q = lo; q < hi; q += piece) st.push_back(q);
-    };
-    if (align && h.p.dopml) {
-      span(0, L + 1);
-      span(L + 1, n - L - 1);
-      span(n - L - 1, n);
-    } else {
-      span(0, n);
+    int lo = 0;
+    for (int b : br) {
+      for (int q = lo; q < b; q += piece) st.push_back(q);
+      lo = std::max(lo, b);
     }
     st.push_back(n);
     return st;
   };
-  xs_ = cut(nx, chunk, true);
+  xs_ = cut(nx, chunk, true, x2lo, x2hi);
   // interior and boundary tiles cut y differently; both cuttings align at
   // L+1 and n-L-1, so a row is in a slab zone in both or in neither
-  ys_ = cut(ny, ty, true);
-  ysb_ = cut(ny, h.p.dopml ? ty_b : ty, true);
-  zs_ = cut(nz, kOwnZ, false);
+  ys_ = cut(ny, ty, true, y2lo, y2hi);
+  ysb_ = cut(ny, h.p.dopml ? ty_b : ty, true, 0, 0);
+  zs_ = cut(nz, kOwnZ, false, z2lo, z2hi);
+  // fdtd_pair tiles: pieces of near-equal length
+  auto even = [](int lo, int hi, int piece) {
+    std::vector<int> st;
+    const int np = hi > lo ? (hi - lo + piece - 1) / piece : 0;
+    for (int q = 0; q <= np; q++) st.push_back(lo + (int)((long long)(hi - lo) * q / std::max(np, 1)));
+    return st;
+  };
+  xs2_ = even(x2lo, x2hi, chunk2);
+  ys2_ = even(y2lo, y2hi, kPairRows - 4);
+  zs2_ = even(z2lo, z2hi, kPairOwnZ);
   const int nch = (int)xs_.size() - 1, tiles_y = (int)ys_.size() - 1, tiles_z = (int)zs_.size() - 1;
   const int tiles_yb = (int)ysb_.size() - 1;
   auto zone = [&](int lo, int hi, int n) {  // owned range [lo, hi) meets p <= L or p >= n-L-1
@@ -392,14 +418,21 @@ size_t Solver<T>::layout() {
   // interior tiles (ty rows) and CPML tiles (ty_b rows), the latter by the
   // set of slab axes their owned cells meet
   std::vector<uint32_t> lists[kNumLists];
+  std::vector<uint32_t> outer;  // interior tiles outside the two-step region
   tiles_host.clear();
+  pair_host.clear();
   if (tiles_z <= 255 && tiles_y <= 4095 && tiles_yb <= 4095 && nch <= 4095) {
     for (int c = 0; c < nch; c++) {
       const bool xz = zone(xs_[c], xs_[c + 1], nx);
       for (int yy = 0; yy < tiles_y; yy++)
         for (int zz = 0; zz < tiles_z; zz++) {
           const bool b = h.p.dopml && (xz || zone(ys_[yy], ys_[yy + 1], ny) || zone(zs_[zz], zs_[zz + 1], nz));
-          if (!b) lists[kListInterior].push_back(pack_tile(zz, yy, c));
+          if (!b) {
+            lists[kListInterior].push_back(pack_tile(zz, yy, c));
+            const bool in = region && xs_[c] >= x2lo && xs_[c + 1] <= x2hi && ys_[yy] >= y2lo &&
+                            ys_[yy + 1] <= y2hi && zs_[zz] >= z2lo && zs_[zz + 1] <= z2hi;
+            if (!in) outer.push_back(pack_tile(zz, yy, c));
+          }
         }
       if (!h.p.dopml) continue;
       for (int yy = 0; yy < tiles_yb; yy++)
@@ -417,6 +450,18 @@ size_t Solver<T>::layout() {
     tiles_host.insert(tiles_host.end(), lists[l].begin(), lists[l].end());
   }
   list_off[kNumLists] = (int)tiles_host.size();
+  if (region && (int)zs2_.size() - 1 <= 255 && (int)ys2_.size() - 1 <= 4095 && (int)xs2_.size() - 1 <= 4095) {
+    // the CPML lists are outside the region as they are
+    for (int l = 0; l < kNumLists; l++) {
+      outer_off[l] = (int)pair_host.size();
+      const auto& src = l == kListInterior ? outer : lists[l];
+      pair_host.insert(pair_host.end(), src.begin(), src.end());
+    }
+    outer_off[kNumLists] = (int)pair_host.size();
+    for (int c = 0; c + 1 < (int)xs2_.size(); c++)
+      for (int yy = 0; yy + 1 < (int)ys2_.size(); yy++)
+        for (int zz = 0; zz + 1 < (int)zs2_.size(); zz++) pair_host.push_back(pack_tile(zz, yy, c));
+  }
   // NTFF: faces x (ny*nz points), y (nx*nz), z (nx*ny), two each
   ntff_host.clear();
   for (int face = 0; face < 6; face++) {
@@ -504,6 +549,10 @@ size_t Solver<T>::carve(char* base) {
   take(xa.py, MSC_MAX_PROBES);
   take(xa.pz, MSC_MAX_PROBES);
   take(tiles, tiles_host.size());
+  take(tiles2, pair_host.size());
+  take(sa.xstart2, xs2_.size());
+  take(sa.ystart2, ys2_.size());
+  take(sa.zstart2, zs2_.size());
   take(sa.xstart, xs_.size());
   take(sa.ystart, ys_.size());
   take(sa.ystart_b, ysb_.size());
@@ -605,6 +654,12 @@ int Solver<T>::init(const double* wp, const double* nu) {
   HIPCHK(hipMemcpyAsync((void*)xa.py, h.p.probey, 4 * MSC_MAX_PROBES, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync((void*)xa.pz, h.p.probez, 4 * MSC_MAX_PROBES, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync(tiles, tiles_host.data(), tiles_host.size() * 4, hipMemcpyHostToDevice, stream));
+  if (!pair_host.empty()) {
+    HIPCHK(hipMemcpyAsync(tiles2, pair_host.data(), pair_host.size() * 4, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync((void*)sa.xstart2, xs2_.data(), xs2_.size() * 4, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync((void*)sa.ystart2, ys2_.data(), ys2_.size() * 4, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync((void*)sa.zstart2, zs2_.data(), zs2_.size() * 4, hipMemcpyHostToDevice, stream));
+  }
   HIPCHK(hipMemcpyAsync((void*)sa.xstart, xs_.data(), xs_.size() * 4, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync((void*)sa.ystart, ys_.data(), ys_.size() * 4, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync((void*)sa.ystart_b, ysb_.data(), ysb_.size() * 4, hipMemcpyHostToDevice, stream));
@@ -634,6 +689,9 @@ int Solver<T>::init(const double* wp, const double* nu) {
   sa.cth = h.cth; sa.sth = h.sth; sa.Se1 = h.Se1; sa.Ce1 = h.Ce1;
   for (int q = 0; q < 9; q++) sa.R[q] = h.R[q];
   sa.tdev = tdev;
+  sa.x2lo = x2lo; sa.x2hi = x2hi; sa.y2lo = y2lo; sa.y2hi = y2hi; sa.z2lo = z2lo; sa.z2hi = z2hi;
+  sa.nprobes = h.p.nprobes;
+  sa.px = xa.px; sa.py = xa.py; sa.pz = xa.pz;
 
   na.nx = nx; na.ny = ny; na.nz = nz; na.M = h.M; na.b = L + 5;
   na.sy = sa.sy; na.sx = sa.sx;
@@ -666,19 +724,48 @@ int Solver<T>::step(int n) {
     ev.assign(2 * (size_t)n, nullptr);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   }
-  for (int s = 0; s < n; s++) {
-    const bool cad = (t_host % plan.cadence) == 0;
-    const bool slice = cad && plan.p.savefields;
-    if (timing) HIPCHK(hipEventRecord(ev[2 * s], stream));
-    for (int l = 0; l < kNumLists; l++)
-      HIPCHK(launch_step<T>(sa_dev, nullptr, l == kListInterior ? ty : ty_b, cur, list_pm(l), slice,
-                            tiles + list_off[l], list_off[l + 1] - list_off[l], stream));
-    if (timing) HIPCHK(hipEventRecord(ev[2 * s + 1], stream));
+  // everything after the field update of step t_host: NTFF samples, frames,
+  // probes and the step tick, the contraction at the end of a K-step block
+  auto after = [&](bool cad) -> int {
     HIPCHK(launch_ntff<T>(na_dev, ntff_blocks, (int)ntff_host.size(), stream));
     if (cad) HIPCHK(launch_frames<T>(xa_dev, stream));
     HIPCHK(launch_probes<T>(xa_dev, 1, stream));
     if ((t_host + 1) % na.K == 0 || t_host + 1 == plan.T) HIPCHK(launch_ntff_contract<T>(na_dev, na.uoff[24], stream));
     ++t_host;
+    return MSC_OK;
+  };
+  auto lists = [&](const uint32_t* base, const int* off, bool slice) -> int {
+    for (int l = 0; l < kNumLists; l++)
+      HIPCHK(launch_step<T>(sa_dev, nullptr, l == kListInterior ? ty : ty_b, cur, list_pm(l), slice,
+                            base + off[l], off[l + 1] - off[l], stream));
+    return MSC_OK;
+  };
+  const int npair = pair_host.empty() ? 0 : (int)pair_host.size() - outer_off[kNumLists];
+  int rc;
+  for (int s = 0; s < n;) {
+    const bool cad = (t_host % plan.cadence) == 0;
+    const bool slice = cad && plan.p.savefields;
+    // Two steps at once when nothing reads the whole grid between them:
+    // no frame or slice after step t, no slice in step t+1.
+    const bool cad1 = ((t_host + 1) % plan.cadence) == 0;
+    if (npair > 0 && s + 1 < n && !cad && !(cad1 && plan.p.savefields)) {
+      if (timing) HIPCHK(hipEventRecord(ev[2 * s], stream));
+      if ((rc = lists(tiles2, outer_off, false))) return rc;
+      HIPCHK(launch_pair<T>(sa_dev, cur, tiles2 + outer_off[kNumLists], npair, stream));
+      if (timing) HIPCHK(hipEventRecord(ev[2 * s + 1], stream));
+      if ((rc = after(false))) return rc;
+      if (timing) HIPCHK(hipEventRecord(ev[2 * s + 2], stream));
+      if ((rc = lists(tiles2, outer_off, false))) return rc;
+      if (timing) HIPCHK(hipEventRecord(ev[2 * s + 3], stream));
+      if ((rc = after(cad1))) return rc;
+      s += 2;
+      continue;
+    }
+    if (timing) HIPCHK(hipEventRecord(ev[2 * s], stream));
+    if ((rc = lists(tiles, list_off, slice))) return rc;
+    if (timing) HIPCHK(hipEventRecord(ev[2 * s + 1], stream));
+    if ((rc = after(cad))) return rc;
+    s++;
   }
   return MSC_OK;
 }

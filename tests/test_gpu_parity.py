@@ -46,18 +46,21 @@ def worst(a, b):
     return f"at {tuple(int(x) for x in idx)}: gpu={a[idx]!r} oracle={b[idx]!r}"
 
 
-def run_pair(Solver, oracle_lib, p, wp, nu, precision, nsteps, check_every, tol, label):
-    """Step GPU and oracle together.  Error of a field kind (E, H or J) is
-    max |gpu - oracle| over its three components divided by the running peak
-    max |oracle| of that kind over the run so far."""
+def run_pair(Solver, oracle_lib, p, wp, nu, precision, nsteps, check_every, tol, label, stride=1):
+    """Step GPU and oracle together, `stride` steps per call.  Error of a
+    field kind (E, H or J) is max |gpu - oracle| over its three components
+    divided by the running peak max |oracle| of that kind over the run so
+    far."""
     g = Solver(p, wp, nu, precision=precision)
     o = oracle_lib.Oracle(p, wp, nu)
     report = []
     peak = [0.0, 0.0, 0.0]
     errs = [0.0, 0.0, 0.0]
-    for t in range(nsteps):
-        g.step(1)
-        o.step(1)
+    for t in range(stride - 1, nsteps + stride - 1, stride):
+        t = min(t, nsteps - 1)
+        n = t + 1 - g.t
+        g.step(n)
+        o.step(n)
         if (t + 1) % check_every and t + 1 != nsteps:
             continue
         pairs = [(g.field(c), o.field(c)) for c in range(9)]
@@ -167,6 +170,62 @@ def test_tilings_are_bit_identical(msc, oracle_lib, monkeypatch):
     for f, n in outs[1:]:
         assert np.array_equal(f, outs[0][0])
         assert np.array_equal(n, outs[0][1])
+
+
+# Two-step grid: with L = 10 the two-step region is [16,44) x [16,34) x
+# [16,189); it holds the TF/SF faces (x 20, 40; y 20, 30; z 20, 185),
+# the plate or sphere, the trail, and probes at its edges.
+PAIR_GRID = dict(nx=60, ny=50, nz=205, nprobes=3, probex=[30, 17, 43], probey=[25, 33, 16],
+                 probez=[100, 64, 188])
+
+
+def pair_case(name, jcol, tsteps):
+    p, _, _ = cases.case(name)
+    p.update(PAIR_GRID, tsteps=tsteps, j_collocated=jcol, numfiles=3)
+    wp, nu = cases.trail_maps(p, n0=2e13, r0=1.5, nu0=1e6, centre=(0.5, -0.25))
+    return p, wp, nu
+
+
+def snapshot(g):
+    frames = [g.frame(f) for f in range(g.nframes)]
+    return [g.field(c) for c in range(9)] + [g.ntff(), g.probes()] + [np.concatenate(
+        [np.ravel(x) for x in fr]) for fr in frames]
+
+
+@pytest.mark.parametrize("name,jcol", [("lk_ricker_ez", 0), ("lk_ricker_ez", 1), ("di_modgauss_ey", 0),
+                                       ("plate_ricker_45", 0), ("sphere_sin_rhcp", 0)])
+@pytest.mark.parametrize("precision", [32, 64])
+def test_two_step_tiles_are_bit_identical(msc, monkeypatch, name, jcol, precision):
+    """Two-step mode (fdtd_pair on the region clear of the CPML and NTFF
+    planes, one-step launches outside it, stepped in uneven chunks so single
+    steps, frame steps and pairs mix) gives the one-step path's fields, NTFF
+    payload, probes and frames bit for bit."""
+    Solver, _ = msc
+    p, wp, nu = pair_case(name, jcol, 90)
+    outs = []
+    for pair, chunks in ((0, [90]), (1, [1, 2, 5, 13, 30, 39])):
+        monkeypatch.setenv("MSC_PAIR", str(pair))
+        g = Solver(p, wp, nu, precision=precision)
+        for c in chunks:
+            g.step(c)
+        outs.append(snapshot(g))
+        g.close()
+    labels = list(_abi.COMPONENTS) + ["ntff", "probes"] + [f"frame{f}" for f in range(len(outs[0]) - 11)]
+    bad = [f"{lab}: {worst(a, b)}" for lab, a, b in zip(labels, outs[1], outs[0]) if not np.array_equal(a, b)]
+    assert not bad, "\n".join(bad)
+
+
+@pytest.mark.parametrize("precision,tol", [(64, 1e-10), (32, 2e-5)])
+def test_two_step_tiles_match_oracle(msc, oracle_lib, monkeypatch, precision, tol):
+    """Two-step mode against the oracle, 10 steps per call."""
+    Solver, _ = msc
+    monkeypatch.setenv("MSC_PAIR", "1")
+    p, wp, nu = pair_case("lk_ricker_ez", 0, 100)
+    g, o, rep = run_pair(Solver, oracle_lib, p, wp, nu, precision, p["tsteps"], 20, tol, f"pair{precision}",
+                         stride=10)
+    assert not rep, "\n".join(rep[:12])
+    assert relerr(g.ntff(), o.ntff()) < tol
+    assert relerr(g.probes(), o.probes()) < tol
 
 
 @pytest.mark.parametrize("precision,tol", [(64, 1e-10), (32, 2e-5)])
