@@ -192,6 +192,21 @@ size_t msc_solver_device_bytes(const msc_solver* s);
 int msc_solver_set_timing(msc_solver* s, int on);
 double msc_solver_kernel_ms(const msc_solver* s);
 
+/* Timing of the last msc_solver_step call (timing on), split by launch
+ * kind.  In two-step mode (the default; MSC_PAIR=0 turns it off) a pair of
+ * steps t, t+1 runs the two-step kernel once over the region pair_cells
+ * (6 cells clear of every grid face) and the one-step kernels twice over
+ * the rest; steps that write a frame or a slice run one-step everywhere.
+ * No counterpart in the reference (its loop is untimed). */
+typedef struct msc_timing {
+  int32_t steps;          /* steps of that call */
+  int32_t pair_launches;  /* two-step launches among them */
+  double step_ms;         /* all field-update kernels, per step */
+  double pair_ms;         /* mean duration of one two-step launch */
+  int64_t pair_cells;     /* cells a two-step launch advances (0: mode off) */
+} msc_timing;
+int msc_solver_get_timing(const msc_solver* s, msc_timing* out);
+
 void msc_solver_destroy(msc_solver* s);
 
 /* Streaming copy of `bytes` between two fresh device buffers, `iters` times;
@@ -249,7 +264,7 @@ typedef struct msc_server_stats {
   long long solve_steps;  /* solve time steps launched */
 } msc_server_stats;
 
-/* Device bytes one solve occupies (0 for invalid parameters). */
+/* Device bytes one solve occupies in a server (0 for invalid parameters). */
 size_t msc_solve_footprint(const msc_params* p, int precision);
 
 int msc_server_create(int precision, size_t budget_bytes, msc_server** out);

@@ -134,6 +134,18 @@ class RequestInfo(C.Structure):
     ]
 
 
+class Timing(C.Structure):
+    """``msc_timing``."""
+
+    _fields_ = [
+        ("steps", C.c_int32),
+        ("pair_launches", C.c_int32),
+        ("step_ms", C.c_double),
+        ("pair_ms", C.c_double),
+        ("pair_cells", C.c_int64),
+    ]
+
+
 class ServerStats(C.Structure):
     """``msc_server_stats``."""
 
@@ -176,6 +188,7 @@ def _declare(lib: C.CDLL) -> None:
         "msc_solver_device_bytes": (C.c_size_t, [vp]),
         "msc_solver_set_timing": (C.c_int, [vp, C.c_int]),
         "msc_solver_kernel_ms": (C.c_double, [vp]),
+        "msc_solver_get_timing": (C.c_int, [vp, P(Timing)]),
         "msc_solver_destroy": (None, [vp]),
         "msc_run_dir": (C.c_int, [C.c_char_p, C.c_int]),
         "msc_stream_copy": (C.c_int, [C.c_size_t, C.c_int, dp]),

@@ -97,7 +97,7 @@ struct StepArgs {
 
 // Rows of a two-step workgroup (2 halo rows below and above the owned ones)
 // and owned z cells per tile (64 lanes less 2 low and 2 high halo lanes).
-constexpr int kPairRows = 12;
+constexpr int kPairRows = 16;
 constexpr int kPairOwnZ = kLanes - 4;
 
 // Near-to-far-field surface DFT.  Every step writes the 24 surface

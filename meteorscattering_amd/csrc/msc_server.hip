@@ -213,6 +213,7 @@ struct Server : ServerBase {
     if (!r->s) {
       r->s.reset(new Solver<T>());
       r->s->tiling_from_env = false;  // the gang launches kTy / kTyB-row tiles
+      r->s->pair = false;             // and one step per iteration
       r->s->ty = kTy;
       r->s->ty_b = kTyB;
       int rc = make_plan(&r->p, r->s->plan);

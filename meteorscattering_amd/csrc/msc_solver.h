@@ -81,13 +81,18 @@ struct SolverBase {
   // and the rest of the grid with the one-step lists restricted to tiles
   // outside it.  pair_host holds those outer lists (offsets outer_off) and
   // then the fdtd_pair tiles (from outer_off[kNumLists] to the end).
-  bool pair = false;
-  int chunk2 = 16;                  // x planes per fdtd_pair tile
+  bool pair = true;
+  int chunk2 = 32;                  // x planes per fdtd_pair tile
+  long long pair_cells = 0;         // cells of the two-step region (0: none)
   std::vector<uint32_t> pair_host;
   int outer_off[kNumLists + 1] = {};
   bool timing = false;
   std::vector<hipEvent_t> ev;       // pairs around the step kernels of the last step() call
-  double kernel_ms = 0;
+  std::vector<char> ev_pair;        // per pair: 1 if it brackets an fdtd_pair launch
+  int timed_steps = 0;
+  double kernel_ms = 0;             // all field-update kernels, per step
+  double pair_ms = 0;               // mean fdtd_pair launch
+  int pair_launches = 0;
   virtual ~SolverBase();
   // Plan the device block: builds the tile and NTFF lists and returns the
   // block's size (no device call).

@@ -330,14 +330,18 @@ SolverBase::~SolverBase() {
 
 int SolverBase::sync() {
   HIPCHK(hipStreamSynchronize(stream));
-  if (timing && !ev.empty()) {
-    double tot = 0;
+  if (timing && !ev.empty() && timed_steps > 0) {
+    double tot = 0, tp = 0;
+    int np = 0;
     for (size_t q = 0; q + 1 < ev.size(); q += 2) {
       float ms = 0;
       HIPCHK(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
       tot += ms;
+      if (ev_pair[q / 2]) { tp += ms; np++; }
     }
-    kernel_ms = tot / (double)(ev.size() / 2);
+    kernel_ms = tot / (double)timed_steps;
+    pair_launches = np;
+    pair_ms = np ? tp / np : 0.0;
   }
   return MSC_OK;
 }
@@ -421,6 +425,7 @@ size_t Solver<T>::layout() {
   std::vector<uint32_t> outer;  // interior tiles outside the two-step region
   tiles_host.clear();
   pair_host.clear();
+  pair_cells = 0;
   if (tiles_z <= 255 && tiles_y <= 4095 && tiles_yb <= 4095 && nch <= 4095) {
     for (int c = 0; c < nch; c++) {
       const bool xz = zone(xs_[c], xs_[c + 1], nx);
@@ -458,6 +463,7 @@ size_t Solver<T>::layout() {
       pair_host.insert(pair_host.end(), src.begin(), src.end());
     }
     outer_off[kNumLists] = (int)pair_host.size();
+    pair_cells = (long long)(x2hi - x2lo) * (y2hi - y2lo) * (z2hi - z2lo);
     for (int c = 0; c + 1 < (int)xs2_.size(); c++)
       for (int yy = 0; yy + 1 < (int)ys2_.size(); yy++)
         for (int zz = 0; zz + 1 < (int)zs2_.size(); zz++) pair_host.push_back(pack_tile(zz, yy, c));
@@ -719,11 +725,26 @@ template <typename T>
 int Solver<T>::step(int n) {
   if (n < 0) return fail(MSC_EINVAL, "nsteps < 0");
   if (t_host + n > plan.T) return fail(MSC_ESTATE, "stepping past tsteps (%d + %d > %d)", t_host, n, plan.T);
-  if (timing) {
-    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-    ev.assign(2 * (size_t)n, nullptr);
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-  }
+  for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  ev.clear();
+  ev_pair.clear();
+  timed_steps = timing ? n : 0;
+  // timed brackets: tick(kind) opens one on the stream, tock() closes it
+  auto tick = [&](bool is_pair) -> int {
+    if (!timing) return MSC_OK;
+    hipEvent_t a = nullptr, b = nullptr;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    ev.push_back(a);
+    ev.push_back(b);
+    ev_pair.push_back(is_pair ? 1 : 0);
+    HIPCHK(hipEventRecord(a, stream));
+    return MSC_OK;
+  };
+  auto tock = [&]() -> int {
+    if (timing) HIPCHK(hipEventRecord(ev.back(), stream));
+    return MSC_OK;
+  };
   // everything after the field update of step t_host: NTFF samples, frames,
   // probes and the step tick, the contraction at the end of a K-step block
   auto after = [&](bool cad) -> int {
@@ -749,21 +770,18 @@ int Solver<T>::step(int n) {
     // no frame or slice after step t, no slice in step t+1.
     const bool cad1 = ((t_host + 1) % plan.cadence) == 0;
     if (npair > 0 && s + 1 < n && !cad && !(cad1 && plan.p.savefields)) {
-      if (timing) HIPCHK(hipEventRecord(ev[2 * s], stream));
-      if ((rc = lists(tiles2, outer_off, false))) return rc;
+      if ((rc = tick(false)) || (rc = lists(tiles2, outer_off, false)) || (rc = tock())) return rc;
+      if ((rc = tick(true))) return rc;
       HIPCHK(launch_pair<T>(sa_dev, cur, tiles2 + outer_off[kNumLists], npair, stream));
-      if (timing) HIPCHK(hipEventRecord(ev[2 * s + 1], stream));
-      if ((rc = after(false))) return rc;
-      if (timing) HIPCHK(hipEventRecord(ev[2 * s + 2], stream));
-      if ((rc = lists(tiles2, outer_off, false))) return rc;
-      if (timing) HIPCHK(hipEventRecord(ev[2 * s + 3], stream));
+      if ((rc = tock()) || (rc = after(false))) return rc;
+      if ((rc = tick(false)) || (rc = lists(tiles2, outer_off, false)) || (rc = tock())) return rc;
       if ((rc = after(cad1))) return rc;
       s += 2;
       continue;
     }
-    if (timing) HIPCHK(hipEventRecord(ev[2 * s], stream));
+    if ((rc = tick(false))) return rc;
     if ((rc = lists(tiles, list_off, slice))) return rc;
-    if (timing) HIPCHK(hipEventRecord(ev[2 * s + 1], stream));
+    if ((rc = tock())) return rc;
     if ((rc = after(cad))) return rc;
     s++;
   }
@@ -857,6 +875,9 @@ size_t solve_footprint(const msc_params* p, int precision) {
   if (precision == 32) s.reset(new Solver<float>());
   else if (precision == 64) s.reset(new Solver<double>());
   else return 0;
+  // the server's settings: its tiling, one step per gang iteration
+  s->tiling_from_env = false;
+  s->pair = false;
   if (make_plan(p, s->plan) != MSC_OK) return 0;
   return s->layout();
 }
@@ -970,6 +991,18 @@ int msc_solver_set_timing(msc_solver* s, int on) {
 }
 
 double msc_solver_kernel_ms(const msc_solver* s) { return (s && s->impl) ? s->impl->kernel_ms : 0.0; }
+
+int msc_solver_get_timing(const msc_solver* s, msc_timing* out) {
+  SOLVER_OK(s);
+  if (!out) return fail(MSC_EINVAL, "NULL argument");
+  const SolverBase& b = *s->impl;
+  out->steps = b.timed_steps;
+  out->pair_launches = b.pair_launches;
+  out->step_ms = b.kernel_ms;
+  out->pair_ms = b.pair_ms;
+  out->pair_cells = b.pair_cells;
+  return MSC_OK;
+}
 
 void msc_solver_destroy(msc_solver* s) { delete s; }
 

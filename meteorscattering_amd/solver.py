@@ -90,6 +90,12 @@ class Solver:
     def kernel_ms(self) -> float:
         return self.lib.msc_solver_kernel_ms(self._h)
 
+    def timing(self) -> dict:
+        """Timing of the last step() call by launch kind (``msc_timing``)."""
+        t = _abi.Timing()
+        _check(self.lib.msc_solver_get_timing(self._h, C.byref(t)), self.lib)
+        return {f: getattr(t, f) for f, _ in t._fields_}
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             self.lib.msc_solver_destroy(self._h)

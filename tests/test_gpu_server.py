@@ -32,7 +32,7 @@ def _alone(p, wp, nu, precision=32):
     with Solver(p, wp, nu, precision=precision) as s:
         s.step(p["tsteps"])
         s.sync()
-        return s.ntff(), s.probes(), s.device_bytes
+        return s.ntff(), s.probes(), footprint(p, precision)
 
 
 @pytest.fixture(scope="module")
