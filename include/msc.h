@@ -204,6 +204,7 @@ typedef struct msc_timing {
   double step_ms;         /* all field-update kernels, per step */
   double pair_ms;         /* mean duration of one two-step launch */
   int64_t pair_cells;     /* cells a two-step launch advances (0: mode off) */
+  double span_ms;         /* all GPU work of that call, first to last, per step */
 } msc_timing;
 int msc_solver_get_timing(const msc_solver* s, msc_timing* out);
 

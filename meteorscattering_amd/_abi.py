@@ -143,6 +143,7 @@ class Timing(C.Structure):
         ("step_ms", C.c_double),
         ("pair_ms", C.c_double),
         ("pair_cells", C.c_int64),
+        ("span_ms", C.c_double),
     ]
 
 

@@ -90,6 +90,8 @@ struct SolverBase {
   std::vector<hipEvent_t> ev;       // pairs around the step kernels of the last step() call
   std::vector<char> ev_pair;        // per pair: 1 if it brackets an fdtd_pair launch
   int timed_steps = 0;
+  hipEvent_t span[2] = {nullptr, nullptr};  // first and last work of the last step() call
+  double span_ms = 0;               // all GPU work of that call, per step
   double kernel_ms = 0;             // all field-update kernels, per step
   double pair_ms = 0;               // mean fdtd_pair launch
   int pair_launches = 0;

@@ -325,6 +325,8 @@ SolverBase::~SolverBase() {
     else (void)hipFree(blk);
   }
   for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  for (hipEvent_t e : span)
+    if (e) (void)hipEventDestroy(e);
   if (stream && own_stream) (void)hipStreamDestroy(stream);
 }
 
@@ -342,6 +344,9 @@ int SolverBase::sync() {
     kernel_ms = tot / (double)timed_steps;
     pair_launches = np;
     pair_ms = np ? tp / np : 0.0;
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, span[0], span[1]));
+    span_ms = ms / timed_steps;
   }
   return MSC_OK;
 }
@@ -370,8 +375,11 @@ size_t Solver<T>::layout() {
   x2lo = x2hi = y2lo = y2hi = z2lo = z2hi = 0;
   if (pair) {
     const int m = L + 6;
-    if (nx - m > m && ny - m > m && nz - m > m) {
-      x2lo = m; x2hi = nx - m; y2lo = m; y2hi = ny - m; z2lo = m; z2hi = nz - m;
+    if (nx - m > m && ny - m > m) {
+      // z bounds on the 63-cell z tiles: the one-step z-slab tiles stay
+      // full width (a narrow [0, L+6) tile costs as much as a full one)
+      const int zlo = (m + kOwnZ - 1) / kOwnZ * kOwnZ, zhi = (nz - m) / kOwnZ * kOwnZ;
+      if (zhi > zlo) { x2lo = m; x2hi = nx - m; y2lo = m; y2hi = ny - m; z2lo = zlo; z2hi = zhi; }
     }
   }
   const bool region = x2hi > x2lo;
@@ -587,6 +595,7 @@ int Solver<T>::init(const double* wp, const double* nu) {
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     own_stream = true;
   }
+
   if (arena) {
     blk = (char*)arena->alloc(bytes);
     if (!blk) return fail(MSC_EBUSY, "arena has no free range of %zu bytes", bytes);
@@ -729,6 +738,11 @@ int Solver<T>::step(int n) {
   ev.clear();
   ev_pair.clear();
   timed_steps = timing ? n : 0;
+  if (timing) {
+    for (hipEvent_t& e : span)
+      if (!e) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipEventRecord(span[0], stream));
+  }
   // timed brackets: tick(kind) opens one on the stream, tock() closes it
   auto tick = [&](bool is_pair) -> int {
     if (!timing) return MSC_OK;
@@ -770,6 +784,9 @@ int Solver<T>::step(int n) {
     // no frame or slice after step t, no slice in step t+1.
     const bool cad1 = ((t_host + 1) % plan.cadence) == 0;
     if (npair > 0 && s + 1 < n && !cad && !(cad1 && plan.p.savefields)) {
+      // step t: fdtd_pair over the region (steps t and t+1) and the
+      // one-step launches outside it; step t+1 outside the region, which
+      // reads the region's rim at step t+1
       if ((rc = tick(false)) || (rc = lists(tiles2, outer_off, false)) || (rc = tock())) return rc;
       if ((rc = tick(true))) return rc;
       HIPCHK(launch_pair<T>(sa_dev, cur, tiles2 + outer_off[kNumLists], npair, stream));
@@ -785,6 +802,7 @@ int Solver<T>::step(int n) {
     if ((rc = after(cad))) return rc;
     s++;
   }
+  if (timing) HIPCHK(hipEventRecord(span[1], stream));
   return MSC_OK;
 }
 
@@ -1001,6 +1019,7 @@ int msc_solver_get_timing(const msc_solver* s, msc_timing* out) {
   out->step_ms = b.kernel_ms;
   out->pair_ms = b.pair_ms;
   out->pair_cells = b.pair_cells;
+  out->span_ms = b.span_ms;
   return MSC_OK;
 }
 

@@ -173,8 +173,8 @@ def test_tilings_are_bit_identical(msc, oracle_lib, monkeypatch):
 
 
 # Two-step grid: with L = 10 the two-step region is [16,44) x [16,34) x
-# [16,189); it holds the TF/SF faces (x 20, 40; y 20, 30; z 20, 185),
-# the plate or sphere, the trail, and probes at its edges.
+# [63,189); it holds the TF/SF x and y faces (20, 40; 20, 30) and the high
+# z face (185), the plate or sphere, the trail, and probes at its edges.
 PAIR_GRID = dict(nx=60, ny=50, nz=205, nprobes=3, probex=[30, 17, 43], probey=[25, 33, 16],
                  probez=[100, 64, 188])
 
