@@ -205,6 +205,9 @@ typedef struct msc_timing {
   double pair_ms;         /* mean duration of one two-step launch */
   int64_t pair_cells;     /* cells a two-step launch advances (0: mode off) */
   double span_ms;         /* all GPU work of that call, first to last, per step */
+  int64_t pair_j_cells;   /* of pair_cells, those in tiles that update J */
+  int64_t step_j_cells;   /* cells per step in tiles that update J (tiles where
+                             eps0 wp^2 is zero everywhere skip J: it stays 0) */
 } msc_timing;
 int msc_solver_get_timing(const msc_solver* s, msc_timing* out);
 

@@ -144,6 +144,8 @@ class Timing(C.Structure):
         ("pair_ms", C.c_double),
         ("pair_cells", C.c_int64),
         ("span_ms", C.c_double),
+        ("pair_j_cells", C.c_int64),
+        ("step_j_cells", C.c_int64),
     ]
 
 

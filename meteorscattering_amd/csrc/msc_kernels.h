@@ -27,11 +27,17 @@ inline int list_pm(int l) {
   return l == kListX ? kPmlX : l == kListY ? kPmlY : l == kListZ ? kPmlZ : l == kListAll ? kPmlAll : 0;
 }
 
-// packed tile descriptor: z-tile (8 bits) | y-tile (12 bits) | x-chunk (12 bits),
-// indices into StepArgs::zstart / ystart / xstart
+// packed tile descriptor: z-tile (8 bits) | y-tile (12 bits) | x-chunk (11 bits)
+// | kTileNoCurrent, indices into StepArgs::zstart / ystart / xstart
 __host__ __device__ inline uint32_t pack_tile(int tz, int ty, int ch) {
   return (uint32_t)tz | ((uint32_t)ty << 8) | ((uint32_t)ch << 20);
 }
+constexpr int kMaxChunks = 2047;
+// Set on tiles where eps0 wp^2 is zero (in the field precision) on every
+// cell the tile updates J on, halo included.  There J starts at zero and
+// J' = A J + eps0 wp^2 K E stays zero, so the kernels skip J and the media
+// (neither read nor written: all three J sets hold zero there).
+constexpr uint32_t kTileNoCurrent = 1u << 31;
 
 // Everything one fused time step of one solve needs.  Kept in device memory
 // (one element of an array indexed by solve id); it does not change from

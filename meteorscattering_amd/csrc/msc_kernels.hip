@@ -187,7 +187,8 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   const ConstArgs<StepArgs<T>>& a = as_const_space(args + (sids ? sids[blk] : 0));
   const int tz = tile & 0xff;
   const int ty = (tile >> 8) & 0xfff;
-  const int chunk = tile >> 20;
+  const int chunk = (tile >> 20) & 0x7ff;
+  const bool cj = (tile & kTileNoCurrent) == 0;  // the tile carries a current
   const int k0 = a.zstart[tz], k1 = a.zstart[tz + 1];
   const int* __restrict__ yst = PM ? a.ystart_b : a.ystart;
   const int j0 = yst[ty], j1 = yst[ty + 1];
@@ -305,7 +306,7 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         if (colok) { pcx = ldo(Exo, c2 + W); pcy = ldo(Eyo, c2 + W); }
       }
       if (valid) { nhx = ldo(Hxo, cn); nhy = ldo(Hyo, cn); nhz = ldo(Hzo, cn); }
-      if (jdom && i + 1 >= 1 && i + 1 < nx - 1) {
+      if (cj && jdom && i + 1 >= 1 && i + 1 < nx - 1) {
         nqx = ldo(Jxo, cn); nqy = ldo(Jyo, cn); nqz = ldo(Jzo, cn);
         nqw = ldo(a.wpe2, cn); nqn = ldo(a.nue, cn);
       }
@@ -422,7 +423,7 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
     if (wr) {
       // J^n from J^(n-1) and E^n; domain [1,N-1)^3   (:1523-1597)
       T jx = 0, jy = 0, jz = 0;
-      if (jdom && i >= 1 && i < nx - 1) {
+      if (cj && jdom && i >= 1 && i < nx - 1) {
         jx = qx;
         jy = qy;
         jz = qz;
@@ -585,7 +586,8 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   const ConstArgs<StepArgs<T>>& a = as_const_space(args);
   const int k0 = a.zstart2[tile & 0xff], k1 = a.zstart2[(tile & 0xff) + 1];
   const int j0 = a.ystart2[(tile >> 8) & 0xfff], j1 = a.ystart2[((tile >> 8) & 0xfff) + 1];
-  const int ib = a.xstart2[tile >> 20], ie = a.xstart2[(tile >> 20) + 1];
+  const int ib = a.xstart2[(tile >> 20) & 0x7ff], ie = a.xstart2[((tile >> 20) & 0x7ff) + 1];
+  const bool cj = (tile & kTileNoCurrent) == 0;  // the tile carries a current
 
   const int nx = a.nx, ny = a.ny, nz = a.nz;
   const int j = j0 - 2 + r;
@@ -676,7 +678,7 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         px = ldo(Exo, cn + sxb); py = ldo(Eyo, cn + sxb); pz = ldo(Ezo, cn + sxb);
         nhx = ldo(Hxo, cn); nhy = ldo(Hyo, cn); nhz = ldo(Hzo, cn);
       }
-      if (e1ok) {
+      if (e1ok && cj) {
         nqx = ldo(Jxo, cn); nqy = ldo(Jyo, cn); nqz = ldo(Jzo, cn);
         nqw = ldo(a.wpe2, cn); nqn = ldo(a.nue, cn);
       }
@@ -738,19 +740,21 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
     // ---- C1: J^t (deferred) and E^(t+1) at plane i
     T e1x = 0, e1y = 0, e1z = 0, j1x = 0, j1y = 0, j1z = 0;
     if (e1ok && i >= ib - 1) {
-      j1x = qx;
-      j1y = qy;
-      j1z = qz;
-      if constexpr (CUR == kCurDI) {
-        current_update<T, CUR>(a, qn, qw, j1x, j1y, j1z, ecx, ecy, ecz);
-      } else {
-        if (a.jcol) {
+      if (cj) {
+        j1x = qx;
+        j1y = qy;
+        j1z = qz;
+        if constexpr (CUR == kCurDI) {
           current_update<T, CUR>(a, qn, qw, j1x, j1y, j1z, ecx, ecy, ecz);
         } else {
-          const T mx = (ecx + e0xp) / 2;
-          const T my = (ecy + s0[b0][1][r - 1][lane]) / 2;
-          const T mz = (ecz + s0[b0][2][r][lane - 1]) / 2;
-          current_update<T, CUR>(a, qn, qw, j1x, j1y, j1z, mx, my, mz);
+          if (a.jcol) {
+            current_update<T, CUR>(a, qn, qw, j1x, j1y, j1z, ecx, ecy, ecz);
+          } else {
+            const T mx = (ecx + e0xp) / 2;
+            const T my = (ecy + s0[b0][1][r - 1][lane]) / 2;
+            const T mz = (ecz + s0[b0][2][r][lane - 1]) / 2;
+            current_update<T, CUR>(a, qn, qw, j1x, j1y, j1z, mx, my, mz);
+          }
         }
       }
       T c1 = 1, c2 = c2e0;
@@ -838,22 +842,27 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
 
     // ---- C2: J^(t+1) and E^(t+2) at plane i-1 (owned cells)
     if (own && ip >= ib) {
-      T j2x = j1px, j2y = j1py, j2z = j1pz;
-      if constexpr (CUR == kCurDI) {
-        current_update<T, CUR>(a, n1p, w1p, j2x, j2y, j2z, e1px, e1py, e1pz);
-      } else {
-        if (a.jcol) {
+      T j2x = 0, j2y = 0, j2z = 0;
+      if (cj) {
+        j2x = j1px;
+        j2y = j1py;
+        j2z = j1pz;
+        if constexpr (CUR == kCurDI) {
           current_update<T, CUR>(a, n1p, w1p, j2x, j2y, j2z, e1px, e1py, e1pz);
         } else {
-          const T mx = (e1px + e1xpp) / 2;
-          const T my = (e1py + s1e[bn][1][r - 1][lane]) / 2;
-          const T mz = (e1pz + s1e[bn][2][r][lane - 1]) / 2;
-          current_update<T, CUR>(a, n1p, w1p, j2x, j2y, j2z, mx, my, mz);
+          if (a.jcol) {
+            current_update<T, CUR>(a, n1p, w1p, j2x, j2y, j2z, e1px, e1py, e1pz);
+          } else {
+            const T mx = (e1px + e1xpp) / 2;
+            const T my = (e1py + s1e[bn][1][r - 1][lane]) / 2;
+            const T mz = (e1pz + s1e[bn][2][r][lane - 1]) / 2;
+            current_update<T, CUR>(a, n1p, w1p, j2x, j2y, j2z, mx, my, mz);
+          }
         }
+        sto(a.j[s1o][0], cp, j2x);
+        sto(a.j[s1o][1], cp, j2y);
+        sto(a.j[s1o][2], cp, j2z);
       }
-      sto(a.j[s1o][0], cp, j2x);
-      sto(a.j[s1o][1], cp, j2y);
-      sto(a.j[s1o][2], cp, j2z);
       T c1 = 1, c2 = c2e0;
       if (a.doplate == 2 && in_sphere(a, ip, j, k)) { c1 = a.c1s; c2 = a.c2s; }
       {

@@ -84,6 +84,10 @@ struct SolverBase {
   bool pair = true;
   int chunk2 = 32;                  // x planes per fdtd_pair tile
   long long pair_cells = 0;         // cells of the two-step region (0: none)
+  // owned cells of tiles that carry a current (see kTileNoCurrent): of the
+  // one-step tiling, of the one-step tiles outside the region, of the
+  // two-step tiles
+  long long jc_single = 0, jc_outer = 0, jc_pair = 0;
   std::vector<uint32_t> pair_host;
   int outer_off[kNumLists + 1] = {};
   bool timing = false;

@@ -434,7 +434,7 @@ size_t Solver<T>::layout() {
   tiles_host.clear();
   pair_host.clear();
   pair_cells = 0;
-  if (tiles_z <= 255 && tiles_y <= 4095 && tiles_yb <= 4095 && nch <= 4095) {
+  if (tiles_z <= 255 && tiles_y <= 4095 && tiles_yb <= 4095 && nch <= kMaxChunks) {
     for (int c = 0; c < nch; c++) {
       const bool xz = zone(xs_[c], xs_[c + 1], nx);
       for (int yy = 0; yy < tiles_y; yy++)
@@ -463,7 +463,7 @@ size_t Solver<T>::layout() {
     tiles_host.insert(tiles_host.end(), lists[l].begin(), lists[l].end());
   }
   list_off[kNumLists] = (int)tiles_host.size();
-  if (region && (int)zs2_.size() - 1 <= 255 && (int)ys2_.size() - 1 <= 4095 && (int)xs2_.size() - 1 <= 4095) {
+  if (region && (int)zs2_.size() - 1 <= 255 && (int)ys2_.size() - 1 <= 4095 && (int)xs2_.size() - 1 <= kMaxChunks) {
     // the CPML lists are outside the region as they are
     for (int l = 0; l < kNumLists; l++) {
       outer_off[l] = (int)pair_host.size();
@@ -589,7 +589,7 @@ int Solver<T>::init(const double* wp, const double* nu) {
     return fail(MSC_EINVAL, "grid of %zu cells exceeds the 4 GiB-per-array limit of the 32-bit offset kernel", N);
   const size_t bytes = layout();
   if ((int)zs_.size() - 1 > 255 || (int)ys_.size() - 1 > 4095 || (int)ysb_.size() - 1 > 4095 ||
-      (int)xs_.size() - 1 > 4095)
+      (int)xs_.size() - 1 > kMaxChunks)
     return fail(MSC_EINVAL, "grid too large for the tile descriptor");
   if (!stream) {
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -619,6 +619,36 @@ int Solver<T>::init(const double* wp, const double* nu) {
   if ((rc = upload(h.bh, sa.bh)) || (rc = upload(h.ah, sa.ah)) || (rc = upload(h.be, sa.be)) ||
       (rc = upload(h.ae, sa.ae)) || (rc = upload(h.inc, sa.inc)))
     return rc;
+  // Tiles that carry a current: those whose J cells (owned cells, plus a
+  // halo of 1 for one-step tiles and 2 for two-step tiles, which also
+  // update J on their halo) meet a nonzero eps0 wp^2; the rest get
+  // kTileNoCurrent below.  Boxes are inclusive, clipped to the grid.
+  struct JBox {
+    int x0, x1, y0, y1, z0, z1;
+    uint32_t* word;
+    long long own;
+    bool hit;
+  };
+  std::vector<JBox> jb;
+  auto add_boxes = [&](std::vector<uint32_t>& v, size_t lo, size_t hi, const std::vector<int>& xs,
+                       const std::vector<int>& ys, const std::vector<int>& zs, int halo) {
+    for (size_t q = lo; q < hi; q++) {
+      const uint32_t w = v[q];
+      const int tz = w & 0xff, tyy = (w >> 8) & 0xfff, ch = (w >> 20) & 0x7ff;
+      const long long own = (long long)(xs[ch + 1] - xs[ch]) * (ys[tyy + 1] - ys[tyy]) * (zs[tz + 1] - zs[tz]);
+      jb.push_back({std::max(0, xs[ch] - halo), std::min(nx - 1, xs[ch + 1] - 1 + halo), std::max(0, ys[tyy] - halo),
+                    std::min(ny - 1, ys[tyy + 1] - 1 + halo), std::max(0, zs[tz] - halo),
+                    std::min(nz - 1, zs[tz + 1] - 1 + halo), &v[q], own, false});
+    }
+  };
+  for (int l = 0; l < kNumLists; l++) {
+    const auto& ys = l == kListInterior ? ys_ : ysb_;
+    add_boxes(tiles_host, list_off[l], list_off[l + 1], xs_, ys, zs_, 1);
+    if (!pair_host.empty()) add_boxes(pair_host, outer_off[l], outer_off[l + 1], xs_, ys, zs_, 1);
+  }
+  const size_t jb_pair = jb.size();
+  if (!pair_host.empty()) add_boxes(pair_host, outer_off[kNumLists], pair_host.size(), xs2_, ys2_, zs2_, 2);
+  std::vector<int> sat((size_t)(ny + 1) * (nz + 1), 0);  // per-plane summed-area table of wpe2 != 0
   // media: eps0 wpe^2 and nu per cell, computed in FP64 as meteor-fork.cpp:336-366
   double numax = 0;
   {
@@ -648,12 +678,33 @@ int Solver<T>::init(const double* wp, const double* nu) {
           pn[(size_t)j * nz + k] = (T)nue;
           numax = nue > numax ? nue : numax;
         }
+      for (int j = 0; j < ny; j++)
+        for (int k = 0; k < nz; k++)
+          sat[(size_t)(j + 1) * (nz + 1) + k + 1] = (pw[(size_t)j * nz + k] != (T)0) + sat[(size_t)j * (nz + 1) + k + 1] +
+                                                    sat[(size_t)(j + 1) * (nz + 1) + k] - sat[(size_t)j * (nz + 1) + k];
+      for (JBox& b : jb) {
+        if (b.hit || i < b.x0 || i > b.x1) continue;
+        auto at = [&](int j, int k) { return sat[(size_t)j * (nz + 1) + k]; };
+        b.hit = at(b.y1 + 1, b.z1 + 1) - at(b.y0, b.z1 + 1) - at(b.y1 + 1, b.z0) + at(b.y0, b.z0) > 0;
+      }
       HIPCHK(hipMemcpyAsync((void*)(sa.wpe2 + (size_t)i * ny * nz), pw.data(), pw.size() * sizeof(T),
                             hipMemcpyHostToDevice, stream));
       HIPCHK(hipMemcpyAsync((void*)(sa.nue + (size_t)i * ny * nz), pn.data(), pn.size() * sizeof(T),
                             hipMemcpyHostToDevice, stream));
       HIPCHK(hipStreamSynchronize(stream));
     }
+  }
+  jc_single = jc_outer = jc_pair = 0;
+  for (size_t q = 0; q < jb.size(); q++) {
+    JBox& b = jb[q];
+    if (!b.hit) *b.word |= kTileNoCurrent;
+    else if (q >= jb_pair) jc_pair += b.own;
+    else if (b.word >= tiles_host.data() && b.word < tiles_host.data() + tiles_host.size()) jc_single += b.own;
+    else jc_outer += b.own;
+  }
+  if (std::getenv("MSC_CURRENT_EVERYWHERE")) {  // every tile updates J (no skipping)
+    for (uint32_t& w : tiles_host) w &= ~kTileNoCurrent;
+    for (uint32_t& w : pair_host) w &= ~kTileNoCurrent;
   }
   // current model: DI, or Lee-Kalluri with polynomial (default) / closed-form coefficients
   cur = kCurDI;
@@ -1020,6 +1071,8 @@ int msc_solver_get_timing(const msc_solver* s, msc_timing* out) {
   out->pair_ms = b.pair_ms;
   out->pair_cells = b.pair_cells;
   out->span_ms = b.span_ms;
+  out->pair_j_cells = b.jc_pair;
+  out->step_j_cells = b.pair_cells > 0 ? b.jc_pair + b.jc_outer : b.jc_single;
   return MSC_OK;
 }
 

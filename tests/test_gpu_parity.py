@@ -215,6 +215,38 @@ def test_two_step_tiles_are_bit_identical(msc, monkeypatch, name, jcol, precisio
     assert not bad, "\n".join(bad)
 
 
+@pytest.mark.parametrize("pair", [0, 1])
+@pytest.mark.parametrize("precision", [32, 64])
+def test_current_free_tiles_change_nothing(msc, monkeypatch, pair, precision):
+    """Tiles where eps0 wp^2 is zero on every cell skip J and the media; a
+    narrow trail tapered in z leaves most tiles current-free.  Fields, NTFF
+    payload and probes equal those of the run that updates J everywhere
+    (zeros may differ in sign only)."""
+    Solver, _ = msc
+    p, _, _ = pair_case("lk_ricker_ez", 0, 60)
+    p.update(metsize=[2.0, 2.0])
+    wp, nu = cases.trail_maps(p, n0=5e13, r0=0.35, nu0=1e6, centre=(0.5, -0.25))
+    monkeypatch.setenv("MSC_PAIR", str(pair))
+    outs = []
+    for everywhere in (True, False):
+        if everywhere:
+            monkeypatch.setenv("MSC_CURRENT_EVERYWHERE", "1")
+        else:
+            monkeypatch.delenv("MSC_CURRENT_EVERYWHERE", raising=False)
+        g = Solver(p, wp, nu, precision=precision)
+        g.step(60)
+        outs.append(snapshot(g))
+        jcells = g.timing()["step_j_cells"]
+        g.close()
+    ncell = p["nx"] * p["ny"] * p["nz"]
+    assert 0 < jcells < ncell * 0.7, jcells  # many tiles skip J (FP64 underflows later: more carry it)
+    labels = list(_abi.COMPONENTS) + ["ntff", "probes"] + [f"frame{f}" for f in range(len(outs[0]) - 11)]
+    bad = [f"{lab}: {worst(a, b)}" for lab, a, b in zip(labels, outs[1], outs[0]) if not np.array_equal(a, b)]
+    assert not bad, "\n".join(bad)
+    # the J fields are nonzero somewhere: the trail carries a current
+    assert np.abs(outs[1][6]).max() > 0
+
+
 @pytest.mark.parametrize("precision,tol", [(64, 1e-10), (32, 2e-5)])
 def test_two_step_tiles_match_oracle(msc, oracle_lib, monkeypatch, precision, tol):
     """Two-step mode against the oracle, 10 steps per call."""
