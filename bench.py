@@ -232,6 +232,9 @@ def main():
                      "traffic": traffic_from_profile(kernel, wname),
                      "kernel": kname, "kernel_ms": round(kernel_ms, 4),
                      "launches": tm["pair_launches"] if kernel == "fdtd_pair" else args.steps,
+                     "cells_per_launch": tm["pair_cells"] if kernel == "fdtd_pair" else ncell,
+                     "j_cells_per_launch": tm["pair_j_cells"] if kernel == "fdtd_pair" else tm["step_j_cells"],
+                     "updates_per_cell": 2 if kernel == "fdtd_pair" else 1,
                      "algorithmic_bytes_per_launch": algo, "algorithmic_basis": basis,
                      "step": {"gpu_ms_per_step": round(span, 4),
                               "achieved": round(step_achieved, 1) if step_achieved else None,
