@@ -362,6 +362,8 @@ size_t Solver<T>::layout() {
   const HostPlan& h = plan;
   const int nx = h.nx, ny = h.ny, nz = h.nz, L = h.L;
   if (tiling_from_env) {
+    // the FP64 two-step kernel spills registers: one-step unless asked for
+    if (sizeof(T) == 8) pair = false;
     if (const char* s = std::getenv("MSC_TY")) ty = std::atoi(s);
     if (const char* s = std::getenv("MSC_TY_B")) ty_b = std::atoi(s);
     if (const char* s = std::getenv("MSC_CHUNK")) chunk = std::max(1, std::atoi(s));
