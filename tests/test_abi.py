@@ -199,13 +199,20 @@ def test_copied_executable_still_finds_the_library(tmp_path):
 
 def test_solve_footprint_is_host_only_and_exact_in_form(lib):
     """msc_solve_footprint plans the block without a device: FP32 holds the
-    29 N-sized arrays (three rotating sets of E, H and J, two media arrays)
-    plus CPML slabs, FP64 nearly twice as much; invalid parameters give 0."""
+    29 N-sized arrays (three rotating sets of E, H and J, two media arrays),
+    the 18 CPML psi slabs (12 for H, 6 for E; 2L x n x n each on a cubic
+    grid) and the NTFF sample buffer (K <= 64 slots of 4 values per face
+    point), plus under 4 MB of NTFF sums, frames and tables; FP64 nearly
+    twice as much; invalid parameters give 0."""
     p, _, _ = _run_case()
-    n = p["nx"] * p["ny"] * p["nz"]
+    nx, ny, nz = p["nx"], p["ny"], p["nz"]
+    n = nx * ny * nz
+    L = p.get("pmllen") or 10
     f32 = lib.msc_solve_footprint(C.byref(_abi.params_to_struct(p)), 32)
     f64 = lib.msc_solve_footprint(C.byref(_abi.params_to_struct(p)), 64)
-    assert 29 * 4 * n < f32 < 2 * 29 * 4 * n  # psi slabs are large on a 44-cell grid
+    base = 4 * (29 * n + 18 * 2 * L * nx * ny)
+    u = 4 * 2 * (ny * nz + nx * nz + nx * ny)
+    assert base < f32 < base + 4 * 64 * u + (4 << 20)
     assert 1.6 < f64 / f32 < 2.0  # the FP64 NTFF sums and tables are common to both
     assert lib.msc_solve_footprint(C.byref(_abi.params_to_struct(dict(p, nx=10))), 32) == 0
     assert lib.msc_solve_footprint(C.byref(_abi.params_to_struct(p)), 16) == 0
