@@ -145,7 +145,8 @@ template <typename T>
 hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int cur, int pm, bool slice,
                        const uint32_t* tiles, int n, hipStream_t s);
 template <typename T>
-hipError_t launch_pair(const StepArgs<T>* a, int cur, const uint32_t* tiles, int n, hipStream_t s);
+hipError_t launch_pair(const StepArgs<T>* a, const uint16_t* sids, int cur, const uint32_t* tiles, int n,
+                       hipStream_t s);
 template <typename T>
 hipError_t launch_ntff(const NtffArgs<T>* a, const uint2* blocks, int nblocks, hipStream_t s);
 template <typename T>

@@ -572,7 +572,8 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
 // arithmetic of every cell and step is the one-step kernel's, term by term.
 template <typename T, int CUR>
 __global__ void __launch_bounds__(kPairRows * kLanes)
-fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ tiles) {
+fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ tiles,
+          const uint16_t* __restrict__ sids) {
   constexpr int R = kPairRows;
   constexpr uint32_t W = sizeof(T);
   __shared__ T s0[2][3][R][kLanes];   // E^t of planes i and i+1 (by parity)
@@ -582,8 +583,9 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
 
   const int lane = threadIdx.x & (kLanes - 1);
   const int r = threadIdx.x / kLanes;
-  const uint32_t tile = tiles[xcd_remap(blockIdx.x, gridDim.x)];
-  const ConstArgs<StepArgs<T>>& a = as_const_space(args);
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t tile = tiles[blk];
+  const ConstArgs<StepArgs<T>>& a = as_const_space(args + (sids ? sids[blk] : 0));
   const int k0 = a.zstart2[tile & 0xff], k1 = a.zstart2[(tile & 0xff) + 1];
   const int j0 = a.ystart2[(tile >> 8) & 0xfff], j1 = a.ystart2[((tile >> 8) & 0xfff) + 1];
   const int ib = a.xstart2[(tile >> 20) & 0x7ff], ie = a.xstart2[((tile >> 20) & 0x7ff) + 1];
@@ -1221,13 +1223,14 @@ hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int c
 
 // Two steps of the tiles of the two-step region (fdtd_pair).
 template <typename T>
-hipError_t launch_pair(const StepArgs<T>* a, int cur, const uint32_t* tiles, int n, hipStream_t s) {
+hipError_t launch_pair(const StepArgs<T>* a, const uint16_t* sids, int cur, const uint32_t* tiles, int n,
+                       hipStream_t s) {
   if (n <= 0) return hipSuccess;
   const dim3 g(n), b(kPairRows * kLanes);
   switch (cur) {
-    case kCurDI: hipLaunchKernelGGL((fdtd_pair<T, kCurDI>), g, b, 0, s, a, tiles); break;
-    case kCurLKPoly: hipLaunchKernelGGL((fdtd_pair<T, kCurLKPoly>), g, b, 0, s, a, tiles); break;
-    default: hipLaunchKernelGGL((fdtd_pair<T, kCurLKClosed>), g, b, 0, s, a, tiles); break;
+    case kCurDI: hipLaunchKernelGGL((fdtd_pair<T, kCurDI>), g, b, 0, s, a, tiles, sids); break;
+    case kCurLKPoly: hipLaunchKernelGGL((fdtd_pair<T, kCurLKPoly>), g, b, 0, s, a, tiles, sids); break;
+    default: hipLaunchKernelGGL((fdtd_pair<T, kCurLKClosed>), g, b, 0, s, a, tiles, sids); break;
   }
   return hipGetLastError();
 }
@@ -1272,7 +1275,7 @@ hipError_t launch_catchup(const StepArgs<T>* a, int cur, T* ox, T* oy, T* oz, hi
 #define MSC_INST(T)                                                                                                \
   template hipError_t launch_step<T>(const StepArgs<T>*, const uint16_t*, int, int, int, bool, const uint32_t*, int, \
                                      hipStream_t);                                                                 \
-  template hipError_t launch_pair<T>(const StepArgs<T>*, int, const uint32_t*, int, hipStream_t);                 \
+  template hipError_t launch_pair<T>(const StepArgs<T>*, const uint16_t*, int, const uint32_t*, int, hipStream_t); \
   template hipError_t launch_ntff<T>(const NtffArgs<T>*, const uint2*, int, hipStream_t);                          \
   template hipError_t launch_ntff_contract<T>(const NtffArgs<T>*, long long, hipStream_t);                        \
   template hipError_t launch_frames<T>(const AuxArgs<T>*, hipStream_t);                                            \

@@ -82,9 +82,11 @@ struct Server : ServerBase {
     size_t host_bytes = 0;
     hipEvent_t last_use = nullptr;
     bool used = false;
-    // one captured gang iteration (fused steps, NTFF samples, probes/tick)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    // one captured gang iteration: [0] step t of every solve (one-step
+    // tiles, two-step tiles, NTFF samples, probes/tick), [1] step t+1 of
+    // the two-step solves outside their regions
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
   };
 
   int device = 0;
@@ -111,10 +113,19 @@ struct Server : ServerBase {
   Gang gang[2];
   int cur_gang = 0;
   bool dirty = true;
-  // launch lists of the current gang: per current model, [interior, boundary]
+  // launch lists of the current gang.  The two-step solves come first in
+  // `sel` (n_pair of them): per current model and one-step variant, the
+  // list holds their tiles outside the two-step region and then the other
+  // solves' tiles (n_tiles), so its first n_tiles_b entries are step t+1 of
+  // the two-step solves; pair tiles per current model; NTFF blocks of the
+  // two-step solves first (n_nb_b of them).
   int n_tiles[3][kNumLists] = {};
+  int n_tiles_b[3][kNumLists] = {};
   size_t tile_off[3][kNumLists] = {};
-  int n_nb = 0;
+  int n_ptiles[3] = {};
+  size_t ptile_off[3] = {};
+  int n_nb = 0, n_nb_b = 0;
+  size_t n_pair = 0;
   // run-ahead limit: at most kDepth iterations queued on the run stream
   static constexpr int kDepth = 4;
   hipEvent_t iter_ev[kDepth] = {};
@@ -168,8 +179,10 @@ struct Server : ServerBase {
       hipFree(g.sa); hipFree(g.na); hipFree(g.xa); hipFree(g.tiles); hipFree(g.sids); hipFree(g.nb);
       if (g.host) hipHostFree(g.host);
       if (g.last_use) hipEventDestroy(g.last_use);
-      if (g.exec) hipGraphExecDestroy(g.exec);
-      if (g.graph) hipGraphDestroy(g.graph);
+      for (int h = 0; h < 2; h++) {
+        if (g.exec[h]) hipGraphExecDestroy(g.exec[h]);
+        if (g.graph[h]) hipGraphDestroy(g.graph[h]);
+      }
     }
     for (auto& e : iter_ev)
       if (e) hipEventDestroy(e);
@@ -213,7 +226,7 @@ struct Server : ServerBase {
     if (!r->s) {
       r->s.reset(new Solver<T>());
       r->s->tiling_from_env = false;  // the gang launches kTy / kTyB-row tiles
-      r->s->pair = false;             // and one step per iteration
+      r->s->pair = sizeof(T) == 4;    // two-step region in FP32 (FP64's kernel spills)
       r->s->ty = kTy;
       r->s->ty_b = kTyB;
       int rc = make_plan(&r->p, r->s->plan);
@@ -324,12 +337,15 @@ struct Server : ServerBase {
     Gang& g = gang[gi];
     if (g.used) HIPCHK(hipEventSynchronize(g.last_use));
     select();
+    // two-step solves (a region, two or more steps left) first
+    std::stable_partition(sel.begin(), sel.end(), [](const Req* r) { return pairs(r); });
+    n_pair = (size_t)std::count_if(sel.begin(), sel.end(), [](const Req* r) { return pairs(r); });
     const std::vector<Req*>& active = sel;
     const size_t ns = active.size();
     if (ns > 65535) return fail(MSC_EINVAL, "more than 65535 resident solves");
     size_t nt = 0, nnb = 0;
     for (Req* r : active) {
-      nt += r->s->tiles_host.size();
+      nt += pairs(r) ? r->s->pair_host.size() : r->s->tiles_host.size();
       nnb += r->s->ntff_host.size();
     }
     int rc = ensure(g, std::max<size_t>(ns, 1), std::max<size_t>(nt, 1), std::max<size_t>(nnb, 1));
@@ -348,24 +364,40 @@ struct Server : ServerBase {
       hxa[q] = active[q]->s->xa;
     }
     size_t o = 0;
-    for (int c = 0; c < 3; c++)
+    auto put = [&](const std::vector<uint32_t>& v, size_t b, size_t e, size_t q) {
+      for (size_t k = b; k < e; k++) {
+        htiles[o] = v[k];
+        hsids[o] = (uint16_t)q;
+        ++o;
+      }
+    };
+    for (int c = 0; c < 3; c++) {
       for (int v = 0; v < kNumLists; v++) {
         tile_off[c][v] = o;
-        for (size_t q = 0; q < ns; q++) {
+        for (size_t q = 0; q < n_pair; q++) {  // two-step solves: one-step tiles outside the region
           const Solver<T>& s = *active[q]->s;
-          if (s.cur != c) continue;
-          const size_t b = s.list_off[v], e = s.list_off[v + 1];
-          for (size_t k = b; k < e; k++) {
-            htiles[o] = s.tiles_host[k];
-            hsids[o] = (uint16_t)q;
-            ++o;
-          }
+          if (s.cur == c) put(s.pair_host, s.outer_off[v], s.outer_off[v + 1], q);
+        }
+        n_tiles_b[c][v] = (int)(o - tile_off[c][v]);
+        for (size_t q = n_pair; q < ns; q++) {  // the others: every tile
+          const Solver<T>& s = *active[q]->s;
+          if (s.cur == c) put(s.tiles_host, s.list_off[v], s.list_off[v + 1], q);
         }
         n_tiles[c][v] = (int)(o - tile_off[c][v]);
       }
+      ptile_off[c] = o;
+      for (size_t q = 0; q < n_pair; q++) {
+        const Solver<T>& s = *active[q]->s;
+        if (s.cur == c) put(s.pair_host, s.outer_off[kNumLists], s.pair_host.size(), q);
+      }
+      n_ptiles[c] = (int)(o - ptile_off[c]);
+    }
     size_t m = 0;
-    for (size_t q = 0; q < ns; q++)
+    for (size_t q = 0; q < ns; q++) {
       for (const uint2& d : active[q]->s->ntff_host) hnb[m++] = make_uint2((uint32_t)q, d.y);
+      if (q + 1 == n_pair) n_nb_b = (int)m;
+    }
+    if (n_pair == 0) n_nb_b = 0;
     n_nb = (int)m;
     HIPCHK(hipMemcpyAsync(g.sa, hsa, ns * sizeof(StepArgs<T>), hipMemcpyHostToDevice, run));
     HIPCHK(hipMemcpyAsync(g.na, hna, ns * sizeof(NtffArgs<T>), hipMemcpyHostToDevice, run));
@@ -381,40 +413,53 @@ struct Server : ServerBase {
     return use_graphs ? capture(g) : MSC_OK;
   }
 
-  // The fixed part of an iteration: the fused-step launches per current
-  // model and variant, the NTFF samples and the probe/tick launch.  The
-  // kernels read each solve's step from device memory, so the same launch
-  // sequence (or its captured graph) serves every iteration until the gang
-  // changes.
-  hipError_t launch_fixed(const Gang& g) {
+  // A solve steps two at a time when it has a two-step region and at least
+  // two steps left (the server's solves write no frames or slices).
+  static bool pairs(const Req* r) { return !r->s->pair_host.empty() && r->t + 2 <= r->s->plan.T; }
+
+  // The fixed part of an iteration, in two halves.  Half 0: step t of every
+  // solve (the one-step launches per current model and variant, the
+  // two-step launches, the NTFF samples, the probe/tick launch).  Half 1:
+  // step t+1 of the two-step solves outside their regions, their NTFF
+  // samples and probes/tick.  The kernels read each solve's step from
+  // device memory, so the same launch sequences (or their captured graphs)
+  // serve every iteration until the gang changes.
+  hipError_t launch_fixed(const Gang& g, int half) {
     hipError_t e = hipSuccess;
-    for (int c = 0; c < 3 && e == hipSuccess; c++)
+    for (int c = 0; c < 3 && e == hipSuccess; c++) {
       for (int v = 0; v < kNumLists && e == hipSuccess; v++) {
-        const int n = n_tiles[c][v];
+        const int n = half ? n_tiles_b[c][v] : n_tiles[c][v];
         if (!n) continue;
         const size_t o = tile_off[c][v];
         e = launch_step<T>(g.sa, g.sids + o, v == kListInterior ? kTy : kTyB, c, list_pm(v), false, g.tiles + o, n, run);
       }
-    if (e == hipSuccess) e = launch_ntff<T>(g.na, g.nb, n_nb, run);
-    if (e == hipSuccess) e = launch_probes<T>(g.xa, (int)sel.size(), run);
+      if (!half && n_ptiles[c] && e == hipSuccess)
+        e = launch_pair<T>(g.sa, g.sids + ptile_off[c], c, g.tiles + ptile_off[c], n_ptiles[c], run);
+    }
+    if (e == hipSuccess) e = launch_ntff<T>(g.na, g.nb, half ? n_nb_b : n_nb, run);
+    if (e == hipSuccess) e = launch_probes<T>(g.xa, half ? (int)n_pair : (int)sel.size(), run);
     return e;
   }
 
   int capture(Gang& g) {
-    if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    if (g.graph) (void)hipGraphDestroy(g.graph);
-    g.exec = nullptr;
-    g.graph = nullptr;
-    HIPCHK(hipStreamBeginCapture(run, hipStreamCaptureModeThreadLocal));
-    const hipError_t e = launch_fixed(g);
-    hipGraph_t gr = nullptr;
-    const hipError_t e2 = hipStreamEndCapture(run, &gr);
-    if (e != hipSuccess || e2 != hipSuccess) {
-      if (gr) (void)hipGraphDestroy(gr);
-      return fail(MSC_EHIP, "gang capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    for (int h = 0; h < 2; h++) {
+      if (g.exec[h]) (void)hipGraphExecDestroy(g.exec[h]);
+      if (g.graph[h]) (void)hipGraphDestroy(g.graph[h]);
+      g.exec[h] = nullptr;
+      g.graph[h] = nullptr;
     }
-    g.graph = gr;
-    HIPCHK(hipGraphInstantiate(&g.exec, gr, nullptr, nullptr, 0));
+    for (int h = 0; h < (n_pair ? 2 : 1); h++) {
+      HIPCHK(hipStreamBeginCapture(run, hipStreamCaptureModeThreadLocal));
+      const hipError_t e = launch_fixed(g, h);
+      hipGraph_t gr = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(run, &gr);
+      if (e != hipSuccess || e2 != hipSuccess) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return fail(MSC_EHIP, "gang capture: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+      }
+      g.graph[h] = gr;
+      HIPCHK(hipGraphInstantiate(&g.exec[h], gr, nullptr, nullptr, 0));
+    }
     return MSC_OK;
   }
 
@@ -452,21 +497,28 @@ struct Server : ServerBase {
     const Gang& g = gang[cur_gang];
     const int slot = (int)(iters % kDepth);
     if (iters >= kDepth) HIPCHK(hipEventSynchronize(iter_ev[slot]));
-    if (g.exec) HIPCHK(hipGraphLaunch(g.exec, run));
-    else HIPCHK(launch_fixed(g));
-    for (size_t q = 0; q < sel.size(); q++) {  // solves whose step closes a sample block
-      const Solver<T>& s = *sel[q]->s;
-      const int t = sel[q]->t;
-      if ((t + 1) % s.na.K == 0 || t + 1 == s.plan.T) HIPCHK(launch_ntff_contract<T>(g.na + q, s.na.uoff[24], run));
+    for (int h = 0; h < (n_pair ? 2 : 1); h++) {
+      if (g.exec[h]) HIPCHK(hipGraphLaunch(g.exec[h], run));
+      else HIPCHK(launch_fixed(g, h));
+      // solves whose step (t, or t+1 in half 1) closes a sample block
+      for (size_t q = 0; q < (h ? n_pair : sel.size()); q++) {
+        const Solver<T>& s = *sel[q]->s;
+        const int t = sel[q]->t + h;
+        if ((t + 1) % s.na.K == 0 || t + 1 == s.plan.T) HIPCHK(launch_ntff_contract<T>(g.na + q, s.na.uoff[24], run));
+      }
     }
     HIPCHK(hipEventRecord(iter_ev[slot], run));
     HIPCHK(hipEventRecord(gang[cur_gang].last_use, run));
     ++iters;
-    steps_done += (long long)sel.size();
-    for (size_t q = 0; q < sel.size();) {
-      Req* r = sel[q];
-      if (++r->t < r->s->plan.T) {
-        ++q;
+    steps_done += (long long)(sel.size() + n_pair);
+    const std::vector<Req*> stepped = sel;  // positions [0, n_pair) stepped twice
+    for (size_t q = 0; q < stepped.size(); q++) {
+      Req* r = stepped[q];
+      const bool two = q < n_pair;
+      r->t += two ? 2 : 1;
+      if (r->t < r->s->plan.T) {
+        // a two-step solve with one step left leaves the two-step group
+        if (two && r->t + 2 > r->s->plan.T) dirty = true;
         continue;
       }
       HIPCHK(hipEventCreateWithFlags(&r->done, hipEventDisableTiming));
@@ -476,7 +528,7 @@ struct Server : ServerBase {
         r->state = MSC_REQ_DRAINING;
       }
       draining.push_back(r);
-      sel.erase(sel.begin() + q);
+      sel.erase(std::find(sel.begin(), sel.end(), r));
       active.erase(std::find(active.begin(), active.end(), r));
       dirty = true;
     }

@@ -843,7 +843,7 @@ int Solver<T>::step(int n) {
       // reads the region's rim at step t+1
       if ((rc = tick(false)) || (rc = lists(tiles2, outer_off, false)) || (rc = tock())) return rc;
       if ((rc = tick(true))) return rc;
-      HIPCHK(launch_pair<T>(sa_dev, cur, tiles2 + outer_off[kNumLists], npair, stream));
+      HIPCHK(launch_pair<T>(sa_dev, nullptr, cur, tiles2 + outer_off[kNumLists], npair, stream));
       if ((rc = tock()) || (rc = after(false))) return rc;
       if ((rc = tick(false)) || (rc = lists(tiles2, outer_off, false)) || (rc = tock())) return rc;
       if ((rc = after(cad1))) return rc;
@@ -947,9 +947,9 @@ size_t solve_footprint(const msc_params* p, int precision) {
   if (precision == 32) s.reset(new Solver<float>());
   else if (precision == 64) s.reset(new Solver<double>());
   else return 0;
-  // the server's settings: its tiling, one step per gang iteration
+  // the server's settings: its tiling; two-step mode in FP32
   s->tiling_from_env = false;
-  s->pair = false;
+  s->pair = precision == 32;
   if (make_plan(p, s->plan) != MSC_OK) return 0;
   return s->layout();
 }
