@@ -126,3 +126,42 @@ def test_scheduling_settings_do_not_change_results(alone, fill, graphs):
         if fill == 1:
             st = srv.stats()
             assert st["iterations"] == st["solve_steps"]
+
+
+def _two_step_case(name, tsteps, **kw):
+    """60x50x205 grid: a two-step region [16,44) x [16,34) x [63,189)."""
+    p, _, _ = cases.case(name)
+    p.update(nx=60, ny=50, nz=205, savefields=0, tsteps=tsteps, nprobes=2, probex=[30, 17], probey=[25, 33],
+             probez=[100, 64], **kw)
+    wp, nu = cases.trail_maps(p, n0=2e13, r0=1.5, nu0=1e6, centre=(0.5, -0.25))
+    return p, wp, nu
+
+
+TWO_STEP_MIX = {
+    "lk61": lambda: _two_step_case("lk_ricker_ez", 61),
+    "di40": lambda: _two_step_case("di_modgauss_ey", 40),
+    "jcol33": lambda: _two_step_case("lk_ricker_ez", 33, j_collocated=1),
+    "small50": lambda: _case("lk_ricker_ez", tsteps=50),  # no two-step region
+}
+
+
+@pytest.mark.parametrize("fill,graphs", [(0, True), (1, False), (5000, True)])
+def test_two_step_gangs_equal_alone(fill, graphs):
+    """Solves with a two-step region advance two steps per gang iteration
+    (the last one alone when their step count is odd) beside one-step
+    solves; fill 1 steps one solve at a time, 5000 a few at a time.  The
+    NTFF payloads and probes equal the single-solve bits."""
+    mix = {k: f() for k, f in TWO_STEP_MIX.items()}
+    alone = {k: _alone(*v) for k, v in mix.items()}
+    with Server(precision=32, budget_bytes=4 << 30, fill_tiles=fill, graphs=graphs) as srv:
+        ids = {k: srv.submit(*v) for k, v in mix.items()}
+        srv.run_until_done()
+        for k in mix:
+            np.testing.assert_array_equal(srv.ntff(ids[k]), alone[k][0], err_msg=k)
+            np.testing.assert_array_equal(srv.probes(ids[k]), alone[k][1], err_msg=k)
+        st = srv.stats()
+        assert st["solve_steps"] == sum(v[0]["tsteps"] for v in mix.values())
+        assert st["iterations"] < st["solve_steps"]
+        if fill == 1:  # one solve per iteration: two-step solves take half the iterations (plus an odd last step)
+            expect = sum((v[0]["tsteps"] + 1) // 2 if k != "small50" else v[0]["tsteps"] for k, v in mix.items())
+            assert st["iterations"] == expect
