@@ -358,3 +358,26 @@ def test_dropin_run_dir_matches_reference_fixture(msc, name, tmp_path):
     assert "All done!" in log
     emax = [float(x) for x in __import__("re").findall(r"Maximum abs\(Ex\) is ([^\s]+)", log)]
     np.testing.assert_allclose(emax, f["emax_log"], rtol=2e-5, atol=1e-12)
+
+
+def test_timing_record_counts_launch_kinds(msc, monkeypatch):
+    """msc_solver_get_timing: in two-step mode a 10-step call runs pairs,
+    reports the region's cell counts and a positive elapsed time; with
+    MSC_PAIR=0 it reports no two-step launches."""
+    Solver, _ = msc
+    p, wp, nu = pair_case("lk_ricker_ez", 0, 40)
+    for pair in (1, 0):
+        monkeypatch.setenv("MSC_PAIR", str(pair))
+        with Solver(p, wp, nu, precision=32) as g:
+            g.step(2)  # past the first frame step
+            g.set_timing(True)
+            g.step(10)
+            g.sync()
+            tm = g.timing()
+        assert tm["steps"] == 10 and tm["span_ms"] > 0 and tm["step_ms"] > 0
+        if pair:
+            assert tm["pair_launches"] >= 4 and tm["pair_ms"] > 0
+            assert tm["pair_cells"] == (44 - 16) * (34 - 16) * (189 - 63)
+            assert 0 < tm["pair_j_cells"] <= tm["pair_cells"]
+        else:
+            assert tm["pair_launches"] == 0 and tm["pair_cells"] == 0
