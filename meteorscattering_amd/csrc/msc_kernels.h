@@ -120,6 +120,7 @@ struct NtffArgs {
   double* acc;              // 24 arrays, each [M][2][n1*n2]
   long long off[24];        // offsets of the 24 arrays in acc
   long long uoff[25];       // offsets of the 24 arrays in one sample slot (uoff[24] = U)
+  long long wofs[25];       // first 16-point wavefront of each array in the MFMA contraction
   T* sbuf;                  // [K][U] surface samples
   const double* sincos;     // [tsteps][M][2]
   const int* tdev;
@@ -150,7 +151,7 @@ hipError_t launch_pair(const StepArgs<T>* a, const uint16_t* sids, int cur, cons
 template <typename T>
 hipError_t launch_ntff(const NtffArgs<T>* a, const uint2* blocks, int nblocks, hipStream_t s);
 template <typename T>
-hipError_t launch_ntff_contract(const NtffArgs<T>* a, long long U, hipStream_t s);
+hipError_t launch_ntff_contract(const NtffArgs<T>* a, long long U, int M, long long nwaves, hipStream_t s);
 template <typename T>
 hipError_t launch_frames(const AuxArgs<T>* a, hipStream_t s);
 template <typename T>

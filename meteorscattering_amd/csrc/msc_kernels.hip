@@ -939,6 +939,67 @@ __global__ void current_catchup(const StepArgs<T>* __restrict__ ap, T* ox, T* oy
   }
 }
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// The same contraction on the matrix cores, for 2M >= 16 frequency rows:
+// per sample array q it is the product acc[q] (2M x n) += Phi (2M x kn) S
+// (kn x n), with S[k][p] the buffered samples and Phi[f][k] =
+// sincos[t0+k][f] (f = 2m + c).  One wavefront per 16 points of one array
+// runs FP64 16x16x4 MFMA tiles over all 2M rows, 8 tiles (128 rows) at a
+// time, so each sample is read once per 128 rows instead of once per 8.
+// Lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and holds
+// D[4 v + l / 16][l % 16] in element v (checked on gfx950).  FP64 products
+// and sums, in a different order than ntff_contract.
+template <typename T>
+__global__ void __launch_bounds__(256) ntff_contract_mfma(const NtffArgs<T>* __restrict__ args) {
+  const ConstArgs<NtffArgs<T>>& a = as_const_space(args);
+  const int t = *a.tdev - 1;
+  if ((t + 1) % a.K != 0 && t + 1 != a.T_steps) return;
+  const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kLanes;
+  if (w >= a.wofs[24]) return;  // uniform per wavefront
+  int q = 0;
+  while (q < 23 && w >= a.wofs[q + 1]) ++q;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int col = lane & 15, kr = lane >> 4;
+  const long long np = a.uoff[q + 1] - a.uoff[q];
+  const long long p = (w - a.wofs[q]) * 16 + col;
+  const bool pv = p < np;
+  const long long U = a.uoff[24];
+  const int kn = t % a.K + 1, t0 = t + 1 - kn, M2 = 2 * a.M;
+  const T* __restrict__ sb = a.sbuf + a.uoff[q] + (pv ? p : 0);
+  const double* __restrict__ tab = a.sincos;
+  double* acc = a.acc + a.off[q] + (pv ? p : 0);
+  for (int f0 = 0; f0 < M2; f0 += 128) {
+    const int ntile = min(8, (M2 - f0 + 15) / 16);
+    v4d d[8];
+#pragma unroll
+    for (int tt = 0; tt < 8; ++tt) d[tt] = v4d{0, 0, 0, 0};
+    for (int k0 = 0; k0 < kn; k0 += 4) {
+      const int k = k0 + kr;
+      const double b = (pv && k < kn) ? (double)sb[(size_t)k * U] : 0.0;
+      const double* row = tab + (size_t)(t0 + (k < kn ? k : 0)) * M2;
+#pragma unroll
+      for (int tt = 0; tt < 8; ++tt) {
+        if (tt < ntile) {
+          const int f = f0 + tt * 16 + col;
+          const double av = (k < kn && f < M2) ? row[f] : 0.0;
+          d[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b, d[tt], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int tt = 0; tt < 8; ++tt) {
+      if (tt < ntile && pv) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int f = f0 + tt * 16 + 4 * v + kr;
+          if (f < M2) acc[(size_t)f * np] += d[tt][v];
+        }
+      }
+    }
+  }
+}
+
 // Surface samples of the running DFT (meteor-fork.cpp:971-1023 for the
 // face-averaged H, :1417-1470 for E) into slot t % K of the sample buffer.
 template <typename T>
@@ -1243,9 +1304,12 @@ hipError_t launch_ntff(const NtffArgs<T>* a, const uint2* blocks, int nblocks, h
 }
 
 template <typename T>
-hipError_t launch_ntff_contract(const NtffArgs<T>* a, long long U, hipStream_t s) {
+hipError_t launch_ntff_contract(const NtffArgs<T>* a, long long U, int M, long long nwaves, hipStream_t s) {
   if (U <= 0) return hipSuccess;
-  hipLaunchKernelGGL((ntff_contract<T>), dim3((unsigned)((U + 255) / 256)), dim3(256), 0, s, a);
+  if (2 * M >= 16)  // matrix cores: 4 wavefronts of 16 points per workgroup
+    hipLaunchKernelGGL((ntff_contract_mfma<T>), dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((ntff_contract<T>), dim3((unsigned)((U + 255) / 256)), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
@@ -1277,7 +1341,7 @@ hipError_t launch_catchup(const StepArgs<T>* a, int cur, T* ox, T* oy, T* oz, hi
                                      hipStream_t);                                                                 \
   template hipError_t launch_pair<T>(const StepArgs<T>*, const uint16_t*, int, const uint32_t*, int, hipStream_t); \
   template hipError_t launch_ntff<T>(const NtffArgs<T>*, const uint2*, int, hipStream_t);                          \
-  template hipError_t launch_ntff_contract<T>(const NtffArgs<T>*, long long, hipStream_t);                        \
+  template hipError_t launch_ntff_contract<T>(const NtffArgs<T>*, long long, int, long long, hipStream_t);         \
   template hipError_t launch_frames<T>(const AuxArgs<T>*, hipStream_t);                                            \
   template hipError_t launch_probes<T>(const AuxArgs<T>*, int, hipStream_t);                                       \
   template hipError_t launch_catchup<T>(const StepArgs<T>*, int, T*, T*, T*, hipStream_t);

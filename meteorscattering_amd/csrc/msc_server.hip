@@ -504,7 +504,8 @@ struct Server : ServerBase {
       for (size_t q = 0; q < (h ? n_pair : sel.size()); q++) {
         const Solver<T>& s = *sel[q]->s;
         const int t = sel[q]->t + h;
-        if ((t + 1) % s.na.K == 0 || t + 1 == s.plan.T) HIPCHK(launch_ntff_contract<T>(g.na + q, s.na.uoff[24], run));
+        if ((t + 1) % s.na.K == 0 || t + 1 == s.plan.T)
+          HIPCHK(launch_ntff_contract<T>(g.na + q, s.na.uoff[24], s.na.M, s.na.wofs[24], run));
       }
     }
     HIPCHK(hipEventRecord(iter_ev[slot], run));

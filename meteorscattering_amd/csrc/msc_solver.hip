@@ -496,6 +496,8 @@ size_t Solver<T>::layout() {
     uo += np;
   }
   na.uoff[24] = (long long)uo;
+  na.wofs[0] = 0;
+  for (int q = 0; q < 24; q++) na.wofs[q + 1] = na.wofs[q] + (na.uoff[q + 1] - na.uoff[q] + 15) / 16;
   acc_n = off;
   // sample-buffer depth: the accumulators' read-modify-write per step
   // (2 x 2M x 8 B per sample point, once per K steps) stays near 2.5% of the
@@ -819,7 +821,8 @@ int Solver<T>::step(int n) {
     HIPCHK(launch_ntff<T>(na_dev, ntff_blocks, (int)ntff_host.size(), stream));
     if (cad) HIPCHK(launch_frames<T>(xa_dev, stream));
     HIPCHK(launch_probes<T>(xa_dev, 1, stream));
-    if ((t_host + 1) % na.K == 0 || t_host + 1 == plan.T) HIPCHK(launch_ntff_contract<T>(na_dev, na.uoff[24], stream));
+    if ((t_host + 1) % na.K == 0 || t_host + 1 == plan.T)
+      HIPCHK(launch_ntff_contract<T>(na_dev, na.uoff[24], na.M, na.wofs[24], stream));
     ++t_host;
     return MSC_OK;
   };

@@ -381,3 +381,23 @@ def test_timing_record_counts_launch_kinds(msc, monkeypatch):
             assert 0 < tm["pair_j_cells"] <= tm["pair_cells"]
         else:
             assert tm["pair_launches"] == 0 and tm["pair_cells"] == 0
+
+
+@pytest.mark.parametrize("nntf", [16, 100])
+@pytest.mark.parametrize("precision,tol", [(64, 1e-10), (32, 2e-5)])
+def test_ntff_matrix_core_contraction_matches_oracle(msc, oracle_lib, nntf, precision, tol):
+    """With 2M >= 16 frequency rows the NTFF contraction runs on the matrix
+    cores (FP64 MFMA tiles; 100 frequencies = 200 rows, two row groups and a
+    partial tile).  75 steps close K-step blocks mid-run and at the end."""
+    Solver, _ = msc
+    p, wp, nu = cases.case("lk_ricker_ez")
+    p.update(tsteps=75, nntf=nntf, fntf=list(np.linspace(30e6, 80e6, nntf)))
+    g = Solver(p, wp, nu, precision=precision)
+    o = oracle_lib.Oracle(p, wp, nu)
+    g.step(75)
+    o.step(75)
+    e = relerr(g.ntff(), o.ntff())
+    print(f"M={nntf} FP{precision}: NTFF rel err {e:.2e}")
+    assert e < tol
+    g.close()
+    o.close()
