@@ -93,14 +93,22 @@ def batch_or_serve(args, copy_gbps) -> dict:
                  "log-uniform, 1500-4000 steps, M=16")
     log(f"{wname}: media built in {time.perf_counter() - t0:.1f} s")
     updates = workloads.cell_updates(trails)
-    ser = workloads.run_serial(trails, precision=args.precision)
+    table = None
+    if args.autotune:
+        # one tiling per size class, tuned on each class's median request;
+        # the server and the serial baseline both use the table
+        from meteorscattering_amd import autotune
+        t1 = time.perf_counter()
+        table = autotune.tune_classes(trails, [4 << 20, 16 << 20], precision=args.precision)
+        log(f"autotuned {len(table)} size classes in {time.perf_counter() - t1:.1f} s: {table}")
+    ser = workloads.run_serial(trails, precision=args.precision, tilings=table)
     log(f"serial: {ser['wall']:.2f} s ({len(trails) / ser['wall'] * 3600:.0f} solves/hr)")
     if args.workload == "batch":
-        res = workloads.run_batch(trails, precision=args.precision)
+        res = workloads.run_batch(trails, precision=args.precision, tilings=table)
         arrivals = np.zeros(len(trails))
     else:
         arrivals = workloads.poisson_arrivals(args.rate, len(trails), args.seed)
-        res = workloads.run_serve(trails, arrivals, precision=args.precision)
+        res = workloads.run_serve(trails, arrivals, precision=args.precision, tilings=table)
     same = all(np.array_equal(a, b, equal_nan=True) for a, b in zip(ser["ntff"], res["ntff"]))
     nonfinite = sum(int(not np.all(np.isfinite(a))) for a in res["ntff"])
     lat = res["latency"] if "latency" in res else np.array([i["t_done"] - i["t_submit"] for i in res["info"]])
@@ -116,7 +124,8 @@ def batch_or_serve(args, copy_gbps) -> dict:
         "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if args.precision == 32 else "f64", "data": "synthetic",
         "config": {"workload": wname, "seed": args.seed, "requests": len(trails),
-                   "cells_total": int(sum(p["nx"] * p["ny"] * p["nz"] for p, _, _ in trails))},
+                   "cells_total": int(sum(p["nx"] * p["ny"] * p["nz"] for p, _, _ in trails)),
+                   "tilings": [{"max_cells": int(m), **t} for m, t in table] if table else "defaults"},
         "gcell_updates_per_s": round(updates / res["wall"] / 1e9, 3),
         "latency_s": {"p50": q(lat, 50), "p99": q(lat, 99), "max": q(lat, 100)},
         "serial_gpu": {"solves_per_hr": round(sph_serial, 1), "wall_s": round(ser["wall"], 3),
@@ -141,6 +150,8 @@ def main():
     ap.add_argument("--rate", type=float, default=2.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--requests", type=int, default=48, help="batch/serve: number of trails")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true",
+                    help="batch/serve: tune one tiling per size class (autotune.tune_classes) first")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the streaming-copy reference (counter calibration)")
     args = ap.parse_args()

@@ -147,6 +147,33 @@ typedef struct msc_solver msc_solver;
 int msc_solver_create(const msc_params* p, const double* wp, const double* nu,
                       int precision, msc_solver** out);
 
+/* Tile shapes of one solve.  No counterpart in the reference (its loops are
+ * untiled).  Results do not depend on them: every cell is computed by the
+ * same code whatever the tiling.  Speed does. */
+typedef struct msc_tiling {
+  int32_t ty;      /* rows of the one-step interior tiles: 3, 7 or 15 */
+  int32_t ty_b;    /* rows of the one-step CPML tiles: 3, 7 or 15 */
+  int32_t chunk;   /* x planes per one-step tile (>= 1) */
+  int32_t chunk2;  /* x planes per two-step tile (>= 1) */
+} msc_tiling;
+
+/* The defaults: 7, 7, 8, 32. */
+void msc_tiling_default(msc_tiling* t);
+
+/* msc_solver_create with the given tiling instead of the defaults (and of
+ * the MSC_TY / MSC_TY_B / MSC_CHUNK / MSC_CHUNK2 environment settings). */
+int msc_solver_create_tiled(const msc_params* p, const double* wp, const double* nu, int precision,
+                            const msc_tiling* tiling, msc_solver** out);
+
+/* Autotuner: times tilings on this solve and returns the fastest.  A
+ * coordinate search from the defaults over chunk2, chunk, ty and ty_b, each
+ * candidate a fresh solver stepped 2 steps and then `steps` timed steps
+ * (at most tsteps-2); a candidate replaces the current one when it is
+ * faster by more than 0.5%.  *best_ms = its elapsed GPU time per step.
+ * No counterpart in the reference. */
+int msc_autotune(const msc_params* p, const double* wp, const double* nu, int precision, int steps,
+                 msc_tiling* best, double* best_ms);
+
 /* Enqueue `nsteps` time steps (meteor-fork.cpp:683-1667) on the solver's
  * stream.  Asynchronous: returns before the GPU finishes. */
 int msc_solver_step(msc_solver* s, int nsteps);
@@ -286,6 +313,11 @@ double msc_server_clock(const msc_server* s);
  * solve, the batch setting); default 16384.  use_graphs: replay each gang's
  * fixed launch sequence as a captured hipGraph (default 1). */
 int msc_server_configure(msc_server* s, int fill_tiles, int use_graphs);
+/* Tilings by size class (e.g. from msc_autotune on one solve per class): a
+ * solve of n cells takes tilings[c] for the first class c with
+ * n <= max_cells[c], the last class if none.  nclasses = 0 restores the
+ * defaults.  Only while no request is queued, resident or draining. */
+int msc_server_set_tilings(msc_server* s, const msc_tiling* tilings, const int64_t* max_cells, int nclasses);
 void msc_server_destroy(msc_server* s);
 
 /*

@@ -134,6 +134,25 @@ class RequestInfo(C.Structure):
     ]
 
 
+class Tiling(C.Structure):
+    """``msc_tiling``."""
+
+    _fields_ = [
+        ("ty", C.c_int32),
+        ("ty_b", C.c_int32),
+        ("chunk", C.c_int32),
+        ("chunk2", C.c_int32),
+    ]
+
+
+def tiling_struct(t: dict) -> Tiling:
+    return Tiling(int(t["ty"]), int(t["ty_b"]), int(t["chunk"]), int(t["chunk2"]))
+
+
+def tiling_dict(t: Tiling) -> dict:
+    return {f: int(getattr(t, f)) for f, _ in t._fields_}
+
+
 class Timing(C.Structure):
     """``msc_timing``."""
 
@@ -177,6 +196,9 @@ def _declare(lib: C.CDLL) -> None:
         "msc_write_run_dir": (C.c_int, [C.c_char_p, P(MscParams), dp, dp]),
         "msc_free": (None, [vp]),
         "msc_solver_create": (C.c_int, [P(MscParams), dp, dp, C.c_int, P(vp)]),
+        "msc_tiling_default": (None, [P(Tiling)]),
+        "msc_solver_create_tiled": (C.c_int, [P(MscParams), dp, dp, C.c_int, P(Tiling), P(vp)]),
+        "msc_autotune": (C.c_int, [P(MscParams), dp, dp, C.c_int, C.c_int, P(Tiling), dp]),
         "msc_solver_step": (C.c_int, [vp, C.c_int]),
         "msc_solver_sync": (C.c_int, [vp]),
         "msc_solver_time": (C.c_int, [vp]),
@@ -206,6 +228,7 @@ def _declare(lib: C.CDLL) -> None:
         "msc_server_get_stats": (C.c_int, [vp, P(ServerStats)]),
         "msc_server_clock": (C.c_double, [vp]),
         "msc_server_configure": (C.c_int, [vp, C.c_int, C.c_int]),
+        "msc_server_set_tilings": (C.c_int, [vp, P(Tiling), P(C.c_int64), C.c_int]),
         "msc_server_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():

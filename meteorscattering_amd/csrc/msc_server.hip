@@ -49,6 +49,7 @@ struct ServerBase {
   virtual int stats(msc_server_stats* out) = 0;
   virtual double clock() const = 0;
   virtual int configure(int fill_tiles, int use_graphs) = 0;
+  virtual int set_tilings(const msc_tiling* tilings, const int64_t* max_cells, int n) = 0;
 };
 
 template <typename T>
@@ -100,7 +101,11 @@ struct Server : ServerBase {
   std::vector<Req*> ready;   // prepared, waiting to join the gang
   std::vector<Req*> active;  // resident and not finished
   std::vector<Req*> sel;     // the subset the current gang lists step
-  static constexpr int kTy = 7, kTyB = 7;  // tile rows of every gang solve
+  // one-step tile rows a gang launch can carry (the kernel's TY variants)
+  static constexpr int kTys[3] = {3, 7, 15};
+  static int ty_index(int ty) { return ty == 3 ? 0 : ty == 7 ? 1 : 2; }
+  // tilings by size class: (max cells, tiling); empty: the defaults
+  std::vector<std::pair<long long, msc_tiling>> tilings;
   int fill_tiles = 16384;   // ~20 rounds of 3 workgroups on each of the 256 CUs
   bool use_graphs = true;     // tiles that keep every CU busy (4 rounds of 3 workgroups x 256 CUs)
   std::vector<Req*> draining;  // all steps launched, waiting for completion
@@ -119,9 +124,9 @@ struct Server : ServerBase {
   // solves' tiles (n_tiles), so its first n_tiles_b entries are step t+1 of
   // the two-step solves; pair tiles per current model; NTFF blocks of the
   // two-step solves first (n_nb_b of them).
-  int n_tiles[3][kNumLists] = {};
-  int n_tiles_b[3][kNumLists] = {};
-  size_t tile_off[3][kNumLists] = {};
+  int n_tiles[3][kNumLists][3] = {};    // [current model][variant][tile rows 3, 7, 15]
+  int n_tiles_b[3][kNumLists][3] = {};
+  size_t tile_off[3][kNumLists][3] = {};
   int n_ptiles[3] = {};
   size_t ptile_off[3] = {};
   int n_nb = 0, n_nb_b = 0;
@@ -140,6 +145,22 @@ struct Server : ServerBase {
     fill_tiles = fill;
     use_graphs = graphs != 0;
     dirty = true;
+    return MSC_OK;
+  }
+  int set_tilings(const msc_tiling* t, const int64_t* max_cells, int n) override {
+    if (n < 0 || (n > 0 && (!t || !max_cells))) return fail(MSC_EINVAL, "bad tiling table");
+    std::vector<std::pair<long long, msc_tiling>> v;
+    for (int c = 0; c < n; c++) {
+      const msc_tiling& x = t[c];
+      auto tyok = [](int r) { return r == 3 || r == 7 || r == 15; };
+      if (!tyok(x.ty) || !tyok(x.ty_b) || x.chunk < 1 || x.chunk2 < 1)
+        return fail(MSC_EINVAL, "tiling class %d: rows must be 3, 7 or 15 and chunks >= 1", c);
+      v.push_back({(long long)max_cells[c], x});
+    }
+    std::lock_guard<std::mutex> g(mu);
+    if (!queue.empty() || !ready.empty() || !active.empty() || !draining.empty() || worker_busy)
+      return fail(MSC_ESTATE, "tilings change only while the server holds no request");
+    tilings = std::move(v);
     return MSC_OK;
   }
 
@@ -225,10 +246,23 @@ struct Server : ServerBase {
   int prepare(Req* r) {
     if (!r->s) {
       r->s.reset(new Solver<T>());
-      r->s->tiling_from_env = false;  // the gang launches kTy / kTyB-row tiles
+      r->s->tiling_from_env = false;  // the size class's tiling (the defaults without a table)
       r->s->pair = sizeof(T) == 4;    // two-step region in FP32 (FP64's kernel spills)
-      r->s->ty = kTy;
-      r->s->ty_b = kTyB;
+      msc_tiling t;
+      msc_tiling_default(&t);
+      if (!tilings.empty()) {
+        const long long n = (long long)r->p.nx * r->p.ny * r->p.nz;
+        t = tilings.back().second;
+        for (const auto& c : tilings)
+          if (n <= c.first) {
+            t = c.second;
+            break;
+          }
+      }
+      r->s->ty = t.ty;
+      r->s->ty_b = t.ty_b;
+      r->s->chunk = t.chunk;
+      r->s->chunk2 = t.chunk2;
       int rc = make_plan(&r->p, r->s->plan);
       if (rc) return rc;
       r->bytes = r->s->layout();
@@ -371,20 +405,22 @@ struct Server : ServerBase {
         ++o;
       }
     };
+    auto rows = [](const Solver<T>& s, int v) { return ty_index(v == kListInterior ? s.ty : s.ty_b); };
     for (int c = 0; c < 3; c++) {
-      for (int v = 0; v < kNumLists; v++) {
-        tile_off[c][v] = o;
-        for (size_t q = 0; q < n_pair; q++) {  // two-step solves: one-step tiles outside the region
-          const Solver<T>& s = *active[q]->s;
-          if (s.cur == c) put(s.pair_host, s.outer_off[v], s.outer_off[v + 1], q);
+      for (int v = 0; v < kNumLists; v++)
+        for (int y = 0; y < 3; y++) {
+          tile_off[c][v][y] = o;
+          for (size_t q = 0; q < n_pair; q++) {  // two-step solves: one-step tiles outside the region
+            const Solver<T>& s = *active[q]->s;
+            if (s.cur == c && rows(s, v) == y) put(s.pair_host, s.outer_off[v], s.outer_off[v + 1], q);
+          }
+          n_tiles_b[c][v][y] = (int)(o - tile_off[c][v][y]);
+          for (size_t q = n_pair; q < ns; q++) {  // the others: every tile
+            const Solver<T>& s = *active[q]->s;
+            if (s.cur == c && rows(s, v) == y) put(s.tiles_host, s.list_off[v], s.list_off[v + 1], q);
+          }
+          n_tiles[c][v][y] = (int)(o - tile_off[c][v][y]);
         }
-        n_tiles_b[c][v] = (int)(o - tile_off[c][v]);
-        for (size_t q = n_pair; q < ns; q++) {  // the others: every tile
-          const Solver<T>& s = *active[q]->s;
-          if (s.cur == c) put(s.tiles_host, s.list_off[v], s.list_off[v + 1], q);
-        }
-        n_tiles[c][v] = (int)(o - tile_off[c][v]);
-      }
       ptile_off[c] = o;
       for (size_t q = 0; q < n_pair; q++) {
         const Solver<T>& s = *active[q]->s;
@@ -427,12 +463,13 @@ struct Server : ServerBase {
   hipError_t launch_fixed(const Gang& g, int half) {
     hipError_t e = hipSuccess;
     for (int c = 0; c < 3 && e == hipSuccess; c++) {
-      for (int v = 0; v < kNumLists && e == hipSuccess; v++) {
-        const int n = half ? n_tiles_b[c][v] : n_tiles[c][v];
-        if (!n) continue;
-        const size_t o = tile_off[c][v];
-        e = launch_step<T>(g.sa, g.sids + o, v == kListInterior ? kTy : kTyB, c, list_pm(v), false, g.tiles + o, n, run);
-      }
+      for (int v = 0; v < kNumLists && e == hipSuccess; v++)
+        for (int y = 0; y < 3 && e == hipSuccess; y++) {
+          const int n = half ? n_tiles_b[c][v][y] : n_tiles[c][v][y];
+          if (!n) continue;
+          const size_t o = tile_off[c][v][y];
+          e = launch_step<T>(g.sa, g.sids + o, kTys[y], c, list_pm(v), false, g.tiles + o, n, run);
+        }
       if (!half && n_ptiles[c] && e == hipSuccess)
         e = launch_pair<T>(g.sa, g.sids + ptile_off[c], c, g.tiles + ptile_off[c], n_ptiles[c], run);
     }
@@ -724,6 +761,11 @@ double msc_server_clock(const msc_server* s) { return (s && s->impl) ? s->impl->
 int msc_server_configure(msc_server* s, int fill_tiles, int use_graphs) {
   SERVER_OK(s);
   return s->impl->configure(fill_tiles, use_graphs);
+}
+
+int msc_server_set_tilings(msc_server* s, const msc_tiling* tilings, const int64_t* max_cells, int nclasses) {
+  SERVER_OK(s);
+  return s->impl->set_tilings(tilings, max_cells, nclasses);
 }
 
 void msc_server_destroy(msc_server* s) { delete s; }

@@ -968,6 +968,33 @@ struct msc_solver {
 #define SOLVER_OK(s) \
   if (!(s) || !(s)->impl) return fail(MSC_EINVAL, "NULL solver")
 
+namespace {
+int check_tiling(const msc_tiling* t) {
+  auto tyok = [](int v) { return v == 3 || v == 7 || v == 15; };
+  if (!tyok(t->ty) || !tyok(t->ty_b)) return fail(MSC_EINVAL, "tiling rows must be 3, 7 or 15 (got %d, %d)", t->ty, t->ty_b);
+  if (t->chunk < 1 || t->chunk2 < 1) return fail(MSC_EINVAL, "tiling chunks must be >= 1 (got %d, %d)", t->chunk, t->chunk2);
+  return MSC_OK;
+}
+
+// a solver with an explicit tiling, set up and synchronised
+int make_tiled(const msc_params* p, const double* wp, const double* nu, int precision, const msc_tiling* t,
+               std::unique_ptr<SolverBase>& s) {
+  if (precision == 32) s.reset(new Solver<float>());
+  else s.reset(new Solver<double>());
+  s->tiling_from_env = false;
+  s->ty = t->ty;
+  s->ty_b = t->ty_b;
+  s->chunk = t->chunk;
+  s->chunk2 = t->chunk2;
+  s->pair = precision == 32;
+  if (const char* e = std::getenv("MSC_PAIR")) s->pair = std::atoi(e) != 0;
+  int rc = make_plan(p, s->plan);
+  if (rc) return rc;
+  if ((rc = s->init(wp, nu))) return rc;
+  return s->sync();
+}
+}  // namespace
+
 extern "C" {
 
 int msc_solver_create(const msc_params* p, const double* wp, const double* nu, int precision, msc_solver** out) {
@@ -988,6 +1015,77 @@ int msc_solver_create(const msc_params* p, const double* wp, const double* nu, i
   msc_solver* h = new msc_solver;
   h->impl = std::move(s);
   *out = h;
+  return MSC_OK;
+}
+
+void msc_tiling_default(msc_tiling* t) {
+  if (!t) return;
+  t->ty = 7;
+  t->ty_b = 7;
+  t->chunk = 8;
+  t->chunk2 = 32;
+}
+
+
+int msc_solver_create_tiled(const msc_params* p, const double* wp, const double* nu, int precision,
+                            const msc_tiling* tiling, msc_solver** out) {
+  if (!p || !wp || !nu || !tiling || !out) return fail(MSC_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (precision != 32 && precision != 64) return fail(MSC_EINVAL, "precision must be 32 or 64 (got %d)", precision);
+  int rc = check_tiling(tiling);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MSC_EHIP, "no HIP device visible");
+  std::unique_ptr<SolverBase> s;
+  if ((rc = make_tiled(p, wp, nu, precision, tiling, s))) return rc;
+  msc_solver* h = new msc_solver;
+  h->impl = std::move(s);
+  *out = h;
+  return MSC_OK;
+}
+
+int msc_autotune(const msc_params* p, const double* wp, const double* nu, int precision, int steps,
+                 msc_tiling* best, double* best_ms) {
+  if (!p || !wp || !nu || !best) return fail(MSC_EINVAL, "NULL argument");
+  if (precision != 32 && precision != 64) return fail(MSC_EINVAL, "precision must be 32 or 64 (got %d)", precision);
+  if (p->tsteps < 4) return fail(MSC_EINVAL, "autotuning needs tsteps >= 4");
+  const int n = std::max(2, std::min(steps, p->tsteps - 2));
+  // elapsed GPU time per step of one candidate
+  auto time_one = [&](const msc_tiling& t, double* ms) -> int {
+    std::unique_ptr<SolverBase> s;
+    int rc = make_tiled(p, wp, nu, precision, &t, s);
+    if (!rc) rc = s->step(2);
+    if (!rc) rc = s->sync();
+    if (rc) return rc;
+    s->timing = true;
+    if ((rc = s->step(n)) || (rc = s->sync())) return rc;
+    *ms = s->span_ms;
+    return MSC_OK;
+  };
+  msc_tiling cur;
+  msc_tiling_default(&cur);
+  double cur_ms = 0;
+  int rc = time_one(cur, &cur_ms);
+  if (rc) return rc;
+  const std::vector<int> chunk2s = {16, 24, 32, 48, 64}, chunks = {4, 6, 8, 12, 16}, tys = {3, 7, 15};
+  auto search = [&](int32_t msc_tiling::*field, const std::vector<int>& values) -> int {
+    const int start = cur.*field;
+    for (int v : values) {
+      if (v == start) continue;
+      msc_tiling c = cur;
+      c.*field = v;
+      double ms = 0;
+      int r = time_one(c, &ms);
+      if (r) return r;
+      if (ms < cur_ms * 0.995) { cur = c; cur_ms = ms; }
+    }
+    return MSC_OK;
+  };
+  if ((rc = search(&msc_tiling::chunk2, chunk2s)) || (rc = search(&msc_tiling::chunk, chunks)) ||
+      (rc = search(&msc_tiling::ty, tys)) || (rc = search(&msc_tiling::ty_b, tys)))
+    return rc;
+  *best = cur;
+  if (best_ms) *best_ms = cur_ms;
   return MSC_OK;
 }
 

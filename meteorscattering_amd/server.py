@@ -49,6 +49,16 @@ class Server:
         self.params: dict[int, dict] = {}
         self._maps: dict[int, tuple] = {}
 
+    def set_tilings(self, classes: list[tuple[int, dict]]) -> None:
+        """Tilings by size class, [(max_cells, tiling dict), ...] in
+        increasing max_cells: a solve of n cells takes the first class with
+        n <= max_cells (the last otherwise).  Only while nothing is queued
+        or resident."""
+        n = len(classes)
+        arr = (_abi.Tiling * max(n, 1))(*[_abi.tiling_struct(t) for _, t in classes])
+        mc = (C.c_int64 * max(n, 1))(*[int(m) for m, _ in classes])
+        _check(self.lib.msc_server_set_tilings(self._h, arr, mc, n), self.lib)
+
     def submit(self, p: dict, wp: np.ndarray, nu: np.ndarray) -> int:
         s = _abi.params_to_struct(p)
         wp = np.ascontiguousarray(wp, dtype=np.float64)

@@ -20,14 +20,22 @@ def _check(rc: int, lib) -> None:
 class Solver:
     """One FDTD solve resident on the GPU."""
 
-    def __init__(self, p: dict, wp: np.ndarray, nu: np.ndarray, precision: int = 32):
+    def __init__(self, p: dict, wp: np.ndarray, nu: np.ndarray, precision: int = 32, tiling: dict | None = None):
+        """`tiling`: {ty, ty_b, chunk, chunk2} (``msc_tiling``, e.g. from
+        autotune.tune); None takes the defaults."""
         self.lib = _abi.load_library()
         self.p = dict(p)
         self._s = _abi.params_to_struct(p)
         wp = np.ascontiguousarray(wp, dtype=np.float64)
         nu = np.ascontiguousarray(nu, dtype=np.float64)
         h = C.c_void_p()
-        _check(self.lib.msc_solver_create(C.byref(self._s), _abi.dptr(wp), _abi.dptr(nu), precision, C.byref(h)), self.lib)
+        if tiling is None:
+            rc = self.lib.msc_solver_create(C.byref(self._s), _abi.dptr(wp), _abi.dptr(nu), precision, C.byref(h))
+        else:
+            t = _abi.tiling_struct(tiling)
+            rc = self.lib.msc_solver_create_tiled(C.byref(self._s), _abi.dptr(wp), _abi.dptr(nu), precision,
+                                                  C.byref(t), C.byref(h))
+        _check(rc, self.lib)
         self._h = h
         self.precision = precision
         self.shape = (p["nx"], p["ny"], p["nz"])

@@ -19,25 +19,39 @@ def cell_updates(trails) -> int:
     return sum(p["nx"] * p["ny"] * p["nz"] * p["tsteps"] for p, _, _ in trails)
 
 
-def run_serial(trails, precision: int = 32) -> dict:
+def class_tiling(table, p: dict):
+    """The tiling a server with size-class table `table` gives solve `p`."""
+    if not table:
+        return None
+    n = p["nx"] * p["ny"] * p["nz"]
+    for m, t in table:
+        if n <= m:
+            return t
+    return table[-1][1]
+
+
+def run_serial(trails, precision: int = 32, tilings=None) -> dict:
     """One solve at a time through msc_solver_*: create (plan, media upload),
-    all steps, NTFF read-back, destroy.  Returns the per-solve wall times."""
+    all steps, NTFF read-back, destroy.  Returns the per-solve wall times.
+    `tilings`: a size-class table as for Server.set_tilings (None: defaults)."""
     times, ntffs = [], []
     t0 = time.perf_counter()
     for p, wp, nu in trails:
         t = time.perf_counter()
-        with Solver(p, wp, nu, precision=precision) as s:
+        with Solver(p, wp, nu, precision=precision, tiling=class_tiling(tilings, p)) as s:
             s.step(p["tsteps"])
             ntffs.append(s.ntff())
         times.append(time.perf_counter() - t)
     return {"wall": time.perf_counter() - t0, "times": np.array(times), "ntff": ntffs}
 
 
-def run_batch(trails, precision: int = 32, budget: int = 0, chunk: int = 4) -> dict:
+def run_batch(trails, precision: int = 32, budget: int = 0, chunk: int = 4, tilings=None) -> dict:
     """Every trail submitted at once to one server that steps every resident
     solve each iteration (throughput setting); wall time to the last result."""
     t0 = time.perf_counter()
     with Server(precision=precision, budget_bytes=budget, fill_tiles=0) as srv:
+        if tilings:
+            srv.set_tilings(tilings)
         ids = [srv.submit(p, wp, nu) for p, wp, nu in trails]
         srv.run_until_done(chunk)
         wall = time.perf_counter() - t0
@@ -63,11 +77,14 @@ def poisson_arrivals(rate: float, n: int, seed: int) -> np.ndarray:
     return np.cumsum(rng.exponential(1.0 / rate, n))
 
 
-def run_serve(trails, arrivals: np.ndarray, precision: int = 32, budget: int = 0, chunk: int = 2) -> dict:
+def run_serve(trails, arrivals: np.ndarray, precision: int = 32, budget: int = 0, chunk: int = 2,
+              tilings=None) -> dict:
     """Requests submitted at their arrival times (server clock) while the
     server keeps stepping the resident solves; latency = completion - arrival."""
     n = len(trails)
     with Server(precision=precision, budget_bytes=budget) as srv:
+        if tilings:
+            srv.set_tilings(tilings)
         base = srv.clock
         ids = [None] * n
         nxt = 0

@@ -401,3 +401,23 @@ def test_ntff_matrix_core_contraction_matches_oracle(msc, oracle_lib, nntf, prec
     assert e < tol
     g.close()
     o.close()
+
+
+def test_autotuned_and_explicit_tilings_give_the_same_bits(msc):
+    """msc_autotune returns a valid tiling; solvers created with it and with
+    an unusual explicit tiling give the default solver's results bit for bit
+    (tilings change speed only)."""
+    Solver, _ = msc
+    from meteorscattering_amd import autotune
+    p, wp, nu = pair_case("lk_ricker_ez", 0, 30)
+    tuned, ms = autotune.tune(p, wp, nu, precision=32, steps=6)
+    assert tuned["ty"] in (3, 7, 15) and tuned["ty_b"] in (3, 7, 15) and tuned["chunk"] >= 1 and tuned["chunk2"] >= 1
+    assert ms > 0
+    outs = []
+    for tiling in (None, tuned, {"ty": 3, "ty_b": 15, "chunk": 5, "chunk2": 7}):
+        g = Solver(p, wp, nu, precision=32, tiling=tiling)
+        g.step(30)
+        outs.append(snapshot(g))
+        g.close()
+    for o in outs[1:]:
+        assert all(np.array_equal(a, b) for a, b in zip(o, outs[0]))

@@ -165,3 +165,23 @@ def test_two_step_gangs_equal_alone(fill, graphs):
         if fill == 1:  # one solve per iteration: two-step solves take half the iterations (plus an odd last step)
             expect = sum((v[0]["tsteps"] + 1) // 2 if k != "small50" else v[0]["tsteps"] for k, v in mix.items())
             assert st["iterations"] == expect
+
+
+def test_size_class_tilings_equal_alone():
+    """Tilings by size class (small grids 3/15-row tiles and 5/7-plane
+    chunks, large ones 15/3 rows and 11/13 planes): the gang launches carry
+    several tile heights at once; results equal the single-solve bits.  The
+    table can change only while the server holds no request."""
+    mix = {k: f() for k, f in TWO_STEP_MIX.items()}
+    alone = {k: _alone(*v) for k, v in mix.items()}
+    small = {"ty": 3, "ty_b": 15, "chunk": 5, "chunk2": 7}
+    large = {"ty": 15, "ty_b": 3, "chunk": 11, "chunk2": 13}
+    with Server(precision=32, budget_bytes=4 << 30, fill_tiles=0) as srv:
+        srv.set_tilings([(100_000, small), (10**12, large)])
+        ids = {k: srv.submit(*v) for k, v in mix.items()}
+        with pytest.raises(RuntimeError):
+            srv.set_tilings([])
+        srv.run_until_done()
+        for k in mix:
+            np.testing.assert_array_equal(srv.ntff(ids[k]), alone[k][0], err_msg=k)
+            np.testing.assert_array_equal(srv.probes(ids[k]), alone[k][1], err_msg=k)
