@@ -131,6 +131,7 @@ def batch_or_serve(args, copy_gbps) -> dict:
         "serial_gpu": {"solves_per_hr": round(sph_serial, 1), "wall_s": round(ser["wall"], 3),
                        "gcell_updates_per_s": round(updates / ser["wall"] / 1e9, 3),
                        "latency_s": {"p50": q(slat, 50), "p99": q(slat, 99), "max": q(slat, 100)}},
+        "server_setup_s": round(res["setup"], 3) if "setup" in res else None,
         "speedup_vs_serial": round(sph / sph_serial, 3),
         "results_identical_to_serial": bool(same),
         "solves_with_nonfinite_ntff": nonfinite,

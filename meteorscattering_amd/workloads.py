@@ -47,9 +47,14 @@ def run_serial(trails, precision: int = 32, tilings=None) -> dict:
 
 def run_batch(trails, precision: int = 32, budget: int = 0, chunk: int = 4, tilings=None) -> dict:
     """Every trail submitted at once to one server that steps every resident
-    solve each iteration (throughput setting); wall time to the last result."""
+    solve each iteration (throughput setting).  `wall` runs from before the
+    server exists to the last result; `setup` is the part of it spent creating
+    the server (its arena allocation: a fraction of a second on fresh device
+    memory, several seconds when a previous server in the same process has
+    just released a large arena)."""
     t0 = time.perf_counter()
     with Server(precision=precision, budget_bytes=budget, fill_tiles=0) as srv:
+        setup = time.perf_counter() - t0
         if tilings:
             srv.set_tilings(tilings)
         ids = [srv.submit(p, wp, nu) for p, wp, nu in trails]
@@ -58,7 +63,7 @@ def run_batch(trails, precision: int = 32, budget: int = 0, chunk: int = 4, tili
         infos = [srv.info(i) for i in ids]
         ntffs = [srv.ntff(i) for i in ids]
         st = srv.stats()
-    return {"wall": wall, "info": infos, "ntff": ntffs, "stats": st}
+    return {"wall": wall, "setup": setup, "info": infos, "ntff": ntffs, "stats": st}
 
 
 def fifo_latencies(arrivals: np.ndarray, service: np.ndarray) -> np.ndarray:
