@@ -90,6 +90,14 @@ struct SolverBase {
   long long jc_single = 0, jc_outer = 0, jc_pair = 0;
   std::vector<uint32_t> pair_host;
   int outer_off[kNumLists + 1] = {};
+  // The one-step lists of one step write disjoint cells of the next set and
+  // read only the current one, so they may run at once: up to kMaxAux
+  // auxiliary streams take lists beside `stream` (fork and join events keep
+  // every other launch ordered on `stream`).
+  static constexpr int kMaxAux = 3;
+  int list_streams = 2;             // auxiliary streams used (0: every list on `stream`)
+  hipStream_t aux[kMaxAux] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[kMaxAux] = {};
   bool timing = false;
   std::vector<hipEvent_t> ev;       // pairs around the step kernels of the last step() call
   std::vector<char> ev_pair;        // per pair: 1 if it brackets an fdtd_pair launch

@@ -320,6 +320,14 @@ size_t Arena::largest_free() const {
 
 SolverBase::~SolverBase() {
   if (stream) (void)hipStreamSynchronize(stream);
+  for (int q = 0; q < kMaxAux; q++) {
+    if (aux[q]) {
+      (void)hipStreamSynchronize(aux[q]);
+      (void)hipStreamDestroy(aux[q]);
+    }
+    if (join_ev[q]) (void)hipEventDestroy(join_ev[q]);
+  }
+  if (fork_ev) (void)hipEventDestroy(fork_ev);
   if (blk) {
     if (arena) arena->release(blk);
     else (void)hipFree(blk);
@@ -600,6 +608,7 @@ int Solver<T>::init(const double* wp, const double* nu) {
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     own_stream = true;
   }
+  if (const char* s = std::getenv("MSC_LIST_STREAMS")) list_streams = std::max(0, std::min(kMaxAux, std::atoi(s)));
 
   if (arena) {
     blk = (char*)arena->alloc(bytes);
@@ -826,10 +835,55 @@ int Solver<T>::step(int n) {
     ++t_host;
     return MSC_OK;
   };
+  // One step's one-step lists.  With auxiliary streams, the lists go to the
+  // least loaded of `stream` and the aux streams, largest first (load = tiles
+  // x tile rows, CPML tiles counted 1.5 times), between a fork and a join on
+  // `stream`.
+  if (list_streams > 0 && !fork_ev) {
+    HIPCHK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+    for (int q = 0; q < std::min(list_streams, kMaxAux); q++) {
+      HIPCHK(hipStreamCreateWithFlags(&aux[q], hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&join_ev[q], hipEventDisableTiming));
+    }
+  }
+  int naux = 0;
+  while (naux < std::min(list_streams, kMaxAux) && aux[naux]) naux++;
   auto lists = [&](const uint32_t* base, const int* off, bool slice) -> int {
-    for (int l = 0; l < kNumLists; l++)
+    if (naux == 0) {
+      for (int l = 0; l < kNumLists; l++)
+        HIPCHK(launch_step<T>(sa_dev, nullptr, l == kListInterior ? ty : ty_b, cur, list_pm(l), slice,
+                              base + off[l], off[l + 1] - off[l], stream));
+      return MSC_OK;
+    }
+    double w[kNumLists];
+    int order[kNumLists];
+    for (int l = 0; l < kNumLists; l++) {
+      w[l] = (double)(off[l + 1] - off[l]) * (l == kListInterior ? ty : 1.5 * ty_b);
+      order[l] = l;
+    }
+    std::sort(order, order + kNumLists, [&](int a, int b) { return w[a] > w[b]; });
+    hipStream_t st[kMaxAux + 1] = {stream};
+    for (int q = 0; q < naux; q++) st[q + 1] = aux[q];
+    double load[kMaxAux + 1] = {};
+    bool used[kMaxAux + 1] = {};
+    HIPCHK(hipEventRecord(fork_ev, stream));
+    for (int l : order) {
+      const int n = off[l + 1] - off[l];
+      if (n == 0) continue;
+      int q = 0;
+      for (int k = 1; k <= naux; k++)
+        if (load[k] < load[q]) q = k;
+      load[q] += w[l];
+      if (q > 0 && !used[q]) HIPCHK(hipStreamWaitEvent(st[q], fork_ev, 0));
+      used[q] = true;
       HIPCHK(launch_step<T>(sa_dev, nullptr, l == kListInterior ? ty : ty_b, cur, list_pm(l), slice,
-                            base + off[l], off[l + 1] - off[l], stream));
+                            base + off[l], n, st[q]));
+    }
+    for (int q = 1; q <= naux; q++) {
+      if (!used[q]) continue;
+      HIPCHK(hipEventRecord(join_ev[q - 1], st[q]));
+      HIPCHK(hipStreamWaitEvent(stream, join_ev[q - 1], 0));
+    }
     return MSC_OK;
   };
   const int npair = pair_host.empty() ? 0 : (int)pair_host.size() - outer_off[kNumLists];
