@@ -9,7 +9,6 @@ namespace msc {
 // Planes of z-tiles: each wavefront covers 64 consecutive k, lane 0 being
 // the k0-1 halo, so a tile owns 63 cells along z.
 constexpr int kLanes = 64;
-constexpr int kOwnZ = kLanes - 1;
 constexpr int kMaxLkDeg = 24;
 constexpr int kLkPolyDeg = 12;  // highest degree the in-kernel Horner takes
 constexpr int kNtffBlock = 256;  // surface points per NTFF workgroup
@@ -143,7 +142,7 @@ struct AuxArgs {
 };
 
 template <typename T>
-hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int cur, int pm, bool slice,
+hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int cur, int pm, bool slice, int zl,
                        const uint32_t* tiles, int n, hipStream_t s);
 template <typename T>
 hipError_t launch_pair(const StepArgs<T>* a, const uint16_t* sids, int cur, const uint32_t* tiles, int n,

@@ -167,8 +167,8 @@ __device__ __forceinline__ bool in_sphere(const A& a, int i, int j, int k) {
   return sqrt(__dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z))) < a.rsph;
 }
 
-template <typename T, int TY, int CUR, int PM, bool SLICE>
-__global__ void __launch_bounds__((TY + 1) * kLanes)
+template <typename T, int TY, int CUR, int PM, bool SLICE, int ZL>
+__global__ void __launch_bounds__((TY + 1) * ZL)
 fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ tiles,
           const uint16_t* __restrict__ sids) {
   constexpr int R = TY + 1;  // rows including the j0-1 halo row
@@ -177,11 +177,11 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   // buffers rotate: plane i (read), plane i+1 (staged at the start of plane
   // i), and plane i-1, which slower threads may still read in phase C of
   // plane i-1 while plane i+1 is being staged.
-  __shared__ T sE[3][3][R + 1][kLanes + 1];
-  __shared__ T sH[3][R][kLanes];
+  __shared__ T sE[3][3][R + 1][ZL + 1];
+  __shared__ T sH[3][R][ZL];
 
-  const int lane = threadIdx.x & (kLanes - 1);
-  const int r = threadIdx.x / kLanes;
+  const int lane = threadIdx.x & (ZL - 1);
+  const int r = threadIdx.x / ZL;
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
   const uint32_t tile = tiles[blk];
   const ConstArgs<StepArgs<T>>& a = as_const_space(args + (sids ? sids[blk] : 0));
@@ -245,7 +245,7 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   const uint32_t sxb = (uint32_t)a.sx * W, syb = (uint32_t)a.sy * W;
 
   const bool xrow = (r == R - 1);
-  const bool xcol = (lane == kLanes - 1);
+  const bool xcol = (lane == ZL - 1);
   const bool rowok = xrow && (j + 1) < ny && k >= 0 && k < nz;
   const bool colok = xcol && j >= 0 && j < ny && (k + 1) < nz;
   // byte offset of (0, j, k); only dereferenced when valid / rowok / colok
@@ -271,7 +271,7 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
     if (colok) { cx = ldo(Exo, c + W); cy = ldo(Eyo, c + W); }
     sE[0][0][r][lane] = ex; sE[0][1][r][lane] = ey; sE[0][2][r][lane] = ez;
     if (xrow) { sE[0][0][R][lane] = rx; sE[0][2][R][lane] = rz; }
-    if (xcol) { sE[0][0][r][kLanes] = cx; sE[0][1][r][kLanes] = cy; }
+    if (xcol) { sE[0][0][r][ZL] = cx; sE[0][1][r][ZL] = cy; }
     if (istart + 1 < nx) {
       const uint32_t cn = c + sxb;
       if (valid) { px = ldo(Exo, cn); py = ldo(Eyo, cn); pz = ldo(Ezo, cn); }
@@ -295,7 +295,7 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
     if (i + 1 < nx) {
       sE[nb][0][r][lane] = px; sE[nb][1][r][lane] = py; sE[nb][2][r][lane] = pz;
       if (xrow) { sE[nb][0][R][lane] = prx; sE[nb][2][R][lane] = prz; }
-      if (xcol) { sE[nb][0][r][kLanes] = pcx; sE[nb][1][r][kLanes] = pcy; }
+      if (xcol) { sE[nb][0][r][ZL] = pcx; sE[nb][1][r][ZL] = pcy; }
     }
     if (i + 1 < ie) {
       const uint32_t cn = c + sxb;
@@ -1229,38 +1229,42 @@ hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStrea
 
 // ---------------------------------------------------------------- launchers
 
-template <typename T, int TY, int CUR, int PM, bool SLICE>
+template <typename T, int TY, int CUR, int PM, bool SLICE, int ZL>
 static hipError_t launch_v(const StepArgs<T>* a, const uint16_t* sids, const uint32_t* tiles, int n, hipStream_t s) {
-  hipLaunchKernelGGL((fdtd_step<T, TY, CUR, PM, SLICE>), dim3(n), dim3((TY + 1) * kLanes), 0, s, a, tiles, sids);
+  hipLaunchKernelGGL((fdtd_step<T, TY, CUR, PM, SLICE, ZL>), dim3(n), dim3((TY + 1) * ZL), 0, s, a, tiles, sids);
   return hipGetLastError();
 }
 
+// 32 z lanes with or without slice capture; 64 z lanes without it
 template <typename T, int TY, int CUR, int PM>
-static hipError_t launch_s(const StepArgs<T>* a, const uint16_t* sids, bool slice, const uint32_t* tiles, int n,
-                           hipStream_t s) {
-  return slice ? launch_v<T, TY, CUR, PM, true>(a, sids, tiles, n, s) : launch_v<T, TY, CUR, PM, false>(a, sids, tiles, n, s);
+static hipError_t launch_s(const StepArgs<T>* a, const uint16_t* sids, bool slice, int zl, const uint32_t* tiles,
+                           int n, hipStream_t s) {
+  if (zl == 64) return slice ? hipErrorInvalidValue : launch_v<T, TY, CUR, PM, false, 64>(a, sids, tiles, n, s);
+  if (zl != 32) return hipErrorInvalidValue;
+  return slice ? launch_v<T, TY, CUR, PM, true, 32>(a, sids, tiles, n, s)
+               : launch_v<T, TY, CUR, PM, false, 32>(a, sids, tiles, n, s);
 }
 
 template <typename T, int TY, int CUR>
-static hipError_t launch_ps(const StepArgs<T>* a, const uint16_t* sids, int pm, bool slice, const uint32_t* tiles,
-                            int n, hipStream_t s) {
+static hipError_t launch_ps(const StepArgs<T>* a, const uint16_t* sids, int pm, bool slice, int zl,
+                            const uint32_t* tiles, int n, hipStream_t s) {
   switch (pm) {
-    case 0: return launch_s<T, TY, CUR, 0>(a, sids, slice, tiles, n, s);
-    case kPmlX: return launch_s<T, TY, CUR, kPmlX>(a, sids, slice, tiles, n, s);
-    case kPmlY: return launch_s<T, TY, CUR, kPmlY>(a, sids, slice, tiles, n, s);
-    case kPmlZ: return launch_s<T, TY, CUR, kPmlZ>(a, sids, slice, tiles, n, s);
-    case kPmlAll: return launch_s<T, TY, CUR, kPmlAll>(a, sids, slice, tiles, n, s);
+    case 0: return launch_s<T, TY, CUR, 0>(a, sids, slice, zl, tiles, n, s);
+    case kPmlX: return launch_s<T, TY, CUR, kPmlX>(a, sids, slice, zl, tiles, n, s);
+    case kPmlY: return launch_s<T, TY, CUR, kPmlY>(a, sids, slice, zl, tiles, n, s);
+    case kPmlZ: return launch_s<T, TY, CUR, kPmlZ>(a, sids, slice, zl, tiles, n, s);
+    case kPmlAll: return launch_s<T, TY, CUR, kPmlAll>(a, sids, slice, zl, tiles, n, s);
   }
   return hipErrorInvalidValue;
 }
 
 template <typename T, int TY>
-static hipError_t launch_c(const StepArgs<T>* a, const uint16_t* sids, int cur, int pm, bool slice,
+static hipError_t launch_c(const StepArgs<T>* a, const uint16_t* sids, int cur, int pm, bool slice, int zl,
                            const uint32_t* tiles, int n, hipStream_t s) {
   switch (cur) {
-    case kCurDI: return launch_ps<T, TY, kCurDI>(a, sids, pm, slice, tiles, n, s);
-    case kCurLKPoly: return launch_ps<T, TY, kCurLKPoly>(a, sids, pm, slice, tiles, n, s);
-    case kCurLKClosed: return launch_ps<T, TY, kCurLKClosed>(a, sids, pm, slice, tiles, n, s);
+    case kCurDI: return launch_ps<T, TY, kCurDI>(a, sids, pm, slice, zl, tiles, n, s);
+    case kCurLKPoly: return launch_ps<T, TY, kCurLKPoly>(a, sids, pm, slice, zl, tiles, n, s);
+    case kCurLKClosed: return launch_ps<T, TY, kCurLKClosed>(a, sids, pm, slice, zl, tiles, n, s);
   }
   return hipErrorInvalidValue;
 }
@@ -1269,15 +1273,16 @@ static hipError_t launch_c(const StepArgs<T>* a, const uint16_t* sids, int cur, 
 // belonging to solve sids[b] of the argument array `a` (sids == nullptr:
 // every tile is solve 0); `pm` is the set of CPML slab axes the variant
 // carries code for (kPmlX | kPmlY | kPmlZ, 0 = interior), `slice` selects
-// the variant capturing E slices.
+// the variant capturing E slices, `zl` the z lanes of a workgroup row (32
+// or 64; tiles own zl - 1 z cells).
 template <typename T>
-hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int cur, int pm, bool slice,
+hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int cur, int pm, bool slice, int zl,
                        const uint32_t* tiles, int n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   switch (ty) {
-    case 3: return launch_c<T, 3>(a, sids, cur, pm, slice, tiles, n, s);
-    case 7: return launch_c<T, 7>(a, sids, cur, pm, slice, tiles, n, s);
-    case 15: return launch_c<T, 15>(a, sids, cur, pm, slice, tiles, n, s);
+    case 3: return launch_c<T, 3>(a, sids, cur, pm, slice, zl, tiles, n, s);
+    case 7: return launch_c<T, 7>(a, sids, cur, pm, slice, zl, tiles, n, s);
+    case 15: return launch_c<T, 15>(a, sids, cur, pm, slice, zl, tiles, n, s);
   }
   return hipErrorInvalidValue;
 }
@@ -1337,7 +1342,7 @@ hipError_t launch_catchup(const StepArgs<T>* a, int cur, T* ox, T* oy, T* oz, hi
 }
 
 #define MSC_INST(T)                                                                                                \
-  template hipError_t launch_step<T>(const StepArgs<T>*, const uint16_t*, int, int, int, bool, const uint32_t*, int, \
+  template hipError_t launch_step<T>(const StepArgs<T>*, const uint16_t*, int, int, int, bool, int, const uint32_t*, int, \
                                      hipStream_t);                                                                 \
   template hipError_t launch_pair<T>(const StepArgs<T>*, const uint16_t*, int, const uint32_t*, int, hipStream_t); \
   template hipError_t launch_ntff<T>(const NtffArgs<T>*, const uint2*, int, hipStream_t);                          \

@@ -270,6 +270,7 @@ struct Server : ServerBase {
       r->s.reset(new Solver<T>());
       r->s->tiling_from_env = false;  // the size class's tiling (the defaults without a table)
       r->s->pair = sizeof(T) == 4;    // two-step region in FP32 (FP64's kernel spills)
+      r->s->zlanes = kServerZLanes;
       msc_tiling t;
       msc_tiling_default(&t);
       if (!tilings.empty()) {
@@ -513,7 +514,8 @@ struct Server : ServerBase {
         if (q > 0 && !used[q]) e = hipStreamWaitEvent(st[q], fork_ev, 0);
         used[q] = true;
         if (e == hipSuccess)
-          e = launch_step<T>(g.sa, g.sids + l.o, kTys[l.y], l.c, list_pm(l.v), false, g.tiles + l.o, l.n, st[q]);
+          e = launch_step<T>(g.sa, g.sids + l.o, kTys[l.y], l.c, list_pm(l.v), false, kServerZLanes, g.tiles + l.o,
+                             l.n, st[q]);
       }
       // every forked stream joins, also after an error (a capture must end joined)
       for (int q = 1; q <= naux; q++) {
@@ -535,7 +537,7 @@ struct Server : ServerBase {
           const int n = half ? n_tiles_b[c][v][y] : n_tiles[c][v][y];
           if (!n) continue;
           const size_t o = tile_off[c][v][y];
-          e = launch_step<T>(g.sa, g.sids + o, kTys[y], c, list_pm(v), false, g.tiles + o, n, run);
+          e = launch_step<T>(g.sa, g.sids + o, kTys[y], c, list_pm(v), false, kServerZLanes, g.tiles + o, n, run);
         }
       if (!half && n_ptiles[c] && e == hipSuccess)
         e = launch_pair<T>(g.sa, g.sids + ptile_off[c], c, g.tiles + ptile_off[c], n_ptiles[c], run);

@@ -38,7 +38,11 @@ struct HostPlan {
 
 int make_plan(const msc_params* p, HostPlan& h);
 
-// Device bytes of one solve (its single block), without allocating.
+// z lanes of the one-step tiles of the server's gang launches
+constexpr int kServerZLanes = 64;
+
+// Device bytes of one solve (its single block) with the server's settings,
+// without allocating.
 size_t solve_footprint(const msc_params* p, int precision);
 
 // One device allocation carved into solve blocks: first fit over a free
@@ -72,6 +76,10 @@ struct SolverBase {
   size_t dev_bytes = 0;
   int t_host = 0;
   int ty = 7, ty_b = 7, chunk = 8;  // interior / CPML-boundary tile rows, x-chunk planes
+  // z lanes of a one-step workgroup row (32: two rows per wave; 64); a
+  // one-step tile owns zlanes - 1 z cells, and the two-step region's z
+  // bounds are multiples of that, so the z bands around it stay full tiles
+  int zlanes = 32;
   bool tiling_from_env = true;      // MSC_TY / MSC_CHUNK may override ty / chunk
   int cur = 0;                      // kCurDI / kCurLKPoly / kCurLKClosed
   std::vector<uint32_t> tiles_host; // the kNumLists tile lists, one after another

@@ -369,15 +369,7 @@ template <typename T>
 size_t Solver<T>::layout() {
   const HostPlan& h = plan;
   const int nx = h.nx, ny = h.ny, nz = h.nz, L = h.L;
-  if (tiling_from_env) {
-    // the FP64 two-step kernel spills registers: one-step unless asked for
-    if (sizeof(T) == 8) pair = false;
-    if (const char* s = std::getenv("MSC_TY")) ty = std::atoi(s);
-    if (const char* s = std::getenv("MSC_TY_B")) ty_b = std::atoi(s);
-    if (const char* s = std::getenv("MSC_CHUNK")) chunk = std::max(1, std::atoi(s));
-    if (const char* s = std::getenv("MSC_PAIR")) pair = std::atoi(s) != 0;
-    if (const char* s = std::getenv("MSC_CHUNK2")) chunk2 = std::max(1, std::atoi(s));
-  }
+  const int ownz = zlanes - 1;  // z cells of a one-step tile (zlanes is 32 or 64)
   // Two-step region: 6 cells clear of every face (L+6: clear of the CPML
   // slabs [0, L] and [n-L-1, n), of the NTFF planes at L+5 and n-L-5 with
   // their one-cell H averages, and far enough in that a one-step tile's
@@ -386,16 +378,16 @@ size_t Solver<T>::layout() {
   if (pair) {
     const int m = L + 6;
     if (nx - m > m && ny - m > m) {
-      // z bounds on the 63-cell z tiles: the one-step z-slab tiles stay
-      // full width (a narrow [0, L+6) tile costs as much as a full one)
-      const int zlo = (m + kOwnZ - 1) / kOwnZ * kOwnZ, zhi = (nz - m) / kOwnZ * kOwnZ;
+      // z bounds on the one-step z tiles: the z-band tiles below and above
+      // the region stay full width (a narrow tile costs as much as a full one)
+      const int zlo = (m + ownz - 1) / ownz * ownz, zhi = (nz - m) / ownz * ownz;
       if (zhi > zlo) { x2lo = m; x2hi = nx - m; y2lo = m; y2hi = ny - m; z2lo = zlo; z2hi = zhi; }
     }
   }
   const bool region = x2hi > x2lo;
   // Tiles: x and y are cut at the CPML slab edges (L+1 and n-L-1), so a tile
   // either meets a slab on some axis (the variant carrying the CPML code) or
-  // meets none (the interior variant); z keeps full 63-wide tiles from 0,
+  // meets none (the interior variant); z keeps full ownz-wide tiles from 0,
   // the first and last of which hold the z slabs.  A tile's one-cell low
   // halo (row/column/plane L when the tile starts at L+1) needs no CPML
   // term: slab_h(L) is outside both H slabs, and no E is computed there.
@@ -421,7 +413,7 @@ size_t Solver<T>::layout() {
   // L+1 and n-L-1, so a row is in a slab zone in both or in neither
   ys_ = cut(ny, ty, true, y2lo, y2hi);
   ysb_ = cut(ny, h.p.dopml ? ty_b : ty, true, 0, 0);
-  zs_ = cut(nz, kOwnZ, false, z2lo, z2hi);
+  zs_ = cut(nz, ownz, false, z2lo, z2hi);
   // fdtd_pair tiles: pieces of near-equal length
   auto even = [](int lo, int hi, int piece) {
     std::vector<int> st;
@@ -596,8 +588,22 @@ int Solver<T>::init(const double* wp, const double* nu) {
   const HostPlan& h = plan;
   const int nx = h.nx, ny = h.ny, nz = h.nz, L = h.L;
   const size_t N = (size_t)nx * ny * nz;
+  if (tiling_from_env) {
+    // the FP64 two-step kernel spills registers: one-step unless asked for
+    if (sizeof(T) == 8) pair = false;
+    if (const char* s = std::getenv("MSC_TY")) ty = std::atoi(s);
+    if (const char* s = std::getenv("MSC_TY_B")) ty_b = std::atoi(s);
+    if (const char* s = std::getenv("MSC_CHUNK")) chunk = std::max(1, std::atoi(s));
+    if (const char* s = std::getenv("MSC_PAIR")) pair = std::atoi(s) != 0;
+    if (const char* s = std::getenv("MSC_CHUNK2")) chunk2 = std::max(1, std::atoi(s));
+    if (const char* s = std::getenv("MSC_ZLANES")) zlanes = std::atoi(s);
+  }
   auto tyok = [](int v) { return v == 3 || v == 7 || v == 15; };
   if (!tyok(ty) || !tyok(ty_b)) return fail(MSC_EINVAL, "MSC_TY and MSC_TY_B must be 3, 7 or 15");
+  if (chunk < 1 || chunk2 < 1) return fail(MSC_EINVAL, "x chunks must be >= 1 plane");
+  if (zlanes != 32 && zlanes != 64) return fail(MSC_EINVAL, "MSC_ZLANES must be 32 or 64");
+  if (zlanes == 64 && plan.p.savefields)
+    return fail(MSC_EINVAL, "64-lane one-step tiles carry no slice capture: savefields needs 32 lanes");
   if ((double)N * sizeof(T) >= 4294967296.0)
     return fail(MSC_EINVAL, "grid of %zu cells exceeds the 4 GiB-per-array limit of the 32-bit offset kernel", N);
   const size_t bytes = layout();
@@ -851,7 +857,7 @@ int Solver<T>::step(int n) {
   auto lists = [&](const uint32_t* base, const int* off, bool slice) -> int {
     if (naux == 0) {
       for (int l = 0; l < kNumLists; l++)
-        HIPCHK(launch_step<T>(sa_dev, nullptr, l == kListInterior ? ty : ty_b, cur, list_pm(l), slice,
+        HIPCHK(launch_step<T>(sa_dev, nullptr, l == kListInterior ? ty : ty_b, cur, list_pm(l), slice, zlanes,
                               base + off[l], off[l + 1] - off[l], stream));
       return MSC_OK;
     }
@@ -876,7 +882,7 @@ int Solver<T>::step(int n) {
       load[q] += w[l];
       if (q > 0 && !used[q]) HIPCHK(hipStreamWaitEvent(st[q], fork_ev, 0));
       used[q] = true;
-      HIPCHK(launch_step<T>(sa_dev, nullptr, l == kListInterior ? ty : ty_b, cur, list_pm(l), slice,
+      HIPCHK(launch_step<T>(sa_dev, nullptr, l == kListInterior ? ty : ty_b, cur, list_pm(l), slice, zlanes,
                             base + off[l], n, st[q]));
     }
     for (int q = 1; q <= naux; q++) {
@@ -1007,6 +1013,7 @@ size_t solve_footprint(const msc_params* p, int precision) {
   // the server's settings: its tiling; two-step mode in FP32
   s->tiling_from_env = false;
   s->pair = precision == 32;
+  s->zlanes = kServerZLanes;
   if (make_plan(p, s->plan) != MSC_OK) return 0;
   return s->layout();
 }

@@ -135,7 +135,7 @@ def test_noncubic_grid_matches_oracle(msc, oracle_lib):
 def test_interior_tiles_with_cpml_match_oracle(msc, oracle_lib, monkeypatch):
     """A grid large enough that tiles clear of every CPML slab (and of the
     slab halos) run the interior variant while the rest run the CPML variant:
-    z = 160 gives a middle 63-cell z tile clear of both z slabs, 8-plane x
+    z = 160 gives middle 31-cell z tiles clear of both z slabs, 8-plane x
     chunks and 7-row y tiles leave interior chunks and rows inside 60 x 48."""
     Solver, _ = msc
     monkeypatch.setenv("MSC_CHUNK", "8")
@@ -173,10 +173,11 @@ def test_tilings_are_bit_identical(msc, oracle_lib, monkeypatch):
 
 
 # Two-step grid: with L = 10 the two-step region is [16,44) x [16,34) x
-# [63,189); it holds the TF/SF x and y faces (20, 40; 20, 30) and the high
-# z face (185), the plate or sphere, the trail, and probes near its faces.
+# [31,186) (z bounds on the 31-cell one-step z tiles); it holds the TF/SF x
+# and y faces (20, 40; 20, 30) and the high z face (185, the region's last
+# layer), the plate or sphere, the trail, and probes in its two rim layers.
 PAIR_GRID = dict(nx=60, ny=50, nz=205, nprobes=3, probex=[30, 17, 43], probey=[25, 33, 16],
-                 probez=[100, 64, 188])
+                 probez=[100, 32, 184])
 
 
 def pair_case(name, jcol, tsteps):
@@ -216,6 +217,43 @@ def test_two_step_tiles_are_bit_identical(msc, monkeypatch, name, jcol, chunk2, 
     labels = list(_abi.COMPONENTS) + ["ntff", "probes"] + [f"frame{f}" for f in range(len(outs[0]) - 11)]
     bad = [f"{lab}: {worst(a, b)}" for lab, a, b in zip(labels, outs[1], outs[0]) if not np.array_equal(a, b)]
     assert not bad, "\n".join(bad)
+
+
+@pytest.mark.parametrize("name", ["lk_ricker_ez", "lk_nopml_lhcp", "plate_ricker_45"])
+@pytest.mark.parametrize("pair,precision", [(1, 32), (0, 32), (0, 64)])
+def test_lane_widths_give_the_same_bits(msc, monkeypatch, name, pair, precision):
+    """One-step tiles of 64 z lanes (63 owned z cells; the server's width)
+    and of 32 lanes (two rows per wave, 31 cells; the single-solve default,
+    whose two-step region then starts and ends on multiples of 31) give the
+    same fields, NTFF payload and probes bit for bit."""
+    Solver, _ = msc
+    p, wp, nu = pair_case(name, 0, 61)
+    p.update(savefields=0)
+    monkeypatch.setenv("MSC_PAIR", str(pair))
+    outs = []
+    for lanes in (64, 32):
+        monkeypatch.setenv("MSC_ZLANES", str(lanes))
+        with Solver(p, wp, nu, precision=precision) as g:
+            for c in (1, 6, 23, 31):
+                g.step(c)
+            outs.append(snapshot(g))
+    labels = list(_abi.COMPONENTS) + ["ntff", "probes"]
+    bad = [f"{lab}: {worst(a, b)}" for lab, a, b in zip(labels, outs[1], outs[0]) if not np.array_equal(a, b)]
+    assert not bad, "\n".join(bad)
+
+
+def test_lane_width_is_checked(msc, monkeypatch):
+    """MSC_ZLANES other than 32 or 64 is refused at creation, as is 64 with
+    slice capture (those tiles carry no slice code)."""
+    Solver, _ = msc
+    p, wp, nu = pair_case("lk_ricker_ez", 0, 10)
+    monkeypatch.setenv("MSC_ZLANES", "48")
+    with pytest.raises(RuntimeError, match="MSC_ZLANES"):
+        Solver(p, wp, nu, precision=32)
+    monkeypatch.setenv("MSC_ZLANES", "64")
+    p.update(savefields=1)
+    with pytest.raises(RuntimeError, match="slice"):
+        Solver(p, wp, nu, precision=32)
 
 
 @pytest.mark.parametrize("pair", [0, 1])
@@ -377,7 +415,7 @@ def test_timing_record_counts_launch_kinds(msc, monkeypatch):
         assert tm["steps"] == 10 and tm["span_ms"] > 0 and tm["step_ms"] > 0
         if pair:
             assert tm["pair_launches"] >= 4 and tm["pair_ms"] > 0
-            assert tm["pair_cells"] == (44 - 16) * (34 - 16) * (189 - 63)
+            assert tm["pair_cells"] == (44 - 16) * (34 - 16) * (186 - 31)
             assert 0 < tm["pair_j_cells"] <= tm["pair_cells"]
         else:
             assert tm["pair_launches"] == 0 and tm["pair_cells"] == 0

@@ -129,7 +129,9 @@ def test_scheduling_settings_do_not_change_results(alone, fill, graphs):
 
 
 def _two_step_case(name, tsteps, **kw):
-    """60x50x205 grid: a two-step region [16,44) x [16,34) x [63,189)."""
+    """60x50x205 grid: in the server (64-lane one-step tiles) a two-step
+    region [16,44) x [16,34) x [63,189) with a probe in its rim; alone (32
+    lanes) the region is [16,44) x [16,34) x [31,186), and the bits match."""
     p, _, _ = cases.case(name)
     p.update(nx=60, ny=50, nz=205, savefields=0, tsteps=tsteps, nprobes=2, probex=[30, 17], probey=[25, 33],
              probez=[100, 64], **kw)
