@@ -242,6 +242,26 @@ def test_lane_widths_give_the_same_bits(msc, monkeypatch, name, pair, precision)
     assert not bad, "\n".join(bad)
 
 
+@pytest.mark.parametrize("pair", [1, 0])
+def test_concurrent_lists_give_the_same_bits(msc, monkeypatch, pair):
+    """The one-step lists of a step on 0 (one after another), 1 and 3
+    auxiliary streams give the same fields, NTFF payload, probes and frames
+    bit for bit: they write disjoint cells and read only the current set."""
+    Solver, _ = msc
+    p, wp, nu = pair_case("lk_ricker_ez", 0, 61)
+    monkeypatch.setenv("MSC_PAIR", str(pair))
+    outs = []
+    for ns in (0, 1, 3):
+        monkeypatch.setenv("MSC_LIST_STREAMS", str(ns))
+        with Solver(p, wp, nu, precision=32) as g:
+            for c in (1, 6, 23, 31):
+                g.step(c)
+            outs.append(snapshot(g))
+    for o in outs[1:]:
+        bad = [i for i, (a, b) in enumerate(zip(o, outs[0])) if not np.array_equal(a, b)]
+        assert not bad, bad
+
+
 def test_lane_width_is_checked(msc, monkeypatch):
     """MSC_ZLANES other than 32 or 64 is refused at creation, as is 64 with
     slice capture (those tiles carry no slice code)."""

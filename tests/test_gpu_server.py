@@ -169,6 +169,23 @@ def test_two_step_gangs_equal_alone(fill, graphs):
             assert st["iterations"] == expect
 
 
+@pytest.mark.parametrize("streams,graphs", [(0, True), (1, True), (1, False)])
+def test_gang_lists_on_fewer_streams_equal_alone(monkeypatch, streams, graphs):
+    """The gang's one-step launches of a half on the run stream alone (0
+    auxiliary streams) or beside one auxiliary stream, captured or launched
+    directly: NTFF payloads and probes equal the single-solve bits (the
+    default, two auxiliary streams, is covered above)."""
+    mix = {k: f() for k, f in TWO_STEP_MIX.items()}
+    alone = {k: _alone(*v) for k, v in mix.items()}
+    monkeypatch.setenv("MSC_LIST_STREAMS", str(streams))
+    with Server(precision=32, budget_bytes=4 << 30, fill_tiles=0, graphs=graphs) as srv:
+        ids = {k: srv.submit(*v) for k, v in mix.items()}
+        srv.run_until_done()
+        for k in mix:
+            np.testing.assert_array_equal(srv.ntff(ids[k]), alone[k][0], err_msg=k)
+            np.testing.assert_array_equal(srv.probes(ids[k]), alone[k][1], err_msg=k)
+
+
 def test_size_class_tilings_equal_alone():
     """Tilings by size class (small grids 3/15-row tiles and 5/7-plane
     chunks, large ones 15/3 rows and 11/13 planes): the gang launches carry
