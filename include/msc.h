@@ -166,7 +166,8 @@ int msc_solver_create_tiled(const msc_params* p, const double* wp, const double*
                             const msc_tiling* tiling, msc_solver** out);
 
 /* Autotuner: times tilings on this solve and returns the fastest.  A
- * coordinate search from the defaults over chunk2, chunk, ty and ty_b, each
+ * coordinate search from the defaults over chunk2 (when the solve runs
+ * two-step tiles: FP32, or MSC_PAIR=1), chunk, ty and ty_b, each
  * candidate a fresh solver stepped 2 steps and then `steps` timed steps
  * (at most tsteps-2); a candidate replaces the current one when it is
  * faster by more than 0.5%.  *best_ms = its elapsed GPU time per step.

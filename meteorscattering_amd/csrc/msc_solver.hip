@@ -1038,6 +1038,12 @@ int check_tiling(const msc_tiling* t) {
 }
 
 // a solver with an explicit tiling, set up and synchronised
+// Two-step mode of a solve with an explicit tiling: FP32, unless MSC_PAIR says otherwise.
+bool tiled_pair(int precision) {
+  if (const char* e = std::getenv("MSC_PAIR")) return std::atoi(e) != 0;
+  return precision == 32;
+}
+
 int make_tiled(const msc_params* p, const double* wp, const double* nu, int precision, const msc_tiling* t,
                std::unique_ptr<SolverBase>& s) {
   if (precision == 32) s.reset(new Solver<float>());
@@ -1047,8 +1053,7 @@ int make_tiled(const msc_params* p, const double* wp, const double* nu, int prec
   s->ty_b = t->ty_b;
   s->chunk = t->chunk;
   s->chunk2 = t->chunk2;
-  s->pair = precision == 32;
-  if (const char* e = std::getenv("MSC_PAIR")) s->pair = std::atoi(e) != 0;
+  s->pair = tiled_pair(precision);
   int rc = make_plan(p, s->plan);
   if (rc) return rc;
   if ((rc = s->init(wp, nu))) return rc;
@@ -1142,8 +1147,10 @@ int msc_autotune(const msc_params* p, const double* wp, const double* nu, int pr
     }
     return MSC_OK;
   };
-  if ((rc = search(&msc_tiling::chunk2, chunk2s)) || (rc = search(&msc_tiling::chunk, chunks)) ||
-      (rc = search(&msc_tiling::ty, tys)) || (rc = search(&msc_tiling::ty_b, tys)))
+  // chunk2 shapes only two-step tiles: searched only when the candidates run them
+  if ((tiled_pair(precision) && (rc = search(&msc_tiling::chunk2, chunk2s))) ||
+      (rc = search(&msc_tiling::chunk, chunks)) || (rc = search(&msc_tiling::ty, tys)) ||
+      (rc = search(&msc_tiling::ty_b, tys)))
     return rc;
   *best = cur;
   if (best_ms) *best_ms = cur_ms;
