@@ -91,6 +91,9 @@ struct SolverBase {
   // then the fdtd_pair tiles (from outer_off[kNumLists] to the end).
   bool pair = true;
   int chunk2 = 32;                  // x planes per fdtd_pair tile
+  // fewest fdtd_pair tiles for which a solve uses the two-step region (0:
+  // any); single solves need enough to fill the GPU, server gangs pool them
+  long long min_pair_tiles = 0;
   long long pair_cells = 0;         // cells of the two-step region (0: none)
   // owned cells of tiles that carry a current (see kTileNoCurrent): of the
   // one-step tiling, of the one-step tiles outside the region, of the

@@ -381,7 +381,14 @@ size_t Solver<T>::layout() {
       // z bounds on the one-step z tiles: the z-band tiles below and above
       // the region stay full width (a narrow tile costs as much as a full one)
       const int zlo = (m + ownz - 1) / ownz * ownz, zhi = (nz - m) / ownz * ownz;
-      if (zhi > zlo) { x2lo = m; x2hi = nx - m; y2lo = m; y2hi = ny - m; z2lo = zlo; z2hi = zhi; }
+      // the region's tile count, as the even split below makes it
+      auto pieces = [](int n, int piece) { return (long long)((n + piece - 1) / piece); };
+      const long long nt2 = zhi > zlo ? pieces(nx - 2 * m, chunk2) * pieces(ny - 2 * m, kPairRows - 4) *
+                                            pieces(zhi - zlo, kPairOwnZ)
+                                      : 0;
+      if (zhi > zlo && nt2 >= min_pair_tiles) {
+        x2lo = m; x2hi = nx - m; y2lo = m; y2hi = ny - m; z2lo = zlo; z2hi = zhi;
+      }
     }
   }
   const bool region = x2hi > x2lo;
@@ -1044,6 +1051,22 @@ bool tiled_pair(int precision) {
   return precision == 32;
 }
 
+// A single solve uses its two-step region only when the region gives at
+// least 1.5 fdtd_pair workgroups (one per CU at a time) per CU of the
+// current device: with fewer, most CUs idle through the long two-step
+// marches and the one-step tiling is faster (on 48 random trails of 1-22 M cells
+// the break-even lies near 400 tiles on 256 CUs).  MSC_PAIR=1 uses the
+// region whatever its size.
+void set_min_pair_tiles(SolverBase& s) {
+  s.min_pair_tiles = 0;
+  if (const char* e = std::getenv("MSC_PAIR"))
+    if (std::atoi(e) != 0) return;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+    s.min_pair_tiles = 3LL * cus / 2;
+}
+
 int make_tiled(const msc_params* p, const double* wp, const double* nu, int precision, const msc_tiling* t,
                std::unique_ptr<SolverBase>& s) {
   if (precision == 32) s.reset(new Solver<float>());
@@ -1054,6 +1077,7 @@ int make_tiled(const msc_params* p, const double* wp, const double* nu, int prec
   s->chunk = t->chunk;
   s->chunk2 = t->chunk2;
   s->pair = tiled_pair(precision);
+  set_min_pair_tiles(*s);
   int rc = make_plan(p, s->plan);
   if (rc) return rc;
   if ((rc = s->init(wp, nu))) return rc;
@@ -1072,6 +1096,7 @@ int msc_solver_create(const msc_params* p, const double* wp, const double* nu, i
   std::unique_ptr<SolverBase> s;
   if (precision == 32) s.reset(new Solver<float>());
   else s.reset(new Solver<double>());
+  set_min_pair_tiles(*s);
   int rc = make_plan(p, s->plan);
   if (rc) return rc;
   rc = s->init(wp, nu);

@@ -421,11 +421,16 @@ def test_dropin_run_dir_matches_reference_fixture(msc, name, tmp_path):
 def test_timing_record_counts_launch_kinds(msc, monkeypatch):
     """msc_solver_get_timing: in two-step mode a 10-step call runs pairs,
     reports the region's cell counts and a positive elapsed time; with
-    MSC_PAIR=0 it reports no two-step launches."""
+    MSC_PAIR=0 it reports no two-step launches, and neither does the default
+    here: the region's 6 two-step tiles are far fewer than 1.5 per CU, so a
+    single solve runs it one-step."""
     Solver, _ = msc
     p, wp, nu = pair_case("lk_ricker_ez", 0, 40)
-    for pair in (1, 0):
-        monkeypatch.setenv("MSC_PAIR", str(pair))
+    for pair in (1, 0, None):
+        if pair is None:
+            monkeypatch.delenv("MSC_PAIR", raising=False)
+        else:
+            monkeypatch.setenv("MSC_PAIR", str(pair))
         with Solver(p, wp, nu, precision=32) as g:
             g.step(2)  # past the first frame step
             g.set_timing(True)
@@ -461,11 +466,12 @@ def test_ntff_matrix_core_contraction_matches_oracle(msc, oracle_lib, nntf, prec
     o.close()
 
 
-def test_autotuned_and_explicit_tilings_give_the_same_bits(msc):
+def test_autotuned_and_explicit_tilings_give_the_same_bits(msc, monkeypatch):
     """msc_autotune returns a valid tiling; solvers created with it and with
     an unusual explicit tiling give the default solver's results bit for bit
-    (tilings change speed only)."""
+    (tilings change speed only; two-step tiles forced on this small grid)."""
     Solver, _ = msc
+    monkeypatch.setenv("MSC_PAIR", "1")
     from meteorscattering_amd import autotune
     p, wp, nu = pair_case("lk_ricker_ez", 0, 30)
     tuned, ms = autotune.tune(p, wp, nu, precision=32, steps=6)
