@@ -142,7 +142,10 @@ typedef struct msc_solver msc_solver;
  * media, CPML tables, source table and incident line (meteor-fork.cpp:
  * 119-646), device state allocated and zeroed.  `precision` is 32 (fields,
  * J and CPML in FP32; plasma coefficients FP64; NTFF sums FP64) or 64.
- * `wp`/`nu` as returned by msc_read_run_dir.
+ * `wp`/`nu` as returned by msc_read_run_dir.  The tiling is the defaults
+ * of msc_tiling_default with one change: one-step x chunks of 4 planes
+ * below 24 M cells.  The two-step region is used in FP32 when it gives at
+ * least 1.5 two-step tiles per CU.
  */
 int msc_solver_create(const msc_params* p, const double* wp, const double* nu,
                       int precision, msc_solver** out);

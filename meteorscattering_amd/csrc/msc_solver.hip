@@ -600,6 +600,10 @@ int Solver<T>::init(const double* wp, const double* nu) {
     if (sizeof(T) == 8) pair = false;
     if (const char* s = std::getenv("MSC_TY")) ty = std::atoi(s);
     if (const char* s = std::getenv("MSC_TY_B")) ty_b = std::atoi(s);
+    // one-step x chunks of 4 planes below 24 M cells: there the tiles are
+    // few and their marches latency-bound, and shorter marches pay (3-27%
+    // faster on 8 grids of 1.3-22 M cells, 3% slower at 33 M and 128 M)
+    if (N < 24000000 && nx / 4 + 8 <= kMaxChunks) chunk = 4;
     if (const char* s = std::getenv("MSC_CHUNK")) chunk = std::max(1, std::atoi(s));
     if (const char* s = std::getenv("MSC_PAIR")) pair = std::atoi(s) != 0;
     if (const char* s = std::getenv("MSC_CHUNK2")) chunk2 = std::max(1, std::atoi(s));
