@@ -93,11 +93,12 @@ struct Server : ServerBase {
   int device = 0;
   hipStream_t run = nullptr, setup = nullptr;
   // The one-step launches of a half write disjoint cells of their solves'
-  // next sets and read only the current ones: they spread over `run` and
-  // up to kAux auxiliary streams between a fork and a join (in a captured
-  // graph, parallel branches).
+  // next sets and read only the current ones: with MSC_LIST_STREAMS=1 or 2
+  // they spread over `run` and that many auxiliary streams between a fork
+  // and a join (in a captured graph, parallel branches).  Off by default:
+  // a full gang already fills the GPU, so it brings no measurable gain.
   static constexpr int kAux = 2;
-  int list_streams = kAux;
+  int list_streams = 0;
   hipStream_t aux[kAux] = {};
   hipEvent_t fork_ev = nullptr, join_ev[kAux] = {};
   Arena arena;

@@ -169,12 +169,12 @@ def test_two_step_gangs_equal_alone(fill, graphs):
             assert st["iterations"] == expect
 
 
-@pytest.mark.parametrize("streams,graphs", [(0, True), (1, True), (1, False)])
-def test_gang_lists_on_fewer_streams_equal_alone(monkeypatch, streams, graphs):
-    """The gang's one-step launches of a half on the run stream alone (0
-    auxiliary streams) or beside one auxiliary stream, captured or launched
-    directly: NTFF payloads and probes equal the single-solve bits (the
-    default, two auxiliary streams, is covered above)."""
+@pytest.mark.parametrize("streams,graphs", [(2, True), (1, True), (1, False)])
+def test_gang_lists_on_several_streams_equal_alone(monkeypatch, streams, graphs):
+    """The gang's one-step launches of a half spread over the run stream and
+    one or two auxiliary streams (MSC_LIST_STREAMS), captured as parallel
+    graph branches or launched directly: NTFF payloads and probes equal the
+    single-solve bits (the default, the run stream alone, is covered above)."""
     mix = {k: f() for k, f in TWO_STEP_MIX.items()}
     alone = {k: _alone(*v) for k, v in mix.items()}
     monkeypatch.setenv("MSC_LIST_STREAMS", str(streams))
