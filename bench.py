@@ -1,6 +1,6 @@
 """Benchmark of the fused FDTD step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config2|config1] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 400x400x800|96x96x192] [--no-cpu]
     python bench.py --workload batch [--requests 48]          # 48-trail sweep, server vs serial
     python bench.py --workload serve --rate R [--requests N]  # Poisson stream, server vs serial
 
@@ -74,8 +74,9 @@ def traffic_from_profile(kernel: str, workload: str):
 
 
 def batch_or_serve(args, copy_gbps) -> dict:
-    """Config 4 (--workload batch: every trail submitted at once) or config 5
-    (--workload serve: Poisson arrivals), on the multi-solve server, against
+    """The trail sweep (--workload batch: every trail submitted at once) or
+    the request stream (--workload serve: Poisson arrivals), on the
+    multi-solve server, against
     the same trails run one at a time through the single-solve path.  The
     synthetic media are built before any timing; both timings include each
     solve's device set-up and NTFF read-back.  The serial baseline's serving
@@ -85,11 +86,11 @@ def batch_or_serve(args, copy_gbps) -> dict:
 
     t0 = time.perf_counter()
     if args.workload == "batch":
-        trails = configs.config4(seed=args.seed, count=args.requests)
-        wname = f"config-4 sweep: {args.requests} trails, nx,ny U[64,256], nz U[128,512], 1500-4000 steps, M=1"
+        trails = configs.trail_sweep(seed=args.seed, count=args.requests)
+        wname = f"trail sweep: {args.requests} trails, nx,ny U[64,256], nz U[128,512], 1500-4000 steps, M=1"
     else:
-        trails = configs.config5(seed=args.seed, count=args.requests)
-        wname = (f"config-5 stream: {args.requests} requests, Poisson {args.rate}/s, 64^3..256x256x1024 "
+        trails = configs.request_stream(seed=args.seed, count=args.requests)
+        wname = (f"request stream: {args.requests} requests, Poisson {args.rate}/s, 64^3..256x256x1024 "
                  "log-uniform, 1500-4000 steps, M=16")
     log(f"{wname}: media built in {time.perf_counter() - t0:.1f} s")
     updates = workloads.cell_updates(trails)
@@ -144,7 +145,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="config2")
+    ap.add_argument("--workload", default="400x400x800")
     ap.add_argument("--precision", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
@@ -176,11 +177,11 @@ def main():
         return
 
     t0 = time.perf_counter()
-    if args.workload == "config2":
-        p, wp, nu = configs.config2(seed=0)
+    if args.workload == "400x400x800":
+        p, wp, nu = configs.trail_400x400x800(seed=0)
         wname = "400x400x800_fp32_lk_cpml12"
-    elif args.workload == "config1":
-        p, wp, nu = configs.config1()
+    elif args.workload == "96x96x192":
+        p, wp, nu = configs.trail_96x96x192()
         wname = "96x96x192_fp32_lk_cpml10"
     else:
         raise SystemExit(f"unknown workload {args.workload}")

@@ -79,7 +79,7 @@ def trail_3d(p: dict, n0: float, r0: float, nu0: float, tilt_deg: float) -> tupl
     return wp, nu
 
 
-def config1(tsteps: int = 2000) -> tuple[dict, np.ndarray, np.ndarray]:
+def trail_96x96x192(tsteps: int = 2000) -> tuple[dict, np.ndarray, np.ndarray]:
     """96x96x192, dx 0.5 m, 10-cell CPML, Courant 0.99/sqrt(3), Ricker at
     50 MHz (nearest reference waveform to a Gaussian derivative), Ez, n0 1e12,
     r0 1.5 m, nu 1e5, B 5e-5 T at 60 deg from z, single-frequency NTFF."""
@@ -88,7 +88,7 @@ def config1(tsteps: int = 2000) -> tuple[dict, np.ndarray, np.ndarray]:
     return p, wp, nu
 
 
-def config2(seed: int = 0, tsteps: int = 6000, media3d: bool = True) -> tuple[dict, np.ndarray, np.ndarray]:
+def trail_400x400x800(seed: int = 0, tsteps: int = 6000, media3d: bool = True) -> tuple[dict, np.ndarray, np.ndarray]:
     """400x400x800 (128M cells), dx 0.25 m, 12-cell CPML, 6000 steps; n0
     log-uniform in [1e11,1e14], r0 uniform in [0.5,4] m from `seed`; nu
     decays along z with a 7 km scale height; 30 deg off broadside."""
@@ -106,16 +106,16 @@ def config2(seed: int = 0, tsteps: int = 6000, media3d: bool = True) -> tuple[di
     return p, wp, nu
 
 
-def config3(tsteps: int = 2000) -> tuple[dict, np.ndarray, np.ndarray]:
-    """Config 1 geometry with 64 NTFF frequencies from 30 to 80 MHz."""
-    p, wp, nu = config1(tsteps)
+def ntff_m64_96x96x192(tsteps: int = 2000) -> tuple[dict, np.ndarray, np.ndarray]:
+    """The 96x96x192 trail with 64 NTFF frequencies from 30 to 80 MHz."""
+    p, wp, nu = trail_96x96x192(tsteps)
     p.update(nntf=64, fntf=list(np.linspace(30e6, 80e6, 64)))
     return p, wp, nu
 
 
 def random_trail(rng: np.random.Generator, nx: int, ny: int, nz: int, dx: float = 0.5,
                  steps_range=(1500, 4000), nntf: int = 1) -> tuple[dict, np.ndarray, np.ndarray]:
-    """One config-4/5 trail: n0 log-uniform [1e10,1e15], r0 U[0.3,5] m,
+    """One trail of the sweep and the request stream: n0 log-uniform [1e10,1e15], r0 U[0.3,5] m,
     nu log-uniform [1e3,1e7], B angle U[0,90] deg, incidence U[0,60] deg."""
     n0 = 10 ** rng.uniform(10, 15)
     r0 = rng.uniform(0.3, 5.0)
@@ -142,8 +142,8 @@ def random_trail(rng: np.random.Generator, nx: int, ny: int, nz: int, dx: float 
     return p, wp, nu
 
 
-def config4(seed: int = 0, count: int = 48):
-    """48 trails, nx,ny ~ U{64..256}, nz ~ U{128..512}, dx 0.5 m."""
+def trail_sweep(seed: int = 0, count: int = 48):
+    """A sweep of random trails (48 by default), nx,ny ~ U{64..256}, nz ~ U{128..512}, dx 0.5 m."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(count):
@@ -153,8 +153,8 @@ def config4(seed: int = 0, count: int = 48):
     return out
 
 
-def config5(seed: int = 0, count: int = 64, nntf: int = 16):
-    """Serving requests: config-4 physics, grids log-uniform in cell count
+def request_stream(seed: int = 0, count: int = 64, nntf: int = 16):
+    """Serving requests: the sweep's physics, grids log-uniform in cell count
     from 64^3 to 256x256x1024 (nx = ny = 64*4^s, nz = 64*16^s, s ~ U[0,1]),
     16-frequency NTFF (40-60 MHz)."""
     rng = np.random.default_rng(seed)

@@ -48,7 +48,7 @@ def directions():
 
 
 def study(steps: int = 2000) -> dict:
-    p, wp, nu = configs.config3(tsteps=steps)
+    p, wp, nu = configs.ntff_m64_96x96x192(tsteps=steps)
     # the reference's node-averaged Lee-Kalluri drive grows without bound
     # over 2000 steps; the study runs the collocated form
     p.update(j_collocated=1, numfiles=1, savefields=0)
