@@ -144,8 +144,8 @@ typedef struct msc_solver msc_solver;
  * J and CPML in FP32; plasma coefficients FP64; NTFF sums FP64) or 64.
  * `wp`/`nu` as returned by msc_read_run_dir.  The tiling is the defaults
  * of msc_tiling_default with one change: one-step x chunks of 4 planes
- * below 24 M cells.  The two-step region is used in FP32 when it gives at
- * least 1.5 two-step tiles per CU.
+ * below 24 M cells.  The two-step region is used when it gives at least
+ * 1.5 two-step tiles per CU.
  */
 int msc_solver_create(const msc_params* p, const double* wp, const double* nu,
                       int precision, msc_solver** out);
@@ -169,8 +169,8 @@ int msc_solver_create_tiled(const msc_params* p, const double* wp, const double*
                             const msc_tiling* tiling, msc_solver** out);
 
 /* Autotuner: times tilings on this solve and returns the fastest.  A
- * coordinate search from the defaults over chunk2 (when the solve runs
- * two-step tiles: FP32, or MSC_PAIR=1), chunk, ty and ty_b, each
+ * coordinate search from the defaults over chunk2 (unless MSC_PAIR=0
+ * turns two-step tiles off), chunk, ty and ty_b, each
  * candidate a fresh solver stepped 2 steps and then `steps` timed steps
  * (at most tsteps-2); a candidate replaces the current one when it is
  * faster by more than 0.5%.  *best_ms = its elapsed GPU time per step.

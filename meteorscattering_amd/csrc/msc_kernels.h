@@ -100,10 +100,14 @@ struct StepArgs {
   const int* px; const int* py; const int* pz;
 };
 
-// Rows of a two-step workgroup (2 halo rows below and above the owned ones)
-// and owned z cells per tile (64 lanes less 2 low and 2 high halo lanes).
+// Rows and z lanes of a two-step workgroup (2 halo rows and lanes on each
+// side) and the z cells a tile owns.  FP64 rows are 32 lanes wide: its
+// 512-thread workgroups may use twice the registers of 1024-thread ones.
 constexpr int kPairRows = 16;
-constexpr int kPairOwnZ = kLanes - 4;
+template <typename T>
+constexpr int kPairLanes = sizeof(T) == 4 ? 64 : 32;
+template <typename T>
+constexpr int kPairOwnZ = kPairLanes<T> - 4;
 
 // Near-to-far-field surface DFT.  Every step writes the 24 surface
 // sample arrays into slot t % K of a K-step buffer; every K steps (and at

@@ -570,19 +570,19 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
 // owned cells the one-step launches around the region read as halo (the
 // two outermost layers of the region) and for probe neighbourhoods.  The
 // arithmetic of every cell and step is the one-step kernel's, term by term.
-template <typename T, int CUR>
-__global__ void __launch_bounds__(kPairRows * kLanes)
+template <typename T, int CUR, int PR, int PL>
+__global__ void __launch_bounds__(PR * PL)
 fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ tiles,
           const uint16_t* __restrict__ sids) {
-  constexpr int R = kPairRows;
+  constexpr int R = PR;  // rows; PL z lanes per row
   constexpr uint32_t W = sizeof(T);
-  __shared__ T s0[2][3][R][kLanes];   // E^t of planes i and i+1 (by parity)
-  __shared__ T s1h[3][R][kLanes];     // H^(t+1/2) of plane i
-  __shared__ T s1e[2][3][R][kLanes];  // E^(t+1) of planes i-1 and i (by parity)
-  __shared__ T s2h[3][R][kLanes];     // H^(t+3/2) of plane i-1
+  __shared__ T s0[2][3][R][PL];   // E^t of planes i and i+1 (by parity)
+  __shared__ T s1h[3][R][PL];     // H^(t+1/2) of plane i
+  __shared__ T s1e[2][3][R][PL];  // E^(t+1) of planes i-1 and i (by parity)
+  __shared__ T s2h[3][R][PL];     // H^(t+3/2) of plane i-1
 
-  const int lane = threadIdx.x & (kLanes - 1);
-  const int r = threadIdx.x / kLanes;
+  const int lane = threadIdx.x & (PL - 1);
+  const int r = threadIdx.x / PL;
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
   const uint32_t tile = tiles[blk];
   const ConstArgs<StepArgs<T>>& a = as_const_space(args + (sids ? sids[blk] : 0));
@@ -598,10 +598,10 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   // of the one-step update holds there
   const bool inside = j >= 1 && j < ny - 1 && k >= 1 && k < nz - 1;
   const bool valid = j >= 0 && j < ny && k >= 0 && k < nz;
-  const bool h1ok = inside && r < R - 1 && lane < kLanes - 1;  // H^(t+1/2): needs the +1 neighbours
+  const bool h1ok = inside && r < R - 1 && lane < PL - 1;  // H^(t+1/2): needs the +1 neighbours
   const bool e1ok = h1ok && r >= 1 && lane >= 1;                // E^(t+1): needs the -1 H neighbours
-  const bool h2ok = e1ok && r < R - 2 && lane < kLanes - 2;     // H^(t+3/2)
-  const bool own = inside && r >= 2 && r < R - 2 && j < j1 && lane >= 2 && lane < kLanes - 2 && k < k1;
+  const bool h2ok = e1ok && r < R - 2 && lane < PL - 2;     // H^(t+3/2)
+  const bool own = inside && r >= 2 && r < R - 2 && j < j1 && lane >= 2 && lane < PL - 2 && k < k1;
   // owned cells whose step-t values the one-step launches or the probes read
   bool rimjk = j < a.y2lo + 2 || j >= a.y2hi - 2 || k < a.z2lo + 2 || k >= a.z2hi - 2;
   unsigned long long pmask = 0;  // probes whose (j, k) box holds this cell
@@ -1287,16 +1287,19 @@ hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int c
   return hipErrorInvalidValue;
 }
 
-// Two steps of the tiles of the two-step region (fdtd_pair).
+// Two steps of the tiles of the two-step region (fdtd_pair): workgroups of
+// kPairRows rows of kPairLanes<T> z lanes (64 in FP32; 32 in FP64, whose
+// 512-thread workgroups leave room for its registers).
 template <typename T>
 hipError_t launch_pair(const StepArgs<T>* a, const uint16_t* sids, int cur, const uint32_t* tiles, int n,
                        hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const dim3 g(n), b(kPairRows * kLanes);
+  constexpr int PL = kPairLanes<T>;
+  const dim3 g(n), b(kPairRows * PL);
   switch (cur) {
-    case kCurDI: hipLaunchKernelGGL((fdtd_pair<T, kCurDI>), g, b, 0, s, a, tiles, sids); break;
-    case kCurLKPoly: hipLaunchKernelGGL((fdtd_pair<T, kCurLKPoly>), g, b, 0, s, a, tiles, sids); break;
-    default: hipLaunchKernelGGL((fdtd_pair<T, kCurLKClosed>), g, b, 0, s, a, tiles, sids); break;
+    case kCurDI: hipLaunchKernelGGL((fdtd_pair<T, kCurDI, kPairRows, PL>), g, b, 0, s, a, tiles, sids); break;
+    case kCurLKPoly: hipLaunchKernelGGL((fdtd_pair<T, kCurLKPoly, kPairRows, PL>), g, b, 0, s, a, tiles, sids); break;
+    default: hipLaunchKernelGGL((fdtd_pair<T, kCurLKClosed, kPairRows, PL>), g, b, 0, s, a, tiles, sids); break;
   }
   return hipGetLastError();
 }

@@ -384,7 +384,7 @@ size_t Solver<T>::layout() {
       // the region's tile count, as the even split below makes it
       auto pieces = [](int n, int piece) { return (long long)((n + piece - 1) / piece); };
       const long long nt2 = zhi > zlo ? pieces(nx - 2 * m, chunk2) * pieces(ny - 2 * m, kPairRows - 4) *
-                                            pieces(zhi - zlo, kPairOwnZ)
+                                            pieces(zhi - zlo, kPairOwnZ<T>)
                                       : 0;
       if (zhi > zlo && nt2 >= min_pair_tiles) {
         x2lo = m; x2hi = nx - m; y2lo = m; y2hi = ny - m; z2lo = zlo; z2hi = zhi;
@@ -430,7 +430,7 @@ size_t Solver<T>::layout() {
   };
   xs2_ = even(x2lo, x2hi, chunk2);
   ys2_ = even(y2lo, y2hi, kPairRows - 4);
-  zs2_ = even(z2lo, z2hi, kPairOwnZ);
+  zs2_ = even(z2lo, z2hi, kPairOwnZ<T>);
   const int nch = (int)xs_.size() - 1, tiles_y = (int)ys_.size() - 1, tiles_z = (int)zs_.size() - 1;
   const int tiles_yb = (int)ysb_.size() - 1;
   auto zone = [&](int lo, int hi, int n) {  // owned range [lo, hi) meets p <= L or p >= n-L-1
@@ -596,8 +596,6 @@ int Solver<T>::init(const double* wp, const double* nu) {
   const int nx = h.nx, ny = h.ny, nz = h.nz, L = h.L;
   const size_t N = (size_t)nx * ny * nz;
   if (tiling_from_env) {
-    // the FP64 two-step kernel spills registers: one-step unless asked for
-    if (sizeof(T) == 8) pair = false;
     if (const char* s = std::getenv("MSC_TY")) ty = std::atoi(s);
     if (const char* s = std::getenv("MSC_TY_B")) ty_b = std::atoi(s);
     // one-step x chunks of 4 planes below 24 M cells: there the tiles are
@@ -1049,10 +1047,10 @@ int check_tiling(const msc_tiling* t) {
 }
 
 // a solver with an explicit tiling, set up and synchronised
-// Two-step mode of a solve with an explicit tiling: FP32, unless MSC_PAIR says otherwise.
-bool tiled_pair(int precision) {
+// Two-step mode of a solve with an explicit tiling: on, unless MSC_PAIR=0.
+bool tiled_pair(int) {
   if (const char* e = std::getenv("MSC_PAIR")) return std::atoi(e) != 0;
-  return precision == 32;
+  return true;
 }
 
 // A single solve uses its two-step region only when the region gives at
