@@ -314,7 +314,7 @@ double msc_server_clock(const msc_server* s);
 /* Scheduling knobs.  fill_tiles: each gang iteration steps the oldest
  * resident solve, then the others by least remaining work (cells x steps
  * left), until their workgroup tiles reach fill_tiles (<= 0: every resident
- * solve, the batch setting); default 16384.  use_graphs: replay each gang's
+ * solve, the batch setting); default 32768.  use_graphs: replay each gang's
  * fixed launch sequence as a captured hipGraph (default 1). */
 int msc_server_configure(msc_server* s, int fill_tiles, int use_graphs);
 /* Tilings by size class (e.g. from msc_autotune on one solve per class): a

@@ -115,7 +115,7 @@ struct Server : ServerBase {
   static int ty_index(int ty) { return ty == 3 ? 0 : ty == 7 ? 1 : 2; }
   // tilings by size class: (max cells, tiling); empty: the defaults
   std::vector<std::pair<long long, msc_tiling>> tilings;
-  int fill_tiles = 16384;   // ~20 rounds of 3 workgroups on each of the 256 CUs
+  int fill_tiles = 32768;   // ~40 rounds of 3 workgroups on each of the 256 CUs
   bool use_graphs = true;     // tiles that keep every CU busy (4 rounds of 3 workgroups x 256 CUs)
   std::vector<Req*> draining;  // all steps launched, waiting for completion
   std::map<int64_t, std::unique_ptr<Req>> reqs;

@@ -43,7 +43,7 @@ class Server:
         _check(self.lib.msc_server_create(precision, int(budget_bytes), C.byref(h)), self.lib)
         self._h = h
         if fill_tiles is not None or not graphs:
-            _check(self.lib.msc_server_configure(h, 16384 if fill_tiles is None else int(fill_tiles), int(graphs)),
+            _check(self.lib.msc_server_configure(h, 32768 if fill_tiles is None else int(fill_tiles), int(graphs)),
                    self.lib)
         self.precision = precision
         self.params: dict[int, dict] = {}
