@@ -446,6 +446,37 @@ def test_timing_record_counts_launch_kinds(msc, monkeypatch):
             assert tm["pair_launches"] == 0 and tm["pair_cells"] == 0
 
 
+def test_default_two_step_mode_follows_the_region_size(msc, monkeypatch):
+    """With MSC_PAIR unset, a single solve runs two-step tiles when its
+    region gives at least 1.5 fdtd_pair tiles per CU: 200x200x300 with L = 10
+    has a region [16,184)^2 x [31,279) of 6 x 14 x 5 = 420 tiles, the
+    60x50x205 grid 1 x 2 x 3 = 6."""
+    import ctypes
+
+    Solver, _ = msc
+    monkeypatch.delenv("MSC_PAIR", raising=False)
+    # the CU count from the HIP runtime libmsc.so uses (attribute 63 is
+    # hipDeviceAttributeMultiprocessorCount in ROCm 7.2's hip_runtime_api.h)
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_int(0)
+    assert hip.hipDeviceGetAttribute(ctypes.byref(n), 63, 0) == 0
+    cus = n.value
+    assert cus > 0
+    for (nx, ny, nz), tiles in (((200, 200, 300), 6 * 14 * 5), ((60, 50, 205), 1 * 2 * 3)):
+        p, _, _ = cases.case("lk_ricker_ez")
+        p.update(nx=nx, ny=ny, nz=nz, tsteps=12, numfiles=1, nprobes=1, probex=[nx // 2], probey=[ny // 2],
+                 probez=[nz // 2])
+        wp, nu = cases.trail_maps(p, n0=2e13, r0=1.5, nu0=1e6, centre=(0.5, -0.25))
+        with Solver(p, wp, nu, precision=32) as g:
+            g.step(2)
+            g.set_timing(True)
+            g.step(6)
+            g.sync()
+            tm = g.timing()
+        expect = tiles >= 3 * cus // 2
+        assert (tm["pair_launches"] > 0) == expect, (nx, ny, nz, tiles, cus, tm["pair_launches"])
+
+
 @pytest.mark.parametrize("nntf", [16, 100])
 @pytest.mark.parametrize("precision,tol", [(64, 1e-10), (32, 2e-5)])
 def test_ntff_matrix_core_contraction_matches_oracle(msc, oracle_lib, nntf, precision, tol):
