@@ -270,7 +270,7 @@ struct Server : ServerBase {
     if (!r->s) {
       r->s.reset(new Solver<T>());
       r->s->tiling_from_env = false;  // the size class's tiling (the defaults without a table)
-      r->s->pair = sizeof(T) == 4;    // two-step gangs in FP32; FP64 gangs run one-step
+      r->s->pair = true;              // two-step gangs (FP64 tiles are 16 x 32)
       r->s->zlanes = kServerZLanes;
       msc_tiling t;
       msc_tiling_default(&t);

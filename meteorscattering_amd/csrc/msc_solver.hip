@@ -1019,9 +1019,9 @@ size_t solve_footprint(const msc_params* p, int precision) {
   if (precision == 32) s.reset(new Solver<float>());
   else if (precision == 64) s.reset(new Solver<double>());
   else return 0;
-  // the server's settings: its tiling; two-step mode in FP32
+  // the server's settings: its tiling, two-step mode, 64-lane one-step tiles
   s->tiling_from_env = false;
-  s->pair = precision == 32;
+  s->pair = true;
   s->zlanes = kServerZLanes;
   if (make_plan(p, s->plan) != MSC_OK) return 0;
   return s->layout();

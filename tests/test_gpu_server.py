@@ -169,6 +169,24 @@ def test_two_step_gangs_equal_alone(fill, graphs):
             assert st["iterations"] == expect
 
 
+def test_fp64_two_step_gangs_equal_alone():
+    """FP64 gangs with two-step tiles (16 rows x 32 lanes, 12 x 28 owned) on
+    the 60x50x205 grids equal FP64 single solves (which run these small
+    regions one-step) bit for bit."""
+    mix = {k: f() for k, f in TWO_STEP_MIX.items()}
+    alone = {k: _alone(*v, precision=64) for k, v in mix.items()}
+    with Server(precision=64, budget_bytes=4 << 30, fill_tiles=1) as srv:
+        ids = {k: srv.submit(*v) for k, v in mix.items()}
+        srv.run_until_done()
+        for k in mix:
+            np.testing.assert_array_equal(srv.ntff(ids[k]), alone[k][0], err_msg=k)
+            np.testing.assert_array_equal(srv.probes(ids[k]), alone[k][1], err_msg=k)
+        st = srv.stats()
+        # one solve per iteration: the two-step solves take half the iterations
+        expect = sum((v[0]["tsteps"] + 1) // 2 if k != "small50" else v[0]["tsteps"] for k, v in mix.items())
+        assert st["iterations"] == expect
+
+
 @pytest.mark.parametrize("streams,graphs", [(2, True), (1, True), (1, False)])
 def test_gang_lists_on_several_streams_equal_alone(monkeypatch, streams, graphs):
     """The gang's one-step launches of a half spread over the run stream and
