@@ -116,7 +116,7 @@ struct Server : ServerBase {
   // tilings by size class: (max cells, tiling); empty: the defaults
   std::vector<std::pair<long long, msc_tiling>> tilings;
   int fill_tiles = 32768;   // ~40 rounds of 3 workgroups on each of the 256 CUs
-  bool use_graphs = true;     // tiles that keep every CU busy (4 rounds of 3 workgroups x 256 CUs)
+  bool use_graphs = true;     // replay each gang half as a captured graph
   std::vector<Req*> draining;  // all steps launched, waiting for completion
   std::map<int64_t, std::unique_ptr<Req>> reqs;
   int64_t next_id = 1;
