@@ -193,9 +193,20 @@ int msc_solver_time(const msc_solver* s);
 int msc_solver_get_field(msc_solver* s, int comp, double* out);
 
 /* Number of doubles in the NTF payload: Eydft[2M], Ezdft[2M], then the 24
- * face arrays of output_NTF.dat in file order (meteor-fork.cpp:1677-1714). */
+ * face arrays of output_NTF.dat in file order (meteor-fork.cpp:1677-1714).
+ * msc_solver_get_ntff returns the running DFT through the last step
+ * enqueued, as the reference's arrays hold it after that step (it flushes
+ * the samples buffered since the last K-step contraction first). */
 size_t msc_solver_ntff_count(const msc_solver* s);
 int msc_solver_get_ntff(msc_solver* s, double* out, size_t n);
+
+/* Contract the NTFF surface samples buffered since the last K-step block
+ * into the FP64 sums now, on the solver's stream (asynchronous).  The
+ * reference adds every step's sample to its sums at once
+ * (meteor-fork.cpp:971-1023, 1417-1470); this build buffers K steps and
+ * contracts them together, so a caller that times or reads a partial block
+ * flushes it.  Results are the same sums in another order (FP64). */
+int msc_solver_flush_ntff(msc_solver* s);
 
 /* Probe histories (Probe.dat body after the index header): 6 arrays
  * [tsteps][nprobes] (Ex,Ey,Ez,Hx,Hy,Hz), meteor-fork.cpp:1719-1729. */

@@ -113,6 +113,8 @@ constexpr int kPairOwnZ = kPairLanes<T> - 4;
 // sample arrays into slot t % K of a K-step buffer; every K steps (and at
 // the last step) the buffer is contracted with the sin/cos table into the
 // FP64 accumulators: acc[q][2m+c][pt] += sum_k s_k[q][pt] * sc[t0+k][m][c].
+// A flush contracts the slots of the block so far at any step and zeroes
+// them, so the block's own contraction adds only the later slots.
 template <typename T>
 struct NtffArgs {
   int nx, ny, nz, M, b;
@@ -154,7 +156,8 @@ hipError_t launch_pair(const StepArgs<T>* a, const uint16_t* sids, int cur, cons
 template <typename T>
 hipError_t launch_ntff(const NtffArgs<T>* a, const uint2* blocks, int nblocks, hipStream_t s);
 template <typename T>
-hipError_t launch_ntff_contract(const NtffArgs<T>* a, long long U, int M, long long nwaves, hipStream_t s);
+hipError_t launch_ntff_contract(const NtffArgs<T>* a, long long U, int M, long long nwaves, hipStream_t s,
+                                int flush = 0);
 template <typename T>
 hipError_t launch_frames(const AuxArgs<T>* a, hipStream_t s);
 template <typename T>

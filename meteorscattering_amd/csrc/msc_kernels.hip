@@ -951,10 +951,10 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 // D[4 v + l / 16][l % 16] in element v (checked on gfx950).  FP64 products
 // and sums, in a different order than ntff_contract.
 template <typename T>
-__global__ void __launch_bounds__(256) ntff_contract_mfma(const NtffArgs<T>* __restrict__ args) {
+__global__ void __launch_bounds__(256) ntff_contract_mfma(const NtffArgs<T>* __restrict__ args, int flush) {
   const ConstArgs<NtffArgs<T>>& a = as_const_space(args);
   const int t = *a.tdev - 1;
-  if ((t + 1) % a.K != 0 && t + 1 != a.T_steps) return;
+  if (t < 0 || (!flush && (t + 1) % a.K != 0 && t + 1 != a.T_steps)) return;
   const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kLanes;
   if (w >= a.wofs[24]) return;  // uniform per wavefront
   int q = 0;
@@ -997,6 +997,12 @@ __global__ void __launch_bounds__(256) ntff_contract_mfma(const NtffArgs<T>* __r
         }
       }
     }
+  }
+  // a flush consumes the block's samples so far: zeroed, they add nothing
+  // when the block's own contraction sums all of its slots
+  if (flush && pv) {
+    T* __restrict__ sw = a.sbuf + a.uoff[q] + p;
+    for (int k = kr; k < kn; k += 4) sw[(size_t)k * U] = (T)0;
   }
 }
 
@@ -1081,15 +1087,16 @@ __device__ __forceinline__ double ldc(const double* p, size_t i) {
 
 // Contraction of the buffered samples at the end of a K-step block (the
 // DFT of meteor-fork.cpp:977-1022, 1417-1470 summed over the block, phase
-// t*dt for both E and H, B2): one thread per sample point, frequencies in
-// groups of 8, FP64 products and sums.  Runs after the step counter moved
-// past step t (so it can follow a captured step sequence); returns at once
-// unless step t closes a block.
+// t*dt for both E and H, as the reference's integer (t+1/2) gives): one
+// thread per sample point, frequencies in groups of 8, FP64 products and
+// sums.  Runs after the step counter moved past step t (so it can follow a
+// captured step sequence); returns at once unless step t closes a block or
+// `flush` asks for the block so far (its slots are then zeroed).
 template <typename T>
-__global__ void __launch_bounds__(256) ntff_contract(const NtffArgs<T>* __restrict__ args) {
+__global__ void __launch_bounds__(256) ntff_contract(const NtffArgs<T>* __restrict__ args, int flush) {
   const ConstArgs<NtffArgs<T>>& a = as_const_space(args);
   const int t = *a.tdev - 1;
-  if ((t + 1) % a.K != 0 && t + 1 != a.T_steps) return;
+  if (t < 0 || (!flush && (t + 1) % a.K != 0 && t + 1 != a.T_steps)) return;
   const long long U = a.uoff[24];
   const long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= U) return;
@@ -1121,6 +1128,10 @@ __global__ void __launch_bounds__(256) ntff_contract(const NtffArgs<T>* __restri
         acc[(size_t)(2 * (m0 + j) + 1) * np] += cc[j];
       }
     }
+  }
+  if (flush) {
+    T* __restrict__ sw = a.sbuf;
+    for (int k = 0; k < kn; ++k) sw[(size_t)k * U + u] = (T)0;
   }
 }
 
@@ -1312,12 +1323,13 @@ hipError_t launch_ntff(const NtffArgs<T>* a, const uint2* blocks, int nblocks, h
 }
 
 template <typename T>
-hipError_t launch_ntff_contract(const NtffArgs<T>* a, long long U, int M, long long nwaves, hipStream_t s) {
+hipError_t launch_ntff_contract(const NtffArgs<T>* a, long long U, int M, long long nwaves, hipStream_t s,
+                                int flush) {
   if (U <= 0) return hipSuccess;
   if (2 * M >= 16)  // matrix cores: 4 wavefronts of 16 points per workgroup
-    hipLaunchKernelGGL((ntff_contract_mfma<T>), dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((ntff_contract_mfma<T>), dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, s, a, flush);
   else
-    hipLaunchKernelGGL((ntff_contract<T>), dim3((unsigned)((U + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((ntff_contract<T>), dim3((unsigned)((U + 255) / 256)), dim3(256), 0, s, a, flush);
   return hipGetLastError();
 }
 
@@ -1349,7 +1361,7 @@ hipError_t launch_catchup(const StepArgs<T>* a, int cur, T* ox, T* oy, T* oz, hi
                                      hipStream_t);                                                                 \
   template hipError_t launch_pair<T>(const StepArgs<T>*, const uint16_t*, int, const uint32_t*, int, hipStream_t); \
   template hipError_t launch_ntff<T>(const NtffArgs<T>*, const uint2*, int, hipStream_t);                          \
-  template hipError_t launch_ntff_contract<T>(const NtffArgs<T>*, long long, int, long long, hipStream_t);         \
+  template hipError_t launch_ntff_contract<T>(const NtffArgs<T>*, long long, int, long long, hipStream_t, int);    \
   template hipError_t launch_frames<T>(const AuxArgs<T>*, hipStream_t);                                            \
   template hipError_t launch_probes<T>(const AuxArgs<T>*, int, hipStream_t);                                       \
   template hipError_t launch_catchup<T>(const StepArgs<T>*, int, T*, T*, T*, hipStream_t);

@@ -127,6 +127,10 @@ struct SolverBase {
   virtual int step(int n) = 0;
   virtual int get_field(int comp, double* out) = 0;
   virtual int get_ntff(double* out) = 0;
+  // Contract the NTFF samples of the current partial K-step block now
+  // (asynchronous, on `stream`; nothing to do when the last step closed a
+  // block or the run is complete)
+  virtual int flush_ntff() = 0;
   virtual int get_probes(double* out) = 0;
   virtual int get_frame(int f, double* e, double* h, double* emax) = 0;
   int sync();
@@ -161,6 +165,7 @@ struct Solver : SolverBase {
   int step(int n) override;
   int get_field(int comp, double* out) override;
   int get_ntff(double* out) override;
+  int flush_ntff() override;
   int get_probes(double* out) override;
   int get_frame(int f, double* e, double* h, double* emax) override;
 

@@ -982,6 +982,13 @@ int Solver<T>::get_ntff(double* out) {
 }
 
 template <typename T>
+int Solver<T>::flush_ntff() {
+  if (t_host == 0 || t_host % na.K == 0 || t_host >= plan.T) return MSC_OK;
+  HIPCHK(launch_ntff_contract<T>(na_dev, na.uoff[24], na.M, na.wofs[24], stream, 1));
+  return MSC_OK;
+}
+
+template <typename T>
 int Solver<T>::get_probes(double* out) {
   const size_t n = (size_t)6 * plan.T * plan.p.nprobes;
   if (!n) return MSC_OK;
@@ -1212,7 +1219,13 @@ size_t msc_solver_ntff_count(const msc_solver* s) {
 int msc_solver_get_ntff(msc_solver* s, double* out, size_t n) {
   SOLVER_OK(s);
   if (!out || n < msc_solver_ntff_count(s)) return fail(MSC_EINVAL, "output buffer too small");
-  return s->impl->get_ntff(out);
+  int rc = s->impl->flush_ntff();
+  return rc ? rc : s->impl->get_ntff(out);
+}
+
+int msc_solver_flush_ntff(msc_solver* s) {
+  SOLVER_OK(s);
+  return s->impl->flush_ntff();
 }
 
 int msc_solver_get_probes(msc_solver* s, double* out, size_t n) {

@@ -10,8 +10,8 @@ ntfb to N-ntfb-1.  The sums are complexified as i*sin_sum + cos_sum
 (:199-230), i.e. phasors of an e^{-i w t} convention, so the radiation
 integral carries exp(-i k r.r').  Backscatter (meteorNTF's only direction)
 is r = -x, theta_hat = -z, phi_hat = -y (:307-317), sigma by Taflove eq.
-8.37 (:332).  Unlike meteorNTF.m (which slices every face with Nx, quirk
-B9) the extents are per axis, identical on cubic grids.
+8.37 (:332).  Unlike meteorNTF.m (which slices every face with Nx, so it
+assumes a cubic grid) the extents are per axis, identical on cubic grids.
 
 Yee positions (grid-index units): Ex (i+1/2, j, k), Ey (i, j+1/2, k),
 Ez (i, j, k+1/2), Hx (i, j+1/2, k+1/2), Hy (i+1/2, j, k+1/2),

@@ -61,6 +61,11 @@ class Solver:
         _check(self.lib.msc_solver_get_ntff(self._h, _abi.dptr(out), n), self.lib)
         return out
 
+    def flush_ntff(self) -> None:
+        """Contract the NTFF samples buffered since the last K-step block
+        (asynchronous; ntff() does it itself)."""
+        _check(self.lib.msc_solver_flush_ntff(self._h), self.lib)
+
     def probes(self) -> np.ndarray:
         n = 6 * self.p["tsteps"] * self.p["nprobes"]
         out = np.empty(max(n, 1), dtype=np.float64)

@@ -882,6 +882,12 @@ void orc_get_field(const orc* o, int comp, double* out) {
   memcpy(out, src[comp], sizeof(double) * (size_t)o->nx * o->ny * o->nz);
 }
 
+void orc_get_media(const orc* o, double* wpe, double* nue) {
+  size_t n = sizeof(double) * (size_t)o->nx * o->ny * o->nz;
+  if (wpe) memcpy(wpe, o->wpe, n);
+  if (nue) memcpy(nue, o->nue, n);
+}
+
 size_t orc_ntff_count(const orc* o) {
   size_t M2 = 2 * (size_t)o->M;
   return 2 * M2 + 8 * M2 * ((size_t)o->ny * o->nz + (size_t)o->nx * o->nz + (size_t)o->nx * o->ny);

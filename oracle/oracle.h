@@ -41,6 +41,10 @@ void orc_get_probes(const orc* o, double* out);
 size_t orc_frame_count(const orc* o);
 void orc_get_frames(const orc* o, double* eframe, double* hframe);
 double orc_max_abs_ex(const orc* o);
+/* The extruded media the time loop uses (meteor-fork.cpp:336-366 and the
+ * media3d / nu scale-height extensions): wpe = plasma frequency (rad/s),
+ * nue = collision frequency (1/s), each nx*ny*nz doubles. */
+void orc_get_media(const orc* o, double* wpe, double* nue);
 /* Lee-Kalluri 3x3 matrices for one cell (meteor-fork.cpp:1546-1573). */
 void orc_lk_matrices(double nu, double dt, const double wce[3], double A[9], double K[9]);
 /* CPML tables: b[2L], a[2L] for the E (half-offset) and H grids. */

@@ -47,6 +47,7 @@ def lib() -> C.CDLL:
         L.orc_get_frames.argtypes = [vp, dp, dp]
         L.orc_max_abs_ex.restype = C.c_double
         L.orc_max_abs_ex.argtypes = [vp]
+        L.orc_get_media.argtypes = [vp, dp, dp]
         L.orc_lk_matrices.argtypes = [C.c_double, C.c_double, dp, dp, dp]
         L.orc_cpml_tables.argtypes = [vp, dp, dp, dp, dp]
         L.orc_source_tables.argtypes = [vp, dp, dp]
@@ -104,6 +105,13 @@ class Oracle:
 
     def max_abs_ex(self) -> float:
         return lib().orc_max_abs_ex(self._h)
+
+    def media(self) -> tuple[np.ndarray, np.ndarray]:
+        """(wpe, nue): the extruded plasma and collision frequencies."""
+        wpe = np.empty(self.shape, dtype=np.float64)
+        nue = np.empty(self.shape, dtype=np.float64)
+        lib().orc_get_media(self._h, _dp(wpe), _dp(nue))
+        return wpe, nue
 
     def cpml_tables(self):
         L = self.p.get("pmllen") or 10

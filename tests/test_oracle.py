@@ -3,10 +3,16 @@
 1. Golden fixtures written by the reference solver compiled from its own
    source (tests/golden/make_golden.py): NTF face arrays, probes, slices,
    incident lines and the log's max|Ex| must agree bit for bit.
+   The 12-cell CPML extension against the reference built with that
+   thickness (oracle/Makefile: meteor-fork-pml12), bit for bit.
 2. Closed forms: Lee-Kalluri A and K against a matrix exponential (Van Loan
-   block form), CPML tables, source waveforms, the source DFT of a tone.
+   block form), CPML tables at L = 10, 12 and 16, source waveforms, the
+   source DFT of a tone, the extruded media with the nu(z) scale height, and
+   the whole current step cell by cell (collocated and node-averaged drive,
+   B = 0 and B != 0, DI).
 3. Invariants: vacuum TF/SF leakage at round-off; DI = L&K with B -> 0 and
-   exponential collision decay.
+   exponential collision decay; 3-D media equal to the extruded 2-D maps
+   give the 2-D run; an infinite scale height gives the uniform run.
 """
 from __future__ import annotations
 
@@ -137,12 +143,20 @@ def test_lee_kalluri_matrices_closed_form(oracle_lib, nu, B):
     np.testing.assert_allclose(A @ w, math.exp(-nu * dt) * w, atol=1e-9 * (np.linalg.norm(w) + 1))
 
 
-def test_cpml_tables_closed_form(oracle_lib):
+@pytest.mark.parametrize("L,gridfactor", [(10, 12.0), (12, 24.0), (16, 12.0)])
+def test_cpml_tables_closed_form(oracle_lib, L, gridfactor):
+    """sigma_E[n] = sigma_max ((n + 1/2)/L)^4, sigma_H[n] = sigma_max ((n + 1)/L)^4
+    with sigma_max = 1.5 (m + 1)/(150 pi dx), b = exp(-sigma dt / eps0),
+    a = b - 1 (kappa = 1, alpha = 0), the low side mirrored
+    (meteor-fork.cpp:444-561), for the reference's L = 10 and the 12- and
+    16-cell extension at dx = 0.5 and 0.25 m."""
     p, wp, nu = cases.case("lk_ricker_ez")
+    p.update(pmllen=L, gridfactor=gridfactor, nx=2 * L + 24, ny=2 * L + 24, nz=2 * L + 24)
+    wp, nu = cases.trail_maps(p)
     o = oracle_lib.Oracle(p, wp, nu)
     be, ae, bh, ah = o.cpml_tables()
-    L = 10
-    dx = 0.5
+    assert be.size == 2 * L
+    dx = 3e8 / p["f0"] / gridfactor
     smax = 1.5 * 5 / (150 * math.pi * dx)
     n = np.arange(L)
     se = smax * ((n + 0.5) / L) ** 4
@@ -153,6 +167,179 @@ def test_cpml_tables_closed_form(oracle_lib):
     np.testing.assert_allclose(ae, be - 1, rtol=1e-15, atol=0)
     np.testing.assert_allclose(ah, bh - 1, rtol=1e-15, atol=0)
     assert np.all((be > 0) & (be <= 1))
+
+
+PML12_BIN = os.path.join(ROOT, "oracle", "_ref", "meteor-fork-pml12")
+
+
+@pytest.mark.skipif(not os.path.exists(PML12_BIN), reason="reference not built here (oracle/_ref)")
+def test_oracle_matches_live_reference_with_12_cell_cpml(oracle_lib):
+    """The 12-cell CPML extension (pmllen = 12) against the reference built
+    with its CPML thickness (meteor-fork.cpp:374) set to 12: CPML tables,
+    slab offsets, TF/SF faces (L + 10) and NTFF planes (L + 5) all move with
+    L.  NTF faces and probes must agree bit for bit."""
+    p, _, _ = cases.case("lk_ricker_ez")
+    n = 56
+    p.update(nx=n, ny=n, nz=n, tsteps=100, pmllen=12, sourcepol=2, nntf=2, fntf=[47e6, 53e6], numfiles=2,
+             nprobes=2, probex=[28, 20], probey=[28, 33], probez=[27, 30])
+    wp, nu = cases.trail_maps(p, n0=3e13, r0=1.5, nu0=4e5, centre=(0.5, -0.5))
+    with tempfile.TemporaryDirectory() as d:
+        runio.write_run_dir(d, p, wp, nu)
+        subprocess.run(["bash", "-c", f"ulimit -s unlimited && cd '{d}' && OMP_NUM_THREADS=2 '{PML12_BIN}'"],
+                       check=True, capture_output=True, timeout=300)
+        ntf = runio.read_ntf(os.path.join(d, "output_NTF.dat"), n, n, n)
+        probe = runio.read_probe(os.path.join(d, "Probe.dat"), p["tsteps"])
+    assert ntf["ntfb"] == 12 + 5
+    r = oracle_lib.run_like_reference(p, wp, nu)
+    faces = ntf_split(r["ntff"], p)
+    for fn in runio.ntf_face_names():
+        assert np.abs(ntf[fn]).max() > 0, fn
+        assert np.array_equal(faces[fn], ntf[fn]), fn
+    assert np.array_equal(r["probes"].reshape(6, p["tsteps"], 2), probe["data"])
+
+
+def _taper(p, z):
+    """meteor-fork.cpp:356-357: exp(-(z - metloc_z)^2 / metsize_0^2)."""
+    return np.exp(-((z - p["metloc"][2]) ** 2) / (p["metsize"][0] ** 2))
+
+
+def _zaxis(p):
+    dz = 3e8 / p["f0"] / p["gridfactor"]
+    return -((p["nz"] - 1) // 2) * dz + dz * np.arange(p["nz"])
+
+
+def test_media3d_equal_to_the_extruded_2d_maps_gives_the_same_run(oracle_lib):
+    """media3d = 1 takes wp.dat/nu.dat as [nx][ny][nz] maps with no taper.
+    On a non-cubic grid, 3-D maps built from the 2-D maps by the reference's
+    z-taper give the 2-D run's media (to an ulp of exp) and fields (to
+    round-off); a map read in another index order would not."""
+    p, _, _ = cases.case("lk_ricker_ez")
+    p.update(nx=46, ny=50, nz=56, tsteps=80, metsize=[3.0, 3.0], metloc=[0.0, 0.0, 1.5],
+             probex=[23, 20], probey=[25, 27], probez=[28, 26])
+    wp2, nu2 = cases.trail_maps(p, n0=3e13, r0=1.5, nu0=1e6, centre=(0.75, -0.5))
+    nu2 = nu2 * np.linspace(0.5, 2.0, nu2.size).reshape(nu2.shape)
+    g = _taper(p, _zaxis(p))
+    wp3 = wp2[:, :, None] * g[None, None, :]
+    nu3 = nu2[:, :, None] * g[None, None, :]
+    o2 = oracle_lib.Oracle(p, wp2, nu2)
+    q = dict(p, media3d=1)
+    o3 = oracle_lib.Oracle(q, wp3, nu3)
+    (we2, ne2), (we3, ne3) = o2.media(), o3.media()
+    assert np.array_equal(we3, wp3) and np.array_equal(ne3, nu3)
+    np.testing.assert_allclose(we2, wp3, rtol=5e-16, atol=0)
+    np.testing.assert_allclose(ne2, nu3, rtol=5e-16, atol=0)
+    o2.step(80)
+    o3.step(80)
+    for c in range(9):
+        a, b = o2.field(c), o3.field(c)
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max(), c
+    # the maps are not symmetric under a swap of x and y: a transposed read
+    # would change the media
+    assert not np.allclose(wp3[:46, :46], np.swapaxes(wp3[:46, :46], 0, 1))
+
+
+def test_nu_scale_height_profile_closed_form(oracle_lib):
+    """nu_scale_height = H multiplies the extruded collision frequency by
+    exp(-(z - metloc_z)/H): larger below the trail centre, smaller above.
+    H -> infinity gives the uniform run bit for bit."""
+    p, _, _ = cases.case("lk_ricker_ez")
+    p.update(nx=46, ny=48, nz=52, tsteps=40, metsize=[6.0, 6.0], metloc=[0.0, 0.0, -2.0],
+             probex=[23, 20], probey=[24, 26], probez=[26, 25])
+    wp2, nu2 = cases.trail_maps(p, n0=3e13, r0=1.5, nu0=1e6)
+    z = _zaxis(p)
+    H = 9.0
+    o = oracle_lib.Oracle(dict(p, nu_scale_height=H), wp2, nu2)
+    _, ne = o.media()
+    expect = nu2[:, :, None] * _taper(p, z)[None, None, :] * np.exp(-(z - p["metloc"][2]) / H)[None, None, :]
+    np.testing.assert_allclose(ne, expect, rtol=1e-15, atol=0)
+    # below the centre (z < metloc_z) collisions are more frequent
+    ratio = ne[23, 24] / (nu2[23, 24] * _taper(p, z))
+    assert ratio[0] > 3.0 and ratio[-1] < 0.25 and np.all(np.diff(ratio) < 0)
+    o.close()
+    runs = []
+    for h in (0.0, 1e300):
+        o = oracle_lib.Oracle(dict(p, nu_scale_height=h), wp2, nu2)
+        o.step(40)
+        runs.append([o.field(c) for c in range(9)] + [o.ntff()])
+        o.close()
+    assert all(np.array_equal(a, b) for a, b in zip(*runs))
+
+
+def _current_case(j_collocated, doB0, B, H=8.0):
+    p, _, _ = cases.case("lk_ricker_ez")
+    p.update(nx=46, ny=46, nz=46, tsteps=80, j_collocated=j_collocated, doB0=doB0, B0=B, nu_scale_height=H,
+             metsize=[5.0, 5.0], probex=[23, 20], probey=[23, 25], probez=[23, 22])
+    wp, nu = cases.trail_maps(p, n0=4e13, r0=1.6, nu0=3e6, centre=(-1.5, 0.5))
+    return p, wp, nu
+
+
+def _drive(E, averaged):
+    """E at J's edge (collocated) or the node average of that edge and the
+    one below it along its own axis (meteor-fork.cpp:1577-1579)."""
+    if not averaged:
+        return E
+    ex, ey, ez = E
+    return (0.5 * (ex + np.roll(ex, 1, axis=0)), 0.5 * (ey + np.roll(ey, 1, axis=1)),
+            0.5 * (ez + np.roll(ez, 1, axis=2)))
+
+
+def _expected_current(J, E, wpe, nue, dt, w, doB0):
+    """One step of the cold-plasma current per cell in closed form:
+    DI (Young's bilinear transform) J' = (2 - nu dt)/(2 + nu dt) J + 2 eps0 wp^2 dt/(2 + nu dt) E;
+    magnetized J' = A J + eps0 wp^2 K E with A = expm(M dt), K = int_0^dt expm(M s) ds,
+    M = -nu I + [w x] (Van Loan block form)."""
+    out = [np.zeros_like(J[0]) for _ in range(3)]
+    sl = (slice(1, -1),) * 3
+    nz = J[0].shape[2]
+    for k in range(1, nz - 1):
+        nus = np.unique(nue[1:-1, 1:-1, k])
+        assert nus.size == 1  # the media vary along z only here
+        nu = float(nus[0])
+        s2 = (slice(1, -1), slice(1, -1), k)
+        wp2 = cases.E0 * wpe[s2] ** 2
+        if not doB0:
+            a = (2 - nu * dt) / (2 + nu * dt)
+            for c in range(3):
+                out[c][s2] = a * J[c][s2] + 2 * (wp2 * dt / (2 + nu * dt)) * E[c][s2]
+            continue
+        A, K = _lk_expm(nu, dt, w)
+        for c in range(3):
+            out[c][s2] = sum(A[c, d] * J[d][s2] for d in range(3)) + wp2 * sum(K[c, d] * E[d][s2] for d in range(3))
+    del sl
+    return out
+
+
+@pytest.mark.parametrize("j_collocated,doB0,B", [
+    (1, 1, (4.33e-5, 0.0, 2.5e-5)), (0, 1, (4.33e-5, 0.0, 2.5e-5)), (1, 1, (0.0, 0.0, 0.0)),
+    (0, 1, (0.0, 0.0, 0.0)), (0, 0, (0.0, 0.0, 0.0))])
+def test_current_update_closed_form_per_cell(oracle_lib, j_collocated, doB0, B):
+    """The oracle's J step against the closed form, cell by cell, after the
+    incident pulse has reached the trail: J^(n+1) from J^n and the step's
+    final E^(n+1), with nu(z) from the 8 m scale height and the taper.  The
+    drive is E at J's own edge (j_collocated = 1; B = 0 is the per-cell
+    exponential integration) or the node average (the reference,
+    meteor-fork.cpp:1577-1579).  The other drive misses by far more than the
+    tolerance, so a wrong edge fails."""
+    p, wp, nu = _current_case(j_collocated, doB0, list(B))
+    o = oracle_lib.Oracle(p, wp, nu)
+    o.step(45)
+    J0 = [o.field(c) for c in (6, 7, 8)]
+    o.step(1)
+    E1 = [o.field(c) for c in (0, 1, 2)]
+    J1 = [o.field(c) for c in (6, 7, 8)]
+    wpe, nue = o.media()
+    o.close()
+    w = -cases.QE * np.asarray(B) / cases.ME
+    scale = max(np.abs(j).max() for j in J1)
+    assert scale > 0
+    averaged = doB0 and not j_collocated
+    exp_ok = _expected_current(J0, _drive(E1, averaged), wpe, nue, p["dt"], w, doB0)
+    err = max(np.abs(a - b).max() for a, b in zip(J1, exp_ok)) / scale
+    assert err < 1e-10, err
+    if doB0:
+        exp_bad = _expected_current(J0, _drive(E1, not averaged), wpe, nue, p["dt"], w, doB0)
+        bad = max(np.abs(a - b).max() for a, b in zip(J1, exp_bad)) / scale
+        assert bad > 1e4 * max(err, 1e-15), bad
 
 
 def test_source_waveforms_closed_form(oracle_lib):
