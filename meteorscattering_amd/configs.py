@@ -114,9 +114,11 @@ def ntff_m64_96x96x192(tsteps: int = 2000) -> tuple[dict, np.ndarray, np.ndarray
 
 
 def random_trail(rng: np.random.Generator, nx: int, ny: int, nz: int, dx: float = 0.5,
-                 steps_range=(1500, 4000), nntf: int = 1) -> tuple[dict, np.ndarray, np.ndarray]:
+                 steps_range=(1500, 4000), nntf: int = 1, maps: bool = True) -> tuple[dict, np.ndarray, np.ndarray]:
     """One trail of the sweep and the request stream: n0 log-uniform [1e10,1e15], r0 U[0.3,5] m,
-    nu log-uniform [1e3,1e7], B angle U[0,90] deg, incidence U[0,60] deg."""
+    nu log-uniform [1e3,1e7], B angle U[0,90] deg, incidence U[0,60] deg.
+    With maps=False the media are not built (wp, nu are None; trail_media(p)
+    builds them later): the draws, and so every later trail, are the same."""
     n0 = 10 ** rng.uniform(10, 15)
     r0 = rng.uniform(0.3, 5.0)
     nu0 = 10 ** rng.uniform(3, 7)
@@ -138,22 +140,30 @@ def random_trail(rng: np.random.Generator, nx: int, ny: int, nz: int, dx: float 
     stab = 1 / math.sqrt(1 + (wpm * math.pi / p["gridfactor"] / 2) ** 2)
     p["dt"] *= stab
     p["_trail"] = dict(n0=n0, r0=r0, nu0=nu0, tilt_deg=inc, b_angle=bang)
-    wp, nu = trail_3d(p, n0, r0, nu0, inc)
+    if not maps:
+        return p, None, None
+    wp, nu = trail_media(p)
     return p, wp, nu
 
 
-def trail_sweep(seed: int = 0, count: int = 48):
+def trail_media(p: dict) -> tuple[np.ndarray, np.ndarray]:
+    """The 3-D media of a random trail from its drawn parameters."""
+    t = p["_trail"]
+    return trail_3d(p, t["n0"], t["r0"], t["nu0"], t["tilt_deg"])
+
+
+def trail_sweep(seed: int = 0, count: int = 48, maps: bool = True):
     """A sweep of random trails (48 by default), nx,ny ~ U{64..256}, nz ~ U{128..512}, dx 0.5 m."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(count):
         nx, ny = (int(v) for v in rng.integers(64, 257, size=2))
         nz = int(rng.integers(128, 513))
-        out.append(random_trail(rng, nx, ny, nz))
+        out.append(random_trail(rng, nx, ny, nz, maps=maps))
     return out
 
 
-def request_stream(seed: int = 0, count: int = 64, nntf: int = 16):
+def request_stream(seed: int = 0, count: int = 64, nntf: int = 16, maps: bool = True):
     """Serving requests: the sweep's physics, grids log-uniform in cell count
     from 64^3 to 256x256x1024 (nx = ny = 64*4^s, nz = 64*16^s, s ~ U[0,1]),
     16-frequency NTFF (40-60 MHz)."""
@@ -163,5 +173,5 @@ def request_stream(seed: int = 0, count: int = 64, nntf: int = 16):
         s = rng.uniform(0.0, 1.0)
         nxy = int(round(64 * 4 ** s))
         nz = int(round(64 * 16 ** s))
-        out.append(random_trail(rng, nxy, nxy, nz, nntf=nntf))
+        out.append(random_trail(rng, nxy, nxy, nz, nntf=nntf, maps=maps))
     return out

@@ -91,16 +91,9 @@ struct Server : ServerBase {
   };
 
   int device = 0;
+  // Every gang launch goes to `run`, one after another (a captured graph is
+  // a single chain).  Two streams in all: `run` and the worker's `setup`.
   hipStream_t run = nullptr, setup = nullptr;
-  // The one-step launches of a half write disjoint cells of their solves'
-  // next sets and read only the current ones: with MSC_LIST_STREAMS=1 or 2
-  // they spread over `run` and that many auxiliary streams between a fork
-  // and a join (in a captured graph, parallel branches).  Off by default:
-  // a full gang already fills the GPU, so it brings no measurable gain.
-  static constexpr int kAux = 2;
-  int list_streams = 0;
-  hipStream_t aux[kAux] = {};
-  hipEvent_t fork_ev = nullptr, join_ev[kAux] = {};
   Arena arena;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
@@ -177,12 +170,6 @@ struct Server : ServerBase {
     HIPCHK(hipGetDevice(&device));
     HIPCHK(hipStreamCreateWithFlags(&run, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&setup, hipStreamNonBlocking));
-    if (const char* f = std::getenv("MSC_LIST_STREAMS")) list_streams = std::max(0, std::min(kAux, std::atoi(f)));
-    HIPCHK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
-    for (int q = 0; q < list_streams; q++) {
-      HIPCHK(hipStreamCreateWithFlags(&aux[q], hipStreamNonBlocking));
-      HIPCHK(hipEventCreateWithFlags(&join_ev[q], hipEventDisableTiming));
-    }
     for (auto& e : iter_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto& g : gang) HIPCHK(hipEventCreateWithFlags(&g.last_use, hipEventDisableTiming));
     if (budget == 0) {
@@ -222,14 +209,6 @@ struct Server : ServerBase {
     }
     for (auto& e : iter_ev)
       if (e) hipEventDestroy(e);
-    for (int q = 0; q < kAux; q++) {
-      if (aux[q]) {
-        hipStreamSynchronize(aux[q]);
-        hipStreamDestroy(aux[q]);
-      }
-      if (join_ev[q]) hipEventDestroy(join_ev[q]);
-    }
-    if (fork_ev) hipEventDestroy(fork_ev);
     if (run) hipStreamDestroy(run);
     if (setup) hipStreamDestroy(setup);
   }
@@ -385,8 +364,12 @@ struct Server : ServerBase {
       const double la = left(a), lb = left(b);
       return la != lb ? la < lb : a->id < b->id;
     });
+    // a solve's workgroups in one iteration: its two-step launch list when
+    // it runs two-step tiles (outer one-step tiles, then the region's), the
+    // one-step tiling otherwise (the count rebuild() sizes the gang by)
     size_t tiles = 0, q = 0;
-    for (; q < sel.size() && (q == 0 || tiles < (size_t)fill_tiles); q++) tiles += sel[q]->s->tiles_host.size();
+    for (; q < sel.size() && (q == 0 || tiles < (size_t)fill_tiles); q++)
+      tiles += pairs(sel[q]) ? sel[q]->s->pair_host.size() : sel[q]->s->tiles_host.size();
     sel.resize(q);
   }
 
@@ -486,52 +469,6 @@ struct Server : ServerBase {
   // serve every iteration until the gang changes.
   hipError_t launch_fixed(const Gang& g, int half) {
     hipError_t e = hipSuccess;
-    int naux = 0;
-    while (naux < list_streams && aux[naux]) naux++;
-    if (naux > 0) {
-      // one-step launches largest first onto the least loaded stream (load
-      // = tiles x tile rows, CPML variants counted 1.5 times), then the
-      // two-step launches, whose outer tiles those launches advanced
-      struct Lst { int c, v, y, n; size_t o; double w; };
-      std::vector<Lst> ls;
-      for (int c = 0; c < 3; c++)
-        for (int v = 0; v < kNumLists; v++)
-          for (int y = 0; y < 3; y++) {
-            const int n = half ? n_tiles_b[c][v][y] : n_tiles[c][v][y];
-            if (n) ls.push_back({c, v, y, n, tile_off[c][v][y], (double)n * kTys[y] * (v == kListInterior ? 1.0 : 1.5)});
-          }
-      std::sort(ls.begin(), ls.end(), [](const Lst& a, const Lst& b) { return a.w > b.w; });
-      hipStream_t st[kAux + 1] = {run};
-      for (int q = 0; q < naux; q++) st[q + 1] = aux[q];
-      double load[kAux + 1] = {};
-      bool used[kAux + 1] = {};
-      e = hipEventRecord(fork_ev, run);
-      for (const Lst& l : ls) {
-        if (e != hipSuccess) break;
-        int q = 0;
-        for (int k = 1; k <= naux; k++)
-          if (load[k] < load[q]) q = k;
-        load[q] += l.w;
-        if (q > 0 && !used[q]) e = hipStreamWaitEvent(st[q], fork_ev, 0);
-        used[q] = true;
-        if (e == hipSuccess)
-          e = launch_step<T>(g.sa, g.sids + l.o, kTys[l.y], l.c, list_pm(l.v), false, kServerZLanes, g.tiles + l.o,
-                             l.n, st[q]);
-      }
-      // every forked stream joins, also after an error (a capture must end joined)
-      for (int q = 1; q <= naux; q++) {
-        if (!used[q]) continue;
-        hipError_t e1 = hipEventRecord(join_ev[q - 1], st[q]);
-        if (e1 == hipSuccess) e1 = hipStreamWaitEvent(run, join_ev[q - 1], 0);
-        if (e == hipSuccess) e = e1;
-      }
-      for (int c = 0; c < 3 && e == hipSuccess; c++)
-        if (!half && n_ptiles[c])
-          e = launch_pair<T>(g.sa, g.sids + ptile_off[c], c, g.tiles + ptile_off[c], n_ptiles[c], run);
-      if (e == hipSuccess) e = launch_ntff<T>(g.na, g.nb, half ? n_nb_b : n_nb, run);
-      if (e == hipSuccess) e = launch_probes<T>(g.xa, half ? (int)n_pair : (int)sel.size(), run);
-      return e;
-    }
     for (int c = 0; c < 3 && e == hipSuccess; c++) {
       for (int v = 0; v < kNumLists && e == hipSuccess; v++)
         for (int y = 0; y < 3 && e == hipSuccess; y++) {
@@ -581,7 +518,7 @@ struct Server : ServerBase {
       }
       if (e != hipSuccess) return fail(MSC_EHIP, "solve %lld: %s", (long long)r->id, hipGetErrorString(e));
       r->ntff.resize(r->s->ntff_doubles());
-      int rc = r->s->get_ntff(r->ntff.data());
+      int rc = r->s->get_ntff(r->ntff.data(), r->t);
       if (rc) return rc;
       r->probes.resize((size_t)6 * r->s->plan.T * r->p.nprobes);
       if (!r->probes.empty() && (rc = r->s->get_probes(r->probes.data()))) return rc;
@@ -767,6 +704,13 @@ int msc_server_create(int precision, size_t budget_bytes, msc_server** out) {
   if (!out) return fail(MSC_EINVAL, "NULL argument");
   *out = nullptr;
   if (precision != 32 && precision != 64) return fail(MSC_EINVAL, "precision must be 32 or 64 (got %d)", precision);
+  // Gang launches spread over auxiliary streams (parallel branches in the
+  // captured graphs) were removed: two large-gang batch runs with them
+  // stopped making progress, with no cause found.  The setting that chose
+  // them is refused rather than ignored.
+  if (const char* f = std::getenv("MSC_LIST_STREAMS"))
+    if (std::atoi(f) != 0)
+      return fail(MSC_EINVAL, "MSC_LIST_STREAMS=%s: the server launches on one stream only (unset it or set 0)", f);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MSC_EHIP, "no HIP device visible");
   std::unique_ptr<ServerBase> s;

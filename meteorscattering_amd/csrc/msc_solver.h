@@ -31,12 +31,13 @@ struct HostPlan {
   std::vector<double> srcA, srcB;      // T
   std::vector<double> inc;             // [T][4][nx]
   std::vector<double> eline, hline;    // [nframes][2][nx] incident lines at frame steps
-  std::vector<double> eydft, ezdft;    // 2M
   std::vector<double> sincos;          // [T][M][2]
   std::vector<double> zcoord;          // nz
 };
 
 int make_plan(const msc_params* p, HostPlan& h);
+// The source DFT sums Eydft[2M], Ezdft[2M] through `steps` time steps.
+void source_dft(const HostPlan& h, int steps, double* ey, double* ez);
 
 // z lanes of the one-step tiles of the server's gang launches
 constexpr int kServerZLanes = 64;
@@ -126,7 +127,10 @@ struct SolverBase {
   virtual int init(const double* wp, const double* nu) = 0;
   virtual int step(int n) = 0;
   virtual int get_field(int comp, double* out) = 0;
-  virtual int get_ntff(double* out) = 0;
+  // The NTF payload (source DFT, then the 24 face arrays) after `steps`
+  // steps: the source DFT is summed through that many steps, the face sums
+  // are what the device holds.
+  virtual int get_ntff(double* out, int steps) = 0;
   // Contract the NTFF samples of the current partial K-step block now
   // (asynchronous, on `stream`; nothing to do when the last step closed a
   // block or the run is complete)
@@ -164,7 +168,7 @@ struct Solver : SolverBase {
   int init(const double* wp, const double* nu) override;
   int step(int n) override;
   int get_field(int comp, double* out) override;
-  int get_ntff(double* out) override;
+  int get_ntff(double* out, int steps) override;
   int flush_ntff() override;
   int get_probes(double* out) override;
   int get_frame(int f, double* e, double* h, double* emax) override;

@@ -84,7 +84,7 @@ static void plan_cpml(HostPlan& h) {
 // 1137,1232,1261,1290,1308-1335,1389), giving per step the TF/SF correction
 // values, the source DFT (:1407-1414) and the lines written with each frame.
 static void plan_incident(HostPlan& h) {
-  const int nx = h.nx, L = h.L, T = h.T, M = h.M;
+  const int nx = h.nx, L = h.L, T = h.T;
   const int xs = nx - 2 * L - 1;
   const double dx = h.dx, c1h = h.c1h, c2h = h.c2h, c1e0 = h.c1e0, c2e0 = h.c2e0;
   std::vector<double> eyi(nx, 0), ezi(nx, 0), hyi(nx, 0), hzi(nx, 0);
@@ -93,8 +93,6 @@ static void plan_incident(HostPlan& h) {
   h.inc.assign((size_t)T * 4 * nx, 0);
   h.eline.assign((size_t)h.nframes * 2 * nx, 0);
   h.hline.assign((size_t)h.nframes * 2 * nx, 0);
-  h.eydft.assign(2 * M, 0);
-  h.ezdft.assign(2 * M, 0);
   for (int t = 0; t < T; t++) {
     double* row = &h.inc[(size_t)t * 4 * nx];
     for (int i = 0; i < nx; i++) {
@@ -135,19 +133,39 @@ static void plan_incident(HostPlan& h) {
     }
     if (pml)
       for (int i = nx - L - 1; i < nx - 1; i++) ezi[i] = ezi[i] + c2e0 * pez[i - xs];
-    for (int m = 0; m < M; m++) {
-      const double sn = h.sincos[((size_t)t * M + m) * 2], cs = h.sincos[((size_t)t * M + m) * 2 + 1];
-      h.ezdft[2 * m] += ezi[0] * sn;
-      h.ezdft[2 * m + 1] += ezi[0] * cs;
-      h.eydft[2 * m] += eyi[0] * sn;
-      h.eydft[2 * m + 1] += eyi[0] * cs;
-    }
     if (t % h.cadence == 0) {
       const int f = t / h.cadence;
       std::copy(eyi.begin(), eyi.end(), h.eline.begin() + (size_t)f * 2 * nx);
       std::copy(ezi.begin(), ezi.end(), h.eline.begin() + (size_t)f * 2 * nx + nx);
       std::copy(hyi.begin(), hyi.end(), h.hline.begin() + (size_t)f * 2 * nx);
       std::copy(hzi.begin(), hzi.end(), h.hline.begin() + (size_t)f * 2 * nx + nx);
+    }
+  }
+}
+
+// Source DFT (meteor-fork.cpp:1407-1414) through `steps` steps: the hard
+// source values of the incident line's first cell (:1308-1335) times the
+// phase table, summed in step order as the reference does.
+void source_dft(const HostPlan& h, int steps, double* ey, double* ez) {
+  const int M = h.M;
+  std::fill(ey, ey + 2 * M, 0.0);
+  std::fill(ez, ez + 2 * M, 0.0);
+  for (int t = 0; t < std::min(steps, h.T); t++) {
+    const double a = h.srcA[t], b = h.srcB[t];
+    double e0z, e0y;
+    switch (h.p.sourcepol) {
+      case 2: e0z = 0; e0y = a; break;
+      case 3: e0z = a; e0y = a; break;
+      case 4: e0z = a; e0y = b; break;
+      case 5: e0z = b; e0y = a; break;
+      default: e0z = a; e0y = 0; break;
+    }
+    for (int m = 0; m < M; m++) {
+      const double sn = h.sincos[((size_t)t * M + m) * 2], cs = h.sincos[((size_t)t * M + m) * 2 + 1];
+      ez[2 * m] += e0z * sn;
+      ez[2 * m + 1] += e0z * cs;
+      ey[2 * m] += e0y * sn;
+      ey[2 * m + 1] += e0y * cs;
     }
   }
 }
@@ -962,13 +980,12 @@ int Solver<T>::get_field(int comp, double* out) {
 }
 
 template <typename T>
-int Solver<T>::get_ntff(double* out) {
+int Solver<T>::get_ntff(double* out, int steps) {
   const size_t M2 = 2 * (size_t)plan.M;
   std::vector<double> a(acc_n);
   HIPCHK(hipMemcpyAsync(a.data(), acc, acc_n * 8, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));
-  std::memcpy(out, plan.eydft.data(), M2 * 8);
-  std::memcpy(out + M2, plan.ezdft.data(), M2 * 8);
+  source_dft(plan, steps, out, out + M2);
   double* o = out + 2 * M2;
   const size_t fb = (size_t)plan.ny * plan.nz, tb = (size_t)plan.nx * plan.nz, lr = (size_t)plan.nx * plan.ny;
   for (int q = 0; q < 24; q++) {
@@ -1220,7 +1237,7 @@ int msc_solver_get_ntff(msc_solver* s, double* out, size_t n) {
   SOLVER_OK(s);
   if (!out || n < msc_solver_ntff_count(s)) return fail(MSC_EINVAL, "output buffer too small");
   int rc = s->impl->flush_ntff();
-  return rc ? rc : s->impl->get_ntff(out);
+  return rc ? rc : s->impl->get_ntff(out, s->impl->t_host);
 }
 
 int msc_solver_flush_ntff(msc_solver* s) {

@@ -19,13 +19,12 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import sys
 import time
 
 import numpy as np
 
-from . import configs, rcs, runio
+from . import configs, rcs
 from .solver import Solver
 
 
@@ -38,13 +37,6 @@ def run(p, wp, nu, precision: int) -> dict:
         fields = [s.field(c) for c in range(9)]
         ntff = s.ntff()
     return {"fields": fields, "ntff": ntff, "seconds": el}
-
-
-def directions():
-    a = np.radians(np.arange(181.0))
-    th = np.concatenate([np.full(181, math.pi / 2), a])
-    ph = np.concatenate([a, np.zeros(181)])
-    return th, ph
 
 
 def study(steps: int = 2000) -> dict:
@@ -65,29 +57,13 @@ def study(steps: int = 2000) -> dict:
     out["field_max_rel_error"] = kinds
     a, b = r32["ntff"], r64["ntff"]
     out["ntff_max_rel_error"] = float(np.abs(a - b).max() / np.abs(b).max())
-    b_ = (p.get("pmllen") or 10) + 5
-    n32 = runio.parse_ntf(runio.encode_ntf(p["nntf"], b_, p["fntf"], a), p["nx"], p["ny"], p["nz"])
-    n64 = runio.parse_ntf(runio.encode_ntf(p["nntf"], b_, p["fntf"], b), p["nx"], p["ny"], p["nz"])
-    th, ph = directions()
     t0 = time.perf_counter()
-    s32 = rcs.bistatic(n32, p, th, ph)["sig"]
-    s64 = rcs.bistatic(n64, p, th, ph)["sig"]
+    e = rcs.db_errors(a, b, p)
     out["rcs_seconds"] = round(time.perf_counter() - t0, 1)
-    db32, db64 = rcs.db(s32), rcs.db(s64)
-    per_f = []
-    for m in range(p["nntf"]):
-        errs = []
-        for cut in (slice(0, 181), slice(181, 362)):
-            ref = db64[cut, m]
-            keep = ref > ref.max() - 40.0
-            errs.append(np.abs(db32[cut, m] - ref)[keep].max())
-        per_f.append(float(max(errs)))
-    out["bistatic_db_error_per_freq"] = [round(e, 5) for e in per_f]
-    out["bistatic_db_error_max"] = round(max(per_f), 5)
-    m32 = rcs.db(rcs.monostatic(n32, p)["sig"])
-    m64 = rcs.db(rcs.monostatic(n64, p)["sig"])
-    out["monostatic_db_error_per_freq"] = [round(float(x), 6) for x in np.abs(m32 - m64)]
-    out["monostatic_dbsm_fp64"] = [round(float(x), 3) for x in m64]
+    out["bistatic_db_error_per_freq"] = [round(x, 5) for x in e["bistatic_db_per_freq"]]
+    out["bistatic_db_error_max"] = round(e["bistatic_db_max"], 5)
+    out["monostatic_db_error_per_freq"] = [round(x, 6) for x in e["monostatic_db_per_freq"]]
+    out["monostatic_dbsm_fp64"] = [round(x, 3) for x in e["monostatic_dbsm_reference"]]
     return out
 
 

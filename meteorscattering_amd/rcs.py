@@ -183,6 +183,50 @@ def db(sig) -> np.ndarray:
     return 10 * np.log10(np.asarray(sig))
 
 
+def principal_cuts(step_deg: float = 1.0):
+    """Two principal-plane cuts: theta = 90 deg with phi from 0 to 180 deg,
+    then phi = 0 with theta from 0 to 180 deg (181 directions each at 1 deg).
+    Returns (theta, phi) in radians and the slices of the two cuts."""
+    a = np.radians(np.arange(0.0, 180.0 + 1e-9, step_deg))
+    n = a.size
+    th = np.concatenate([np.full(n, math.pi / 2), a])
+    ph = np.concatenate([a, np.zeros(n)])
+    return th, ph, (slice(0, n), slice(n, 2 * n))
+
+
+def payload_to_ntf(payload: np.ndarray, p: dict) -> dict:
+    """The msc_solver_get_ntff payload as the face dict parse_ntf returns."""
+    b = (p.get("pmllen") or 10) + 5
+    return runio.parse_ntf(runio.encode_ntf(p["nntf"], b, p["fntf"], payload), p["nx"], p["ny"], p["nz"])
+
+
+def db_errors(payload: np.ndarray, reference: np.ndarray, p: dict, floor_db: float = 40.0,
+              step_deg: float = 1.0) -> dict:
+    """RCS of `payload` against that of `reference` (same solve), in dB.
+
+    Per frequency: the largest |dB difference| of the bistatic RCS over the
+    two principal cuts, taken over directions where the reference RCS is
+    within `floor_db` of its own maximum on that cut (deep nulls carry no
+    meaningful dB figure), and the monostatic (meteorNTF.m backscatter)
+    difference."""
+    a, r = payload_to_ntf(payload, p), payload_to_ntf(reference, p)
+    th, ph, cuts = principal_cuts(step_deg)
+    da, dr = db(bistatic(a, p, th, ph)["sig"]), db(bistatic(r, p, th, ph)["sig"])
+    per_f = []
+    for m in range(da.shape[1]):
+        worst = 0.0
+        for cut in cuts:
+            ref = dr[cut, m]
+            keep = ref > ref.max() - floor_db
+            worst = max(worst, float(np.abs(da[cut, m] - ref)[keep].max()))
+        per_f.append(worst)
+    ma, mr = db(monostatic(a, p)["sig"]), db(monostatic(r, p)["sig"])
+    return {"bistatic_db_per_freq": per_f, "bistatic_db_max": max(per_f),
+            "monostatic_db_per_freq": [float(x) for x in np.abs(ma - mr)],
+            "monostatic_db_max": float(np.abs(ma - mr).max()),
+            "monostatic_dbsm_reference": [float(x) for x in mr]}
+
+
 def mie_pec_backscatter(f, radius: float, terms: int = 100, c: float = 3e8) -> np.ndarray:
     """PEC-sphere backscatter RCS (m^2), the series of
     analysis/sphereRCSvsFreq.m:3-31 with spherical Bessel/Hankel functions."""

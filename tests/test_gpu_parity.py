@@ -220,7 +220,7 @@ def test_two_step_tiles_are_bit_identical(msc, monkeypatch, name, jcol, chunk2, 
 
 
 @pytest.mark.parametrize("name", ["lk_ricker_ez", "lk_nopml_lhcp", "plate_ricker_45"])
-@pytest.mark.parametrize("pair,precision", [(1, 32), (0, 32), (0, 64)])
+@pytest.mark.parametrize("pair,precision", [(1, 32), (0, 32), (1, 64), (0, 64)])
 def test_lane_widths_give_the_same_bits(msc, monkeypatch, name, pair, precision):
     """One-step tiles of 64 z lanes (63 owned z cells; the server's width)
     and of 32 lanes (two rows per wave, 31 cells; the single-solve default,
@@ -446,11 +446,13 @@ def test_timing_record_counts_launch_kinds(msc, monkeypatch):
             assert tm["pair_launches"] == 0 and tm["pair_cells"] == 0
 
 
-def test_default_two_step_mode_follows_the_region_size(msc, monkeypatch):
+@pytest.mark.parametrize("precision", [32, 64])
+def test_default_two_step_mode_follows_the_region_size(msc, monkeypatch, precision):
     """With MSC_PAIR unset, a single solve runs two-step tiles when its
     region gives at least 1.5 fdtd_pair tiles per CU: 200x200x300 with L = 10
-    has a region [16,184)^2 x [31,279) of 6 x 14 x 5 = 420 tiles, the
-    60x50x205 grid 1 x 2 x 3 = 6."""
+    has a region [16,184)^2 x [31,279) of 6 x 14 x 5 = 420 tiles in FP32
+    (60 owned z cells per tile) and 6 x 14 x 9 = 756 in FP64 (28 owned z
+    cells), the 60x50x205 grid 1 x 2 x 3 = 6 and 1 x 2 x 6 = 12."""
     import ctypes
 
     Solver, _ = msc
@@ -462,12 +464,13 @@ def test_default_two_step_mode_follows_the_region_size(msc, monkeypatch):
     assert hip.hipDeviceGetAttribute(ctypes.byref(n), 63, 0) == 0
     cus = n.value
     assert cus > 0
-    for (nx, ny, nz), tiles in (((200, 200, 300), 6 * 14 * 5), ((60, 50, 205), 1 * 2 * 3)):
+    zt = {32: (5, 3), 64: (9, 6)}[precision]
+    for (nx, ny, nz), tiles in (((200, 200, 300), 6 * 14 * zt[0]), ((60, 50, 205), 1 * 2 * zt[1])):
         p, _, _ = cases.case("lk_ricker_ez")
         p.update(nx=nx, ny=ny, nz=nz, tsteps=12, numfiles=1, nprobes=1, probex=[nx // 2], probey=[ny // 2],
                  probez=[nz // 2])
         wp, nu = cases.trail_maps(p, n0=2e13, r0=1.5, nu0=1e6, centre=(0.5, -0.25))
-        with Solver(p, wp, nu, precision=32) as g:
+        with Solver(p, wp, nu, precision=precision) as g:
             g.step(2)
             g.set_timing(True)
             g.step(6)
@@ -516,3 +519,25 @@ def test_autotuned_and_explicit_tilings_give_the_same_bits(msc, monkeypatch):
         g.close()
     for o in outs[1:]:
         assert all(np.array_equal(a, b) for a, b in zip(o, outs[0]))
+
+
+@pytest.mark.parametrize("nntf", [2, 16])
+@pytest.mark.parametrize("precision,tol", [(64, 1e-10), (32, 2e-5)])
+def test_ntff_read_mid_run_matches_oracle(msc, oracle_lib, nntf, precision, tol):
+    """NTFF payloads read at steps inside K-step blocks (the buffered samples
+    are flushed into the sums, the source DFT is summed through the step),
+    twice in one block, at a block end and at the end of the run: each equals
+    the oracle's running DFT after the same step, and reading does not change
+    the later sums."""
+    Solver, _ = msc
+    p, wp, nu = cases.case("lk_ricker_ez")
+    p.update(nntf=nntf, fntf=list(np.linspace(40e6, 60e6, nntf)))
+    g = Solver(p, wp, nu, precision=precision)
+    o = oracle_lib.Oracle(p, wp, nu)
+    for t in (45, 50, 64, 101, p["tsteps"]):
+        g.step(t - g.t)
+        o.step(t - o.t)
+        e = relerr(g.ntff(), o.ntff())
+        assert e < tol, (t, e)
+    g.close()
+    o.close()

@@ -187,21 +187,16 @@ def test_fp64_two_step_gangs_equal_alone():
         assert st["iterations"] == expect
 
 
-@pytest.mark.parametrize("streams,graphs", [(2, True), (1, True), (1, False)])
-def test_gang_lists_on_several_streams_equal_alone(monkeypatch, streams, graphs):
-    """The gang's one-step launches of a half spread over the run stream and
-    one or two auxiliary streams (MSC_LIST_STREAMS), captured as parallel
-    graph branches or launched directly: NTFF payloads and probes equal the
-    single-solve bits (the default, the run stream alone, is covered above)."""
-    mix = {k: f() for k, f in TWO_STEP_MIX.items()}
-    alone = {k: _alone(*v) for k, v in mix.items()}
-    monkeypatch.setenv("MSC_LIST_STREAMS", str(streams))
-    with Server(precision=32, budget_bytes=4 << 30, fill_tiles=0, graphs=graphs) as srv:
-        ids = {k: srv.submit(*v) for k, v in mix.items()}
-        srv.run_until_done()
-        for k in mix:
-            np.testing.assert_array_equal(srv.ntff(ids[k]), alone[k][0], err_msg=k)
-            np.testing.assert_array_equal(srv.probes(ids[k]), alone[k][1], err_msg=k)
+def test_server_refuses_auxiliary_list_streams(monkeypatch):
+    """The server launches its gangs on one stream: MSC_LIST_STREAMS other
+    than 0 is refused at creation (the single-solve setting of that name
+    does not carry over), and 0 is accepted."""
+    monkeypatch.setenv("MSC_LIST_STREAMS", "2")
+    with pytest.raises(RuntimeError, match="MSC_LIST_STREAMS"):
+        Server(precision=32, budget_bytes=1 << 30)
+    monkeypatch.setenv("MSC_LIST_STREAMS", "0")
+    with Server(precision=32, budget_bytes=1 << 30) as srv:
+        assert srv.stats()["arena_bytes"] >= 1 << 30
 
 
 def test_size_class_tilings_equal_alone():

@@ -17,6 +17,7 @@ import math
 import numpy as np
 import pytest
 
+import cases
 from meteorscattering_amd import configs, rcs, runio
 
 
@@ -151,3 +152,24 @@ def test_even_frequency_recurrence_equals_direct_phases():
         for name in runio.ntf_face_names():
             one[name] = ntf[name][..., 2 * m:2 * m + 2]
         np.testing.assert_allclose(rcs.bistatic(one, p, th, ph)["sig"][:, 0], many[:, m], rtol=1e-10)
+
+
+def test_db_errors_closed_form(oracle_lib):
+    """rcs.db_errors: a payload against itself is 0 dB; scaling every face
+    sum by c (the source DFT unchanged) scales L and N by c and sigma by c^2,
+    20 log10(c) dB in every direction and at every frequency."""
+    p, wp, nu = cases.case("lk_ricker_ez")
+    p.update(tsteps=120, nntf=3, fntf=[45e6, 50e6, 55e6])
+    o = oracle_lib.Oracle(p, wp, nu)
+    o.step(120)
+    a = o.ntff()
+    o.close()
+    e0 = rcs.db_errors(a, a, p)
+    assert e0["bistatic_db_max"] == 0.0 and e0["monostatic_db_max"] == 0.0
+    c = 1.0 + 3e-3
+    b = a.copy()
+    b[4 * p["nntf"]:] *= c
+    e = rcs.db_errors(b, a, p)
+    want = 20 * math.log10(c)
+    np.testing.assert_allclose(e["bistatic_db_per_freq"], want, rtol=1e-9)
+    np.testing.assert_allclose(e["monostatic_db_per_freq"], want, rtol=1e-9)
