@@ -192,6 +192,19 @@ int msc_solver_time(const msc_solver* s);
  * components are those the reference holds after the same number of steps. */
 int msc_solver_get_field(msc_solver* s, int comp, double* out);
 
+/* msc_solver_get_field without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops and harnesses that forbid hipMalloc).  Same values as
+ * msc_solver_get_field, bit for bit; `n` (the capacity of `out`) must be at
+ * least nx*ny*nz.  The component is converted to FP64 (and J caught up by
+ * the deferred current update) in a fixed device scratch array of 2^20
+ * doubles (8 MiB) that the library holds on every device it runs on until
+ * the process ends, in passes of at most 2^20 cells copied straight into
+ * `out`, so any grid size fits.  Every solver on a device shares that
+ * device's scratch: calls that overlap on one device race (make them from
+ * one host thread per device at a time).  Synchronous. */
+int msc_solver_get_field_noalloc(msc_solver* s, int comp, double* out, size_t n);
+
 /* Number of doubles in the NTF payload: Eydft[2M], Ezdft[2M], then the 24
  * face arrays of output_NTF.dat in file order (meteor-fork.cpp:1677-1714).
  * msc_solver_get_ntff returns the running DFT through the last step

@@ -203,6 +203,7 @@ def _declare(lib: C.CDLL) -> None:
         "msc_solver_sync": (C.c_int, [vp]),
         "msc_solver_time": (C.c_int, [vp]),
         "msc_solver_get_field": (C.c_int, [vp, C.c_int, dp]),
+        "msc_solver_get_field_noalloc": (C.c_int, [vp, C.c_int, dp, C.c_size_t]),
         "msc_solver_ntff_count": (C.c_size_t, [vp]),
         "msc_solver_get_ntff": (C.c_int, [vp, dp, C.c_size_t]),
         "msc_solver_flush_ntff": (C.c_int, [vp]),

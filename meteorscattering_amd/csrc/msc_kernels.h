@@ -164,6 +164,11 @@ template <typename T>
 hipError_t launch_probes(const AuxArgs<T>* a, int nsolves, hipStream_t s);
 template <typename T>
 hipError_t launch_catchup(const StepArgs<T>* a, int cur, T* ox, T* oy, T* oz, hipStream_t s);
+// Component `comp` (MSC_EX..MSC_JZ, J caught up) of cells [c0, c0 + nc) as
+// FP64 into dst (device memory of nc doubles).
+template <typename T>
+hipError_t launch_field_slab(const StepArgs<T>* a, int cur, int comp, long long c0, long long nc, double* dst,
+                             hipStream_t s);
 hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 }  // namespace msc

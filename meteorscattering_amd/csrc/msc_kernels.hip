@@ -23,6 +23,8 @@
 // advance the tiles of many independent solves together.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "msc_kernels.h"
 
 namespace msc {
@@ -939,6 +941,46 @@ __global__ void current_catchup(const StepArgs<T>* __restrict__ ap, T* ox, T* oy
   }
 }
 
+// One field component of cells [c0, c0 + nc) (flat [nx][ny][nz] index),
+// as FP64, into dst[0, nc): E or H of the current set, or J caught up as
+// current_catchup does (all three components are updated, one is kept, so
+// the arithmetic is current_catchup's).  Reads the step from the device
+// counter, so it follows the solver's stream.
+template <typename T, int CUR>
+__global__ void __launch_bounds__(256) field_slab(const StepArgs<T>* __restrict__ ap, int comp, long long c0,
+                                                  long long nc, double* __restrict__ dst) {
+  const ConstArgs<StepArgs<T>>& a = as_const_space(ap);
+  const int t = *a.tdev;
+  const T* Ex = a.e[t % 3][0];
+  const T* Ey = a.e[t % 3][1];
+  const T* Ez = a.e[t % 3][2];
+  const int sj = (t + 2) % 3;  // J^(t-1)
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nc; q += (long long)gridDim.x * blockDim.x) {
+    const long long c = c0 + q;
+    T v;
+    if (comp < 3) {
+      v = a.e[t % 3][comp][c];
+    } else if (comp < 6) {
+      v = a.h[t % 3][comp - 3][c];
+    } else {
+      const int k = (int)(c % a.nz);
+      const int j = (int)((c / a.nz) % a.ny);
+      const int i = (int)(c / a.sx);
+      T jx = a.j[sj][0][c], jy = a.j[sj][1][c], jz = a.j[sj][2][c];
+      if (i >= 1 && i < a.nx - 1 && j >= 1 && j < a.ny - 1 && k >= 1 && k < a.nz - 1) {
+        if (CUR == kCurDI || a.jcol) {
+          current_update<T, CUR>(a, a.nue[c], a.wpe2[c], jx, jy, jz, Ex[c], Ey[c], Ez[c]);
+        } else {
+          current_update<T, CUR>(a, a.nue[c], a.wpe2[c], jx, jy, jz, (Ex[c] + Ex[c - a.sx]) / 2,
+                                 (Ey[c] + Ey[c - a.sy]) / 2, (Ez[c] + Ez[c - 1]) / 2);
+        }
+      }
+      v = comp == 6 ? jx : comp == 7 ? jy : jz;
+    }
+    dst[q] = (double)v;
+  }
+}
+
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 // The same contraction on the matrix cores, for 2M >= 16 frequency rows:
@@ -1356,6 +1398,23 @@ hipError_t launch_catchup(const StepArgs<T>* a, int cur, T* ox, T* oy, T* oz, hi
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_field_slab(const StepArgs<T>* a, int cur, int comp, long long c0, long long nc, double* dst,
+                             hipStream_t s) {
+  if (nc <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)std::min<long long>((nc + 255) / 256, 4096);
+  switch (cur) {
+    case kCurDI: hipLaunchKernelGGL((field_slab<T, kCurDI>), dim3(blocks), dim3(256), 0, s, a, comp, c0, nc, dst); break;
+    case kCurLKPoly:
+      hipLaunchKernelGGL((field_slab<T, kCurLKPoly>), dim3(blocks), dim3(256), 0, s, a, comp, c0, nc, dst);
+      break;
+    default:
+      hipLaunchKernelGGL((field_slab<T, kCurLKClosed>), dim3(blocks), dim3(256), 0, s, a, comp, c0, nc, dst);
+      break;
+  }
+  return hipGetLastError();
+}
+
 #define MSC_INST(T)                                                                                                \
   template hipError_t launch_step<T>(const StepArgs<T>*, const uint16_t*, int, int, int, bool, int, const uint32_t*, int, \
                                      hipStream_t);                                                                 \
@@ -1364,7 +1423,8 @@ hipError_t launch_catchup(const StepArgs<T>* a, int cur, T* ox, T* oy, T* oz, hi
   template hipError_t launch_ntff_contract<T>(const NtffArgs<T>*, long long, int, long long, hipStream_t, int);    \
   template hipError_t launch_frames<T>(const AuxArgs<T>*, hipStream_t);                                            \
   template hipError_t launch_probes<T>(const AuxArgs<T>*, int, hipStream_t);                                       \
-  template hipError_t launch_catchup<T>(const StepArgs<T>*, int, T*, T*, T*, hipStream_t);
+  template hipError_t launch_catchup<T>(const StepArgs<T>*, int, T*, T*, T*, hipStream_t);                     \
+  template hipError_t launch_field_slab<T>(const StepArgs<T>*, int, int, long long, long long, double*, hipStream_t);
 MSC_INST(float)
 MSC_INST(double)
 #undef MSC_INST

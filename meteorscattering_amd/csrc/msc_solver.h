@@ -127,6 +127,10 @@ struct SolverBase {
   virtual int init(const double* wp, const double* nu) = 0;
   virtual int step(int n) = 0;
   virtual int get_field(int comp, double* out) = 0;
+  // get_field through the device's fixed read-back scratch (no allocation)
+  virtual int get_field_noalloc(int comp, double* out) = 0;
+  int device = 0;                   // the device the solver was created on
+  double* scratch_dev = nullptr;    // that device's read-back scratch
   // The NTF payload (source DFT, then the 24 face arrays) after `steps`
   // steps: the source DFT is summed through that many steps, the face sums
   // are what the device holds.
@@ -168,6 +172,7 @@ struct Solver : SolverBase {
   int init(const double* wp, const double* nu) override;
   int step(int n) override;
   int get_field(int comp, double* out) override;
+  int get_field_noalloc(int comp, double* out) override;
   int get_ntff(double* out, int steps) override;
   int flush_ntff() override;
   int get_probes(double* out) override;

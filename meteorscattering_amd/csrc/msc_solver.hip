@@ -8,11 +8,47 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
 
 namespace msc {
+
+// ------------------------------------------------- allocation-free read-back
+// Device scratch of msc_solver_get_field_noalloc: one fixed array of
+// kScratchDoubles FP64 values (8 MiB) on each device the library runs on,
+// shared by every solver there and held until the process ends.  In host
+// code `scratch` names a host-side stand-in, never the device memory: the
+// device address is looked up once per device, with that device current,
+// at solver creation (scratch_on_current_device), and only that address is
+// handed to kernels and copies.
+constexpr size_t kScratchDoubles = size_t(1) << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+double* scratch_addr[kMaxDevices] = {};
+}  // namespace
+
+// The scratch array's address on the current device (looked up once per
+// device).
+static int scratch_on_current_device(int* dev_out, double** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(MSC_EINVAL, "device %d beyond the %d supported", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (!scratch_addr[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(MSC_EHIP, "no device address for the read-back scratch on device %d", dev);
+    scratch_addr[dev] = static_cast<double*>(p);
+  }
+  *dev_out = dev;
+  *out = scratch_addr[dev];
+  return MSC_OK;
+}
 
 // ------------------------------------------------------------ host precompute
 
@@ -979,6 +1015,25 @@ int Solver<T>::get_field(int comp, double* out) {
   return MSC_OK;
 }
 
+// msc_solver_get_field without allocating: the grid in passes of at most
+// kScratchDoubles cells through the device scratch (field_slab writes the
+// component as FP64, J caught up as current_catchup does), each pass copied
+// straight into `out`.  Kernel and copy of a pass share the solver's stream,
+// so the next pass's kernel starts after the copy has read the scratch.
+template <typename T>
+int Solver<T>::get_field_noalloc(int comp, double* out) {
+  if (comp < 0 || comp > 8) return fail(MSC_EINVAL, "component %d out of range", comp);
+  if (!scratch_dev) return fail(MSC_ESTATE, "no read-back scratch on this solver's device");
+  const long long N = (long long)plan.nx * plan.ny * plan.nz, per = (long long)kScratchDoubles;
+  for (long long c0 = 0; c0 < N; c0 += per) {
+    const long long nc = std::min(per, N - c0);
+    HIPCHK(launch_field_slab<T>(sa_dev, cur, comp, c0, nc, scratch_dev, stream));
+    HIPCHK(hipMemcpyAsync(out + c0, scratch_dev, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost, stream));
+  }
+  HIPCHK(hipStreamSynchronize(stream));
+  return MSC_OK;
+}
+
 template <typename T>
 int Solver<T>::get_ntff(double* out, int steps) {
   const size_t M2 = 2 * (size_t)plan.M;
@@ -1107,6 +1162,7 @@ int make_tiled(const msc_params* p, const double* wp, const double* nu, int prec
   int rc = make_plan(p, s->plan);
   if (rc) return rc;
   if ((rc = s->init(wp, nu))) return rc;
+  if ((rc = scratch_on_current_device(&s->device, &s->scratch_dev))) return rc;
   return s->sync();
 }
 }  // namespace
@@ -1127,6 +1183,7 @@ int msc_solver_create(const msc_params* p, const double* wp, const double* nu, i
   if (rc) return rc;
   rc = s->init(wp, nu);
   if (rc) return rc;
+  if ((rc = scratch_on_current_device(&s->device, &s->scratch_dev))) return rc;
   rc = s->sync();
   if (rc) return rc;
   msc_solver* h = new msc_solver;
@@ -1224,6 +1281,14 @@ int msc_solver_get_field(msc_solver* s, int comp, double* out) {
   SOLVER_OK(s);
   if (!out) return fail(MSC_EINVAL, "NULL output");
   return s->impl->get_field(comp, out);
+}
+
+int msc_solver_get_field_noalloc(msc_solver* s, int comp, double* out, size_t n) {
+  SOLVER_OK(s);
+  const HostPlan& h = s->impl->plan;
+  const size_t N = (size_t)h.nx * h.ny * h.nz;
+  if (!out || n < N) return fail(MSC_EINVAL, "output holds %zu values, the grid has %zu", out ? n : 0, N);
+  return s->impl->get_field_noalloc(comp, out);
 }
 
 size_t msc_solver_ntff_count(const msc_solver* s) {

@@ -55,6 +55,14 @@ class Solver:
         _check(self.lib.msc_solver_get_field(self._h, comp, _abi.dptr(out)), self.lib)
         return out
 
+    def field_noalloc(self, comp: int, out: np.ndarray | None = None) -> np.ndarray:
+        """field() through the library's fixed device scratch
+        (msc_solver_get_field_noalloc: no device allocation)."""
+        if out is None:
+            out = np.empty(self.shape, dtype=np.float64)
+        _check(self.lib.msc_solver_get_field_noalloc(self._h, comp, _abi.dptr(out), out.size), self.lib)
+        return out
+
     def ntff(self) -> np.ndarray:
         n = self.lib.msc_solver_ntff_count(self._h)
         out = np.empty(n, dtype=np.float64)
