@@ -268,8 +268,9 @@ int msc_solver_get_timing(const msc_solver* s, msc_timing* out);
 
 void msc_solver_destroy(msc_solver* s);
 
-/* Streaming copy of `bytes` between two fresh device buffers, `iters` times;
- * *gbps = bytes read + written per second.  No counterpart in the
+/* Streaming copy of `bytes` between two fresh device buffers, `iters` times
+ * at each of several grid sizes; *gbps = bytes read + written per second of
+ * the fastest.  No counterpart in the
  * reference: it measures the device's attainable HBM bandwidth next to the
  * solver's roofline and serves as the known-size calibration of the HBM byte
  * counters. */

@@ -129,6 +129,11 @@ struct NtffArgs {
   T* sbuf;                  // [K][U] surface samples
   const double* sincos;     // [tsteps][M][2]
   const int* tdev;
+  // compensated FP32 sums (acc32 = 1): each 8-byte slot of acc holds the
+  // pair (sum, compensation) as two floats, the contraction runs on FP32
+  // matrix tiles with the FP32 copy of the phase table
+  int acc32;
+  const float* sincos32;    // [tsteps][M][2] when acc32
 };
 
 template <typename T>
@@ -157,7 +162,7 @@ template <typename T>
 hipError_t launch_ntff(const NtffArgs<T>* a, const uint2* blocks, int nblocks, hipStream_t s);
 template <typename T>
 hipError_t launch_ntff_contract(const NtffArgs<T>* a, long long U, int M, long long nwaves, hipStream_t s,
-                                int flush = 0);
+                                int flush = 0, int acc32 = 0);
 template <typename T>
 hipError_t launch_frames(const AuxArgs<T>* a, hipStream_t s);
 template <typename T>
@@ -169,6 +174,7 @@ hipError_t launch_catchup(const StepArgs<T>* a, int cur, T* ox, T* oy, T* oz, hi
 template <typename T>
 hipError_t launch_field_slab(const StepArgs<T>* a, int cur, int comp, long long c0, long long nc, double* dst,
                              hipStream_t s);
-hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+// Grid-stride 16-byte copy of `bytes` with `blocks` workgroups of 256 threads.
+hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, int blocks, hipStream_t s);
 
 }  // namespace msc

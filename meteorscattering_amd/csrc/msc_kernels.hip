@@ -1048,6 +1048,78 @@ __global__ void __launch_bounds__(256) ntff_contract_mfma(const NtffArgs<T>* __r
   }
 }
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Neumaier's compensated addition of x to the pair (s, c) held in `slot`.
+__device__ __forceinline__ void add_compensated(float2* slot, float x) {
+  const float2 e = *slot;
+  const float t = e.x + x;
+  const float c = e.y + (fabsf(e.x) >= fabsf(x) ? (e.x - t) + x : (x - t) + e.x);
+  *slot = make_float2(t, c);
+}
+
+// The contraction with FP32 inputs and FP32 accumulation (compensated
+// FP32 sums): FP32 samples and the FP32 phase table on FP32 16x16x4 matrix
+// tiles, summed over the block in FP32; each block sum is added to the
+// running sum of its entry with Neumaier's compensation.  Inputs as in
+// ntff_contract_mfma (lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]),
+// but the FP32 instruction's result layout differs from the FP64 one: lane l
+// holds D[4 (l / 16) + v][l % 16] in element v (checked on gfx950).  Every 2M
+// (from 2 rows up) takes this path in the compensated mode.
+template <typename T>
+__global__ void __launch_bounds__(256) ntff_contract_mfma32(const NtffArgs<T>* __restrict__ args, int flush) {
+  const ConstArgs<NtffArgs<T>>& a = as_const_space(args);
+  const int t = *a.tdev - 1;
+  if (t < 0 || (!flush && (t + 1) % a.K != 0 && t + 1 != a.T_steps)) return;
+  const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kLanes;
+  if (w >= a.wofs[24]) return;  // uniform per wavefront
+  int q = 0;
+  while (q < 23 && w >= a.wofs[q + 1]) ++q;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int col = lane & 15, kr = lane >> 4;
+  const long long np = a.uoff[q + 1] - a.uoff[q];
+  const long long p = (w - a.wofs[q]) * 16 + col;
+  const bool pv = p < np;
+  const long long U = a.uoff[24];
+  const int kn = t % a.K + 1, t0 = t + 1 - kn, M2 = 2 * a.M;
+  const T* __restrict__ sb = a.sbuf + a.uoff[q] + (pv ? p : 0);
+  const float* __restrict__ tab = a.sincos32;
+  float2* acc = reinterpret_cast<float2*>(a.acc + a.off[q] + (pv ? p : 0));
+  for (int f0 = 0; f0 < M2; f0 += 128) {
+    const int ntile = min(8, (M2 - f0 + 15) / 16);
+    v4f d[8];
+#pragma unroll
+    for (int tt = 0; tt < 8; ++tt) d[tt] = v4f{0, 0, 0, 0};
+    for (int k0 = 0; k0 < kn; k0 += 4) {
+      const int k = k0 + kr;
+      const float b = (pv && k < kn) ? (float)sb[(size_t)k * U] : 0.0f;
+      const float* row = tab + (size_t)(t0 + (k < kn ? k : 0)) * M2;
+#pragma unroll
+      for (int tt = 0; tt < 8; ++tt) {
+        if (tt < ntile) {
+          const int f = f0 + tt * 16 + col;
+          const float av = (k < kn && f < M2) ? row[f] : 0.0f;
+          d[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b, d[tt], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int tt = 0; tt < 8; ++tt) {
+      if (tt < ntile && pv) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int f = f0 + tt * 16 + 4 * kr + v;
+          if (f < M2) add_compensated(acc + (size_t)f * np, d[tt][v]);
+        }
+      }
+    }
+  }
+  if (flush && pv) {
+    T* __restrict__ sw = a.sbuf + a.uoff[q] + p;
+    for (int k = kr; k < kn; k += 4) sw[(size_t)k * U] = (T)0;
+  }
+}
+
 // Surface samples of the running DFT (meteor-fork.cpp:971-1023 for the
 // face-averaged H, :1417-1470 for E) into slot t % K of the sample buffer.
 template <typename T>
@@ -1274,9 +1346,9 @@ __global__ void __launch_bounds__(256) stream_copy(const float4* __restrict__ sr
     dst[q] = src[q];
 }
 
-hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+hipError_t launch_stream_copy(const void* src, void* dst, size_t bytes, int blocks, hipStream_t s) {
   const size_t n = bytes / 16;
-  hipLaunchKernelGGL(stream_copy, dim3(4096), dim3(256), 0, s, (const float4*)src, (float4*)dst, n);
+  hipLaunchKernelGGL(stream_copy, dim3(blocks), dim3(256), 0, s, (const float4*)src, (float4*)dst, n);
   return hipGetLastError();
 }
 
@@ -1366,9 +1438,11 @@ hipError_t launch_ntff(const NtffArgs<T>* a, const uint2* blocks, int nblocks, h
 
 template <typename T>
 hipError_t launch_ntff_contract(const NtffArgs<T>* a, long long U, int M, long long nwaves, hipStream_t s,
-                                int flush) {
+                                int flush, int acc32) {
   if (U <= 0) return hipSuccess;
-  if (2 * M >= 16)  // matrix cores: 4 wavefronts of 16 points per workgroup
+  if (acc32)  // compensated FP32 sums on FP32 matrix tiles, every M
+    hipLaunchKernelGGL((ntff_contract_mfma32<T>), dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, s, a, flush);
+  else if (2 * M >= 16)  // matrix cores: 4 wavefronts of 16 points per workgroup
     hipLaunchKernelGGL((ntff_contract_mfma<T>), dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, s, a, flush);
   else
     hipLaunchKernelGGL((ntff_contract<T>), dim3((unsigned)((U + 255) / 256)), dim3(256), 0, s, a, flush);
@@ -1420,7 +1494,7 @@ hipError_t launch_field_slab(const StepArgs<T>* a, int cur, int comp, long long 
                                      hipStream_t);                                                                 \
   template hipError_t launch_pair<T>(const StepArgs<T>*, const uint16_t*, int, const uint32_t*, int, hipStream_t); \
   template hipError_t launch_ntff<T>(const NtffArgs<T>*, const uint2*, int, hipStream_t);                          \
-  template hipError_t launch_ntff_contract<T>(const NtffArgs<T>*, long long, int, long long, hipStream_t, int);    \
+  template hipError_t launch_ntff_contract<T>(const NtffArgs<T>*, long long, int, long long, hipStream_t, int, int); \
   template hipError_t launch_frames<T>(const AuxArgs<T>*, hipStream_t);                                            \
   template hipError_t launch_probes<T>(const AuxArgs<T>*, int, hipStream_t);                                       \
   template hipError_t launch_catchup<T>(const StepArgs<T>*, int, T*, T*, T*, hipStream_t);                     \

@@ -549,7 +549,7 @@ struct Server : ServerBase {
         const Solver<T>& s = *sel[q]->s;
         const int t = sel[q]->t + h;
         if ((t + 1) % s.na.K == 0 || t + 1 == s.plan.T)
-          HIPCHK(launch_ntff_contract<T>(g.na + q, s.na.uoff[24], s.na.M, s.na.wofs[24], run));
+          HIPCHK(launch_ntff_contract<T>(g.na + q, s.na.uoff[24], s.na.M, s.na.wofs[24], run, 0, s.na.acc32));
       }
     }
     HIPCHK(hipEventRecord(iter_ev[slot], run));

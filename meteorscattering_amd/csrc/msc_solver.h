@@ -81,6 +81,9 @@ struct SolverBase {
   // one-step tile owns zlanes - 1 z cells, and the two-step region's z
   // bounds are multiples of that, so the z bands around it stay full tiles
   int zlanes = 32;
+  // NTFF sums as compensated FP32 pairs on FP32 matrix tiles (MSC_NTFF_ACC=32)
+  // instead of FP64 sums (the default, MSC_NTFF_ACC=64)
+  bool acc32 = false;
   bool tiling_from_env = true;      // MSC_TY / MSC_CHUNK may override ty / chunk
   int cur = 0;                      // kCurDI / kCurLKPoly / kCurLKClosed
   std::vector<uint32_t> tiles_host; // the kNumLists tile lists, one after another

@@ -624,6 +624,8 @@ size_t Solver<T>::carve(char* base) {
   take(acc, acc_n);
   take(na.sbuf, (size_t)na.K * (size_t)na.uoff[24]);
   take(na.sincos, (size_t)h.T * h.M * 2);
+  if (acc32) take(na.sincos32, (size_t)h.T * h.M * 2);
+  else na.sincos32 = nullptr;
   take(probes, (size_t)6 * h.T * std::max(1, h.p.nprobes));
   take(xa.px, MSC_MAX_PROBES);
   take(xa.py, MSC_MAX_PROBES);
@@ -660,6 +662,11 @@ int Solver<T>::init(const double* wp, const double* nu) {
     if (const char* s = std::getenv("MSC_PAIR")) pair = std::atoi(s) != 0;
     if (const char* s = std::getenv("MSC_CHUNK2")) chunk2 = std::max(1, std::atoi(s));
     if (const char* s = std::getenv("MSC_ZLANES")) zlanes = std::atoi(s);
+  }
+  if (const char* s = std::getenv("MSC_NTFF_ACC")) {
+    const int v = std::atoi(s);
+    if (v != 32 && v != 64) return fail(MSC_EINVAL, "MSC_NTFF_ACC must be 32 or 64 (got %s)", s);
+    acc32 = v == 32;
   }
   auto tyok = [](int v) { return v == 3 || v == 7 || v == 15; };
   if (!tyok(ty) || !tyok(ty_b)) return fail(MSC_EINVAL, "MSC_TY and MSC_TY_B must be 3, 7 or 15");
@@ -799,6 +806,11 @@ int Solver<T>::init(const double* wp, const double* nu) {
     else cur = kCurLKClosed;
   }
   HIPCHK(hipMemcpyAsync((void*)na.sincos, h.sincos.data(), h.sincos.size() * 8, hipMemcpyHostToDevice, stream));
+  std::vector<float> sincos32;
+  if (acc32) {
+    sincos32.assign(h.sincos.begin(), h.sincos.end());  // FP64 phases rounded once
+    HIPCHK(hipMemcpyAsync((void*)na.sincos32, sincos32.data(), sincos32.size() * 4, hipMemcpyHostToDevice, stream));
+  }
   HIPCHK(hipMemcpyAsync((void*)xa.px, h.p.probex, 4 * MSC_MAX_PROBES, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync((void*)xa.py, h.p.probey, 4 * MSC_MAX_PROBES, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync((void*)xa.pz, h.p.probez, 4 * MSC_MAX_PROBES, hipMemcpyHostToDevice, stream));
@@ -847,6 +859,7 @@ int Solver<T>::init(const double* wp, const double* nu) {
   for (int q = 0; q < 3; q++)
     for (int c = 0; c < 3; c++) { na.e[q][c] = sa.e[q][c]; na.h[q][c] = sa.h[q][c]; }
   na.acc = acc; na.tdev = tdev;
+  na.acc32 = acc32 ? 1 : 0;
 
   xa.nx = nx; xa.ny = ny; xa.nz = nz; xa.nprobes = h.p.nprobes; xa.tsteps = h.T;
   xa.cadence = h.cadence; xa.nframes = h.nframes;
@@ -900,7 +913,7 @@ int Solver<T>::step(int n) {
     if (cad) HIPCHK(launch_frames<T>(xa_dev, stream));
     HIPCHK(launch_probes<T>(xa_dev, 1, stream));
     if ((t_host + 1) % na.K == 0 || t_host + 1 == plan.T)
-      HIPCHK(launch_ntff_contract<T>(na_dev, na.uoff[24], na.M, na.wofs[24], stream));
+      HIPCHK(launch_ntff_contract<T>(na_dev, na.uoff[24], na.M, na.wofs[24], stream, 0, na.acc32));
     ++t_host;
     return MSC_OK;
   };
@@ -1041,6 +1054,12 @@ int Solver<T>::get_ntff(double* out, int steps) {
   HIPCHK(hipMemcpyAsync(a.data(), acc, acc_n * 8, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));
   source_dft(plan, steps, out, out + M2);
+  if (acc32)  // (sum, compensation) float pairs: the value is their sum
+    for (size_t e = 0; e < acc_n; e++) {
+      float f[2];
+      std::memcpy(f, &a[e], sizeof f);
+      a[e] = (double)f[0] + (double)f[1];
+    }
   double* o = out + 2 * M2;
   const size_t fb = (size_t)plan.ny * plan.nz, tb = (size_t)plan.nx * plan.nz, lr = (size_t)plan.nx * plan.ny;
   for (int q = 0; q < 24; q++) {
@@ -1056,7 +1075,7 @@ int Solver<T>::get_ntff(double* out, int steps) {
 template <typename T>
 int Solver<T>::flush_ntff() {
   if (t_host == 0 || t_host % na.K == 0 || t_host >= plan.T) return MSC_OK;
-  HIPCHK(launch_ntff_contract<T>(na_dev, na.uoff[24], na.M, na.wofs[24], stream, 1));
+  HIPCHK(launch_ntff_contract<T>(na_dev, na.uoff[24], na.M, na.wofs[24], stream, 1, na.acc32));
   return MSC_OK;
 }
 
@@ -1102,6 +1121,7 @@ size_t solve_footprint(const msc_params* p, int precision) {
   s->tiling_from_env = false;
   s->pair = true;
   s->zlanes = kServerZLanes;
+  if (const char* e = std::getenv("MSC_NTFF_ACC")) s->acc32 = std::atoi(e) == 32;
   if (make_plan(p, s->plan) != MSC_OK) return 0;
   return s->layout();
 }
@@ -1385,15 +1405,20 @@ int msc_stream_copy(size_t bytes, int iters, double* gbps) {
   } else if (hipStreamCreate(&st) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
     rc = fail(MSC_EHIP, "stream/event creation failed");
   } else {
+    // the best of several grid sizes (1 to 16 workgroups per CU on 256 CUs)
     hipError_t e = hipMemsetAsync(a, 0, bytes, st);
-    if (e == hipSuccess) e = launch_stream_copy(a, b, bytes, st);  // warm
-    if (e == hipSuccess) e = hipEventRecord(e0, st);
-    for (int q = 0; q < iters && e == hipSuccess; ++q) e = launch_stream_copy(a, b, bytes, st);
-    if (e == hipSuccess) e = hipEventRecord(e1, st);
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    double best = 0;
+    for (int blocks : {256, 512, 1024, 2048, 4096}) {
+      if (e == hipSuccess) e = launch_stream_copy(a, b, bytes, blocks, st);  // warm
+      if (e == hipSuccess) e = hipEventRecord(e0, st);
+      for (int q = 0; q < iters && e == hipSuccess; ++q) e = launch_stream_copy(a, b, bytes, blocks, st);
+      if (e == hipSuccess) e = hipEventRecord(e1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e == hipSuccess) best = std::max(best, 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9);
+    }
     if (e != hipSuccess) rc = fail(MSC_EHIP, "stream copy: %s", hipGetErrorString(e));
-    else *gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
+    else *gbps = best;
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);

@@ -2,23 +2,31 @@
 
     python -m meteorscattering_amd.precision [--steps 2000] [--json out.json]
 
-Runs the same solve through the FP32 path (fields, J and CPML in FP32;
-plasma coefficients and NTFF sums in FP64) and the all-FP64 path, both on
-the GPU, and reports:
-  - max relative error of each field kind (E, H, J) at the end, relative to
-    the FP64 field's peak;
+The same solve (30-80 MHz, collocated Lee-Kalluri current) runs through
+  - the all-FP64 GPU path (fields, J, CPML FP64; FP64 NTFF sums);
+  - the FP32 GPU path (fields, J and CPML FP32; plasma coefficients FP64;
+    FP64 NTFF sums on FP64 matrix tiles) - the default;
+  - the FP32 GPU path with compensated FP32 NTFF sums (MSC_NTFF_ACC=32:
+    FP32 samples and phase table on FP32 matrix tiles, FP32 accumulation
+    over each K-step block, Neumaier-compensated FP32 running sums);
+  - the FP64 GPU path with compensated FP32 NTFF sums (the accumulation's
+    own error, free of the FP32 fields').
+Each run is compared with a reference run: the all-FP64 GPU path here, or
+the FP64 oracle's run when the caller passes it (tests/test_gpu_precision.py
+does), reporting
+  - the max relative error of each field kind (E, H, J) at the end,
+    relative to the reference's end-time peak of that kind (late-time fields
+    are small after the pulse has left, which inflates these);
   - the NTFF payload's max relative error;
   - per frequency, the bistatic RCS error in dB over two principal-plane
-    cuts at 1 degree (phi cut theta = 90 deg, phi 0..180; theta cut phi = 0,
-    theta 0..180), over directions where the FP64 RCS is within 40 dB of its
-    cut maximum, and the monostatic (backscatter) error.
-The FP64 GPU path itself is pinned to the FP64 oracle at small sizes by
-tests/test_gpu_parity.py.
+    cuts at 1 degree, over directions within 40 dB of each cut's peak, and
+    the monostatic (backscatter, meteorNTF.m) error.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import os
 import sys
 import time
 
@@ -28,42 +36,82 @@ from . import configs, rcs
 from .solver import Solver
 
 
-def run(p, wp, nu, precision: int) -> dict:
-    t0 = time.perf_counter()
-    with Solver(p, wp, nu, precision=precision) as s:
-        s.step(p["tsteps"])
-        s.sync()
-        el = time.perf_counter() - t0
-        fields = [s.field(c) for c in range(9)]
-        ntff = s.ntff()
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu(p, wp, nu, precision: int, acc: int) -> dict:
+    old = os.environ.get("MSC_NTFF_ACC")
+    os.environ["MSC_NTFF_ACC"] = str(acc)
+    try:
+        t0 = time.perf_counter()
+        with Solver(p, wp, nu, precision=precision) as s:
+            s.step(p["tsteps"])
+            s.sync()
+            el = time.perf_counter() - t0
+            fields = [s.field(c) for c in range(9)]
+            ntff = s.ntff()
+    finally:
+        if old is None:
+            os.environ.pop("MSC_NTFF_ACC", None)
+        else:
+            os.environ["MSC_NTFF_ACC"] = old
     return {"fields": fields, "ntff": ntff, "seconds": el}
 
 
-def study(steps: int = 2000) -> dict:
+def compare(run: dict, ref: dict, e: dict) -> dict:
+    """Field and payload errors of `run` against `ref`; `e` its RCS errors
+    (rcs.db_errors)."""
+    kinds = {}
+    for name, comps in (("E", range(0, 3)), ("H", range(3, 6)), ("J", range(6, 9))):
+        peak = max(np.abs(ref["fields"][c]).max() for c in comps)
+        err = max(np.abs(run["fields"][c] - ref["fields"][c]).max() for c in comps)
+        kinds[name] = float(err / peak) if peak > 0 else 0.0
+    a, b = run["ntff"], ref["ntff"]
+    return {"field_max_rel_error": kinds,
+            "ntff_max_rel_error": float(np.abs(a - b).max() / np.abs(b).max()),
+            "bistatic_db_error_per_freq": [round(x, 6) for x in e["bistatic_db_per_freq"]],
+            "bistatic_db_error_max": round(e["bistatic_db_max"], 6),
+            "monostatic_db_error_per_freq": [round(x, 6) for x in e["monostatic_db_per_freq"]],
+            "monostatic_db_error_max": round(e["monostatic_db_max"], 6),
+            "seconds": round(run["seconds"], 2)}
+
+
+def workload(steps: int = 2000):
     p, wp, nu = configs.ntff_m64_96x96x192(tsteps=steps)
     # the reference's node-averaged Lee-Kalluri drive grows without bound
     # over 2000 steps; the study runs the collocated form
     p.update(j_collocated=1, numfiles=1, savefields=0)
-    r64 = run(p, wp, nu, 64)
-    r32 = run(p, wp, nu, 32)
-    out = {"grid": [p["nx"], p["ny"], p["nz"]], "steps": steps, "nntf": p["nntf"],
-           "fntf_mhz": [round(f / 1e6, 4) for f in p["fntf"]],
-           "seconds": {"fp64": round(r64["seconds"], 2), "fp32": round(r32["seconds"], 2)}}
-    kinds = {}
-    for name, comps in (("E", range(0, 3)), ("H", range(3, 6)), ("J", range(6, 9))):
-        peak = max(np.abs(r64["fields"][c]).max() for c in comps)
-        err = max(np.abs(r32["fields"][c] - r64["fields"][c]).max() for c in comps)
-        kinds[name] = float(err / peak) if peak > 0 else 0.0
-    out["field_max_rel_error"] = kinds
-    a, b = r32["ntff"], r64["ntff"]
-    out["ntff_max_rel_error"] = float(np.abs(a - b).max() / np.abs(b).max())
+    return p, wp, nu
+
+
+def study(steps: int = 2000, reference: dict | None = None, reference_name: str | None = None) -> dict:
+    """`reference`: {"fields": [9 arrays], "ntff": payload, "seconds": s} of
+    the same workload (workload(steps)); None measures against the all-FP64
+    GPU path."""
+    p, wp, nu = workload(steps)
+    runs = {
+        "gpu_fp64": run_gpu(p, wp, nu, 64, 64),
+        "gpu_fp32": run_gpu(p, wp, nu, 32, 64),
+        "gpu_fp32_compensated_fp32_sums": run_gpu(p, wp, nu, 32, 32),
+        "gpu_fp64_compensated_fp32_sums": run_gpu(p, wp, nu, 64, 32),
+    }
+    log("GPU runs done: " + ", ".join(f"{k} {v['seconds']:.1f} s" for k, v in runs.items()))
+    if reference is None:
+        reference, reference_name = runs["gpu_fp64"], "all-FP64 GPU path"
+    ref = reference
+    names = [n for n, r in runs.items() if r is not ref]
     t0 = time.perf_counter()
-    e = rcs.db_errors(a, b, p)
-    out["rcs_seconds"] = round(time.perf_counter() - t0, 1)
-    out["bistatic_db_error_per_freq"] = [round(x, 5) for x in e["bistatic_db_per_freq"]]
-    out["bistatic_db_error_max"] = round(e["bistatic_db_max"], 5)
-    out["monostatic_db_error_per_freq"] = [round(x, 6) for x in e["monostatic_db_per_freq"]]
-    out["monostatic_dbsm_fp64"] = [round(x, 3) for x in e["monostatic_dbsm_reference"]]
+    errs = rcs.db_errors([runs[n]["ntff"] for n in names], ref["ntff"], p)
+    out = {"grid": [p["nx"], p["ny"], p["nz"]], "steps": steps, "nntf": p["nntf"],
+           "fntf_mhz": [round(f / 1e6, 4) for f in p["fntf"]], "reference": reference_name,
+           "reference_seconds": round(ref["seconds"], 1), "rcs_seconds": round(time.perf_counter() - t0, 1),
+           "monostatic_dbsm_reference": [round(x, 3) for x in errs[0]["monostatic_dbsm_reference"]]}
+    for name, e in zip(names, errs):
+        out[name] = compare(runs[name], ref, e)
+        log(f"{name}: {json.dumps({k: v for k, v in out[name].items() if 'per_freq' not in k})}")
+    a, b = runs["gpu_fp32_compensated_fp32_sums"]["ntff"], runs["gpu_fp32"]["ntff"]
+    out["compensated_fp32_vs_fp64_sums_ntff_rel"] = float(np.abs(a - b).max() / np.abs(b).max())
     return out
 
 

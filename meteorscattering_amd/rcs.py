@@ -103,47 +103,56 @@ def source_phasors(ntf: dict):
     return ey, ez
 
 
-def bistatic(ntf: dict, p: dict, theta, phi) -> dict:
+def bistatic(ntf, p: dict, theta, phi):
     """sigma (m^2) at directions (theta, phi) [rad] for every NTF frequency.
 
     Returns arrays shaped [n_directions, M]: 'sig' (total, Taflove 8.37),
     'e_theta2' = |L_phi + eta0 N_theta|^2, 'e_phi2' = |L_theta - eta0 N_phi|^2.
+    `ntf` may also be a list of payloads of one grid and frequency set (e.g.
+    the same solve in several precisions): they share the phase factors, and
+    a list of results comes back.
     """
+    many = isinstance(ntf, (list, tuple))
+    ntfs = list(ntf) if many else [ntf]
     theta = np.atleast_1d(np.asarray(theta, dtype=np.float64))
     phi = np.atleast_1d(np.asarray(phi, dtype=np.float64))
     theta, phi = np.broadcast_arrays(theta, phi)
-    f = np.asarray(ntf["fntf"], dtype=np.float64)
+    f = np.asarray(ntfs[0]["fntf"], dtype=np.float64)
     k = 2 * math.pi * f / VP
     rhat, thhat, phhat = _direction(theta, phi)
-    nd, M = theta.size, f.size
-    L = np.zeros((nd, M, 3), dtype=np.complex128)
-    N = np.zeros((nd, M, 3), dtype=np.complex128)
+    nd, M, P = theta.size, f.size, len(ntfs)
+    L = np.zeros((P, nd, M, 3), dtype=np.complex128)
+    N = np.zeros((P, nd, M, 3), dtype=np.complex128)
     # evenly spaced frequencies: exp(-i k_m p) = exp(-i k_0 p) exp(-i dk p)^m,
     # one complex product per sample and frequency instead of an exp
     even = M > 2 and np.allclose(np.diff(k), k[1] - k[0], rtol=1e-12, atol=0)
-    for kind, comp, z, w, pos in surface_samples(ntf, p):
+    for parts in zip(*[surface_samples(n, p) for n in ntfs]):
+        kind, comp, _, w, pos = parts[0]
         proj = rhat @ pos.T  # [nd, S]
-        zw = z * w[None, :]  # [M, S]
-        acc = (L if kind == "M" else N)[:, :, comp]
+        zw = np.stack([z for _, _, z, _, _ in parts], axis=-1) * w[None, :, None]  # [M, S, P]
+        acc = L if kind == "M" else N
         if even:
             ph = np.exp(-1j * k[0] * proj)
             step = np.exp(-1j * (k[1] - k[0]) * proj)
             for m in range(M):
-                acc[:, m] += ph @ zw[m]
+                acc[:, :, m, comp] += (ph @ zw[m]).T
                 ph *= step
         else:
             for m in range(M):
-                acc[:, m] += np.exp(-1j * k[m] * proj) @ zw[m]
-    Lth = np.einsum("dmc,dc->dm", L, thhat)
-    Lph = np.einsum("dmc,dc->dm", L, phhat)
-    Nth = np.einsum("dmc,dc->dm", N, thhat)
-    Nph = np.einsum("dmc,dc->dm", N, phhat)
-    ey, ez = source_phasors(ntf)
-    pinc = (np.abs(ez) ** 2 + np.abs(ey) ** 2) / 2 / ETA0
-    a = np.abs(Lph + ETA0 * Nth) ** 2
-    c = np.abs(Lth - ETA0 * Nph) ** 2
-    sig = k ** 2 / (8 * math.pi * ETA0) * (a + c) / np.abs(pinc)
-    return {"f": f, "k": k, "sig": sig, "e_theta2": a, "e_phi2": c, "theta": theta, "phi": phi}
+                acc[:, :, m, comp] += (np.exp(-1j * k[m] * proj) @ zw[m]).T
+    out = []
+    for q, n in enumerate(ntfs):
+        Lth = np.einsum("dmc,dc->dm", L[q], thhat)
+        Lph = np.einsum("dmc,dc->dm", L[q], phhat)
+        Nth = np.einsum("dmc,dc->dm", N[q], thhat)
+        Nph = np.einsum("dmc,dc->dm", N[q], phhat)
+        ey, ez = source_phasors(n)
+        pinc = (np.abs(ez) ** 2 + np.abs(ey) ** 2) / 2 / ETA0
+        a = np.abs(Lph + ETA0 * Nth) ** 2
+        c = np.abs(Lth - ETA0 * Nph) ** 2
+        sig = k ** 2 / (8 * math.pi * ETA0) * (a + c) / np.abs(pinc)
+        out.append({"f": f, "k": k, "sig": sig, "e_theta2": a, "e_phi2": c, "theta": theta, "phi": phi})
+    return out if many else out[0]
 
 
 def monostatic(ntf: dict, p: dict) -> dict:
@@ -200,31 +209,38 @@ def payload_to_ntf(payload: np.ndarray, p: dict) -> dict:
     return runio.parse_ntf(runio.encode_ntf(p["nntf"], b, p["fntf"], payload), p["nx"], p["ny"], p["nz"])
 
 
-def db_errors(payload: np.ndarray, reference: np.ndarray, p: dict, floor_db: float = 40.0,
-              step_deg: float = 1.0) -> dict:
+def db_errors(payload, reference: np.ndarray, p: dict, floor_db: float = 40.0, step_deg: float = 1.0):
     """RCS of `payload` against that of `reference` (same solve), in dB.
 
     Per frequency: the largest |dB difference| of the bistatic RCS over the
     two principal cuts, taken over directions where the reference RCS is
     within `floor_db` of its own maximum on that cut (deep nulls carry no
     meaningful dB figure), and the monostatic (meteorNTF.m backscatter)
-    difference."""
-    a, r = payload_to_ntf(payload, p), payload_to_ntf(reference, p)
+    difference.  `payload` may be a list of payloads of the same solve: a
+    list of results comes back, the phase factors computed once for all."""
+    many = isinstance(payload, (list, tuple))
+    pays = list(payload) if many else [payload]
+    ntfs = [payload_to_ntf(a, p) for a in pays] + [payload_to_ntf(reference, p)]
     th, ph, cuts = principal_cuts(step_deg)
-    da, dr = db(bistatic(a, p, th, ph)["sig"]), db(bistatic(r, p, th, ph)["sig"])
-    per_f = []
-    for m in range(da.shape[1]):
-        worst = 0.0
-        for cut in cuts:
-            ref = dr[cut, m]
-            keep = ref > ref.max() - floor_db
-            worst = max(worst, float(np.abs(da[cut, m] - ref)[keep].max()))
-        per_f.append(worst)
-    ma, mr = db(monostatic(a, p)["sig"]), db(monostatic(r, p)["sig"])
-    return {"bistatic_db_per_freq": per_f, "bistatic_db_max": max(per_f),
-            "monostatic_db_per_freq": [float(x) for x in np.abs(ma - mr)],
-            "monostatic_db_max": float(np.abs(ma - mr).max()),
-            "monostatic_dbsm_reference": [float(x) for x in mr]}
+    dbs = [db(r["sig"]) for r in bistatic(ntfs, p, th, ph)]
+    dr = dbs[-1]
+    mr = db(monostatic(ntfs[-1], p)["sig"])
+    out = []
+    for da, a in zip(dbs[:-1], ntfs[:-1]):
+        per_f = []
+        for m in range(da.shape[1]):
+            worst = 0.0
+            for cut in cuts:
+                ref = dr[cut, m]
+                keep = ref > ref.max() - floor_db
+                worst = max(worst, float(np.abs(da[cut, m] - ref)[keep].max()))
+            per_f.append(worst)
+        ma = db(monostatic(a, p)["sig"])
+        out.append({"bistatic_db_per_freq": per_f, "bistatic_db_max": max(per_f),
+                    "monostatic_db_per_freq": [float(x) for x in np.abs(ma - mr)],
+                    "monostatic_db_max": float(np.abs(ma - mr).max()),
+                    "monostatic_dbsm_reference": [float(x) for x in mr]})
+    return out if many else out[0]
 
 
 def mie_pec_backscatter(f, radius: float, terms: int = 100, c: float = 3e8) -> np.ndarray:

@@ -541,3 +541,30 @@ def test_ntff_read_mid_run_matches_oracle(msc, oracle_lib, nntf, precision, tol)
         assert e < tol, (t, e)
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("nntf", [2, 16, 100])
+@pytest.mark.parametrize("precision", [32, 64])
+def test_ntff_compensated_fp32_sums_match_oracle(msc, oracle_lib, monkeypatch, nntf, precision):
+    """MSC_NTFF_ACC=32: FP32 samples and the FP32 phase table on FP32 matrix
+    tiles (FP32 accumulation over each K-step block), block sums added to
+    compensated FP32 (sum, compensation) pairs.  Read mid-block and at the
+    end, the payload matches the oracle's FP64 running DFT to 2e-5 of its
+    largest entry (2, 16 and 100 frequencies: one partial row tile, one full
+    tile, two row groups)."""
+    Solver, _ = msc
+    monkeypatch.setenv("MSC_NTFF_ACC", "32")
+    p, wp, nu = cases.case("lk_ricker_ez")
+    p.update(nntf=nntf, fntf=list(np.linspace(30e6, 80e6, nntf)))
+    g = Solver(p, wp, nu, precision=precision)
+    o = oracle_lib.Oracle(p, wp, nu)
+    worst = 0.0
+    for t in (50, 101, p["tsteps"]):
+        g.step(t - g.t)
+        o.step(t - o.t)
+        e = relerr(g.ntff(), o.ntff())
+        worst = max(worst, e)
+        assert e < 2e-5, (t, e)
+    print(f"compensated FP32 sums, M={nntf} FP{precision}: NTFF rel err {worst:.2e}")
+    g.close()
+    o.close()

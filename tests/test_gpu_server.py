@@ -217,3 +217,17 @@ def test_size_class_tilings_equal_alone():
         for k in mix:
             np.testing.assert_array_equal(srv.ntff(ids[k]), alone[k][0], err_msg=k)
             np.testing.assert_array_equal(srv.probes(ids[k]), alone[k][1], err_msg=k)
+
+
+def test_compensated_fp32_ntff_sums_in_the_server_equal_alone(monkeypatch):
+    """With MSC_NTFF_ACC=32 the gang's contractions run the compensated FP32
+    kernel on each solve's own sums: payloads equal the single solves'."""
+    monkeypatch.setenv("MSC_NTFF_ACC", "32")
+    mix = {k: f() for k, f in TWO_STEP_MIX.items()}
+    alone = {k: _alone(*v) for k, v in mix.items()}
+    with Server(precision=32, budget_bytes=4 << 30, fill_tiles=0) as srv:
+        ids = {k: srv.submit(*v) for k, v in mix.items()}
+        srv.run_until_done()
+        for k in mix:
+            np.testing.assert_array_equal(srv.ntff(ids[k]), alone[k][0], err_msg=k)
+            np.testing.assert_array_equal(srv.probes(ids[k]), alone[k][1], err_msg=k)
