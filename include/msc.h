@@ -221,6 +221,14 @@ int msc_solver_get_ntff(msc_solver* s, double* out, size_t n);
  * flushes it.  Results are the same sums in another order (FP64). */
 int msc_solver_flush_ntff(msc_solver* s);
 
+/* Non-finite watchdog: *flag = 1 once a NaN or infinite value has reached
+ * the NTFF surface (every step's surface samples are checked on the
+ * device; a non-finite value anywhere gets there within the steps the
+ * stencil needs to carry it), else 0.  Waits for the work enqueued so far.
+ * The reference has no such check: its only health signal is the max|Ex|
+ * it logs at each output frame (meteor-fork.cpp:1616-1632). */
+int msc_solver_nonfinite(msc_solver* s, int* flag);
+
 /* Probe histories (Probe.dat body after the index header): 6 arrays
  * [tsteps][nprobes] (Ex,Ey,Ez,Hx,Hy,Hz), meteor-fork.cpp:1719-1729. */
 int msc_solver_get_probes(msc_solver* s, double* out, size_t n);
@@ -306,7 +314,9 @@ enum {
   MSC_REQ_RUNNING = 2,  /* stepping */
   MSC_REQ_DRAINING = 3, /* all steps launched, completing */
   MSC_REQ_DONE = 4,     /* results available */
-  MSC_REQ_FAILED = 5    /* rejected: msc_server_info sets msc_last_error */
+  MSC_REQ_FAILED = 5,   /* rejected, over the time limit, or its fields became
+                           non-finite: msc_server_info sets msc_last_error */
+  MSC_REQ_CANCELLED = 6 /* msc_server_cancel before its last step */
 };
 
 typedef struct msc_request_info {
@@ -318,7 +328,7 @@ typedef struct msc_request_info {
 } msc_request_info;
 
 typedef struct msc_server_stats {
-  int32_t queued, ready, running, draining;
+  int32_t queued, ready, running, draining;  /* draining: completing or being evicted */
   size_t arena_bytes, arena_used;
   long long iterations;   /* gang iterations launched */
   long long solve_steps;  /* solve time steps launched */
@@ -334,6 +344,20 @@ int msc_server_info(msc_server* s, int64_t id, msc_request_info* info);
 int msc_server_get_ntff(msc_server* s, int64_t id, double* out, size_t n);
 int msc_server_get_probes(msc_server* s, int64_t id, double* out, size_t n);
 int msc_server_release(msc_server* s, int64_t id);
+/* Failure handling, none of which the reference has (a PBS job runs until
+ * its walltime; nothing checks its fields): msc_server_cancel stops a
+ * request that is queued, ready or running (MSC_ESTATE once all its steps
+ * are launched); it ends MSC_REQ_CANCELLED at a later msc_server_advance,
+ * and its arena block returns once the launches already queued for it have
+ * completed.  Other solves' results do not change.  msc_server_set_time_limit
+ * ends a request that has been in the gang longer than `seconds` (0: no
+ * limit, the default) as MSC_REQ_FAILED.  A request whose fields become
+ * non-finite (the watchdog of msc_solver_nonfinite, read every 16 gang
+ * iterations and at completion) ends MSC_REQ_FAILED as well; a failed
+ * request that completed keeps its results for inspection.  Media with
+ * non-finite values are refused at submission. */
+int msc_server_cancel(msc_server* s, int64_t id);
+int msc_server_set_time_limit(msc_server* s, double seconds);
 int msc_server_get_stats(msc_server* s, msc_server_stats* out);
 double msc_server_clock(const msc_server* s);
 /* Scheduling knobs.  fill_tiles: each gang iteration steps the oldest

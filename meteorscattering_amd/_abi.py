@@ -116,8 +116,8 @@ def load_library(path: str | None = None) -> C.CDLL:
 
 
 # request states (msc.h)
-REQ_QUEUED, REQ_READY, REQ_RUNNING, REQ_DRAINING, REQ_DONE, REQ_FAILED = range(6)
-REQ_STATES = ("queued", "ready", "running", "draining", "done", "failed")
+REQ_QUEUED, REQ_READY, REQ_RUNNING, REQ_DRAINING, REQ_DONE, REQ_FAILED, REQ_CANCELLED = range(7)
+REQ_STATES = ("queued", "ready", "running", "draining", "done", "failed", "cancelled")
 
 
 class RequestInfo(C.Structure):
@@ -208,6 +208,7 @@ def _declare(lib: C.CDLL) -> None:
         "msc_solver_get_ntff": (C.c_int, [vp, dp, C.c_size_t]),
         "msc_solver_flush_ntff": (C.c_int, [vp]),
         "msc_solver_get_probes": (C.c_int, [vp, dp, C.c_size_t]),
+        "msc_solver_nonfinite": (C.c_int, [vp, P(C.c_int)]),
         "msc_solver_frame_count": (C.c_size_t, [vp]),
         "msc_solver_nframes": (C.c_int, [vp]),
         "msc_solver_get_frame": (C.c_int, [vp, C.c_int, dp, dp, dp]),
@@ -227,6 +228,8 @@ def _declare(lib: C.CDLL) -> None:
         "msc_server_get_ntff": (C.c_int, [vp, C.c_int64, dp, C.c_size_t]),
         "msc_server_get_probes": (C.c_int, [vp, C.c_int64, dp, C.c_size_t]),
         "msc_server_release": (C.c_int, [vp, C.c_int64]),
+        "msc_server_cancel": (C.c_int, [vp, C.c_int64]),
+        "msc_server_set_time_limit": (C.c_int, [vp, C.c_double]),
         "msc_server_get_stats": (C.c_int, [vp, P(ServerStats)]),
         "msc_server_clock": (C.c_double, [vp]),
         "msc_server_configure": (C.c_int, [vp, C.c_int, C.c_int]),

@@ -134,6 +134,7 @@ struct NtffArgs {
   // matrix tiles with the FP32 copy of the phase table
   int acc32;
   const float* sincos32;    // [tsteps][M][2] when acc32
+  int* health;              // set to 1 once a surface sample is NaN or infinite
 };
 
 template <typename T>

@@ -1186,6 +1186,9 @@ __global__ void __launch_bounds__(256) ntff_sample(const NtffArgs<T>* __restrict
       q3 = 0.5 * ((double)Hx[c] + (double)Hx[c - 1]);
       break;
   }
+  // watchdog: a non-finite field anywhere reaches the NTFF surface within
+  // the steps the stencil needs to carry it there
+  if (!(isfinite(q0) && isfinite(q1) && isfinite(q2) && isfinite(q3))) atomicOr(a.health, 1);
   T* slot = a.sbuf + (size_t)(t % a.K) * (size_t)a.uoff[24];
   slot[a.uoff[4 * face + 0] + pt] = (T)q0;
   slot[a.uoff[4 * face + 1] + pt] = (T)q1;

@@ -50,6 +50,8 @@ struct ServerBase {
   virtual double clock() const = 0;
   virtual int configure(int fill_tiles, int use_graphs) = 0;
   virtual int set_tilings(const msc_tiling* tilings, const int64_t* max_cells, int n) = 0;
+  virtual int cancel(int64_t id) = 0;
+  virtual int set_time_limit(double seconds) = 0;
 };
 
 template <typename T>
@@ -67,6 +69,10 @@ struct Server : ServerBase {
     double t_submit = 0, t_start = 0, t_done = 0;
     std::vector<double> ntff, probes;
     std::string err;
+    // evict at the next chance (cancelled, over the time limit or flagged
+    // non-finite), ending in end_state with `err` as the reason
+    bool cancel = false;
+    int end_state = MSC_REQ_CANCELLED;
   };
   // device copies of the gang's argument blocks and launch lists; two sets
   // so a rebuild never overwrites lists that queued launches still read
@@ -111,6 +117,18 @@ struct Server : ServerBase {
   int fill_tiles = 32768;   // ~40 rounds of 3 workgroups on each of the 256 CUs
   bool use_graphs = true;     // replay each gang half as a captured graph
   std::vector<Req*> draining;  // all steps launched, waiting for completion
+  std::vector<Req*> evicted;   // left the gang unfinished, waiting for its queued launches
+  double time_limit = 0;       // seconds a request may run once in the gang (0: no limit)
+  // Non-finite watchdog: every kHealthEvery iterations the flag words of
+  // the running solves (set by ntff_sample) are copied to pinned host slots
+  // on the run stream; once that copy has completed, flagged solves are
+  // evicted as failed.
+  static constexpr int kHealthEvery = 16;
+  static constexpr size_t kHealthSlots = 65536;
+  int* health_host = nullptr;
+  hipEvent_t health_ev = nullptr;
+  std::vector<int64_t> health_ids;
+  bool health_pending = false;
   std::map<int64_t, std::unique_ptr<Req>> reqs;
   int64_t next_id = 1;
   bool stop = false;
@@ -171,6 +189,8 @@ struct Server : ServerBase {
     HIPCHK(hipStreamCreateWithFlags(&run, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&setup, hipStreamNonBlocking));
     for (auto& e : iter_ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&health_ev, hipEventDisableTiming));
+    HIPCHK(hipHostMalloc((void**)&health_host, kHealthSlots * sizeof(int)));
     for (auto& g : gang) HIPCHK(hipEventCreateWithFlags(&g.last_use, hipEventDisableTiming));
     if (budget == 0) {
       size_t fr = 0, tot = 0;
@@ -209,6 +229,8 @@ struct Server : ServerBase {
     }
     for (auto& e : iter_ev)
       if (e) hipEventDestroy(e);
+    if (health_ev) hipEventDestroy(health_ev);
+    if (health_host) hipHostFree(health_host);
     if (run) hipStreamDestroy(run);
     if (setup) hipStreamDestroy(setup);
   }
@@ -221,6 +243,12 @@ struct Server : ServerBase {
       cv_work.wait(lk, [&] { return stop || !queue.empty(); });
       if (stop) return;
       Req* r = queue.front();
+      if (r->cancel) {  // cancelled while queued (also while waiting for space)
+        queue.pop_front();
+        r->s.reset();
+        end_unrun(r);
+        continue;
+      }
       worker_busy = true;
       lk.unlock();
       int rc = prepare(r);
@@ -232,7 +260,10 @@ struct Server : ServerBase {
         continue;
       }
       queue.pop_front();
-      if (rc == MSC_OK) {
+      if (rc == MSC_OK && r->cancel) {
+        r->s.reset();  // its block back (its set-up has completed)
+        end_unrun(r);
+      } else if (rc == MSC_OK) {
         r->state = MSC_REQ_READY;
         ready.push_back(r);
       } else {
@@ -243,6 +274,35 @@ struct Server : ServerBase {
       }
       cv_ready.notify_all();
     }
+  }
+
+  // A request that never joined the gang ends in its end state (mu held).
+  void end_unrun(Req* r) {
+    r->state = r->end_state;
+    r->t_done = now();
+    r->wp = r->nu = nullptr;
+  }
+
+  int cancel(int64_t id) override {
+    std::lock_guard<std::mutex> g(mu);
+    Req* r = find(id);
+    if (!r) return fail(MSC_EINVAL, "unknown request %lld", (long long)id);
+    if (r->state >= MSC_REQ_DRAINING)
+      return fail(MSC_ESTATE, "request %lld has no steps left to cancel", (long long)id);
+    if (!r->cancel) {
+      r->cancel = true;
+      r->end_state = MSC_REQ_CANCELLED;
+      r->err = "cancelled";
+    }
+    cv_work.notify_all();
+    return MSC_OK;
+  }
+
+  int set_time_limit(double seconds) override {
+    if (!(seconds >= 0)) return fail(MSC_EINVAL, "time limit must be >= 0 s (got %g)", seconds);
+    std::lock_guard<std::mutex> g(mu);
+    time_limit = seconds;
+    return MSC_OK;
   }
 
   int prepare(Req* r) {
@@ -289,6 +349,8 @@ struct Server : ServerBase {
     int rc = msc_params_validate(p);
     if (rc) return rc;
     if (p->savefields) return fail(MSC_EINVAL, "server solves produce NTFF and probe results only: set savefields = 0");
+    if (!wp || !nu) return fail(MSC_EINVAL, "NULL media");
+    if ((rc = check_media(p, wp, nu))) return rc;
     std::unique_ptr<Req> r(new Req());
     r->p = *p;
     r->wp = wp;
@@ -522,18 +584,106 @@ struct Server : ServerBase {
       if (rc) return rc;
       r->probes.resize((size_t)6 * r->s->plan.T * r->p.nprobes);
       if (!r->probes.empty() && (rc = r->s->get_probes(r->probes.data()))) return rc;
+      int bad = 0;
+      if ((rc = r->s->nonfinite(&bad))) return rc;
       r->s.reset();  // block back to the arena
       hipEventDestroy(r->done);
       r->done = nullptr;
       {
         std::lock_guard<std::mutex> g(mu);
-        r->state = MSC_REQ_DONE;
+        r->state = bad ? MSC_REQ_FAILED : MSC_REQ_DONE;
+        if (bad) r->err = "non-finite fields (NaN or infinity) reached the NTFF surface";
         r->t_done = now();
       }
       cv_work.notify_all();
       draining.erase(draining.begin() + q);
       if (ndone) ++*ndone;
     }
+    for (size_t q = 0; q < evicted.size();) {
+      Req* r = evicted[q];
+      hipError_t e = hipEventQuery(r->done);
+      if (e == hipErrorNotReady) {
+        ++q;
+        continue;
+      }
+      if (e != hipSuccess) return fail(MSC_EHIP, "solve %lld: %s", (long long)r->id, hipGetErrorString(e));
+      r->s.reset();  // its queued launches are complete: block back to the arena
+      hipEventDestroy(r->done);
+      r->done = nullptr;
+      {
+        std::lock_guard<std::mutex> g(mu);
+        r->state = r->end_state;
+        r->t_done = now();
+      }
+      cv_work.notify_all();
+      evicted.erase(evicted.begin() + q);
+    }
+    return MSC_OK;
+  }
+
+  // Running solves that were cancelled, ran past the time limit or were
+  // flagged non-finite leave the gang; their blocks return in retire() once
+  // the launches already queued for them have completed.
+  int evict() {
+    const double tnow = now();
+    for (size_t q = 0; q < active.size();) {
+      Req* r = active[q];
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!r->cancel && time_limit > 0 && tnow - r->t_start > time_limit) {
+          r->cancel = true;
+          r->end_state = MSC_REQ_FAILED;
+          char buf[128];
+          std::snprintf(buf, sizeof buf, "time limit of %g s exceeded at step %d", time_limit, r->t.load());
+          r->err = buf;
+        }
+        if (!r->cancel) {
+          ++q;
+          continue;
+        }
+      }
+      HIPCHK(hipEventCreateWithFlags(&r->done, hipEventDisableTiming));
+      HIPCHK(hipEventRecord(r->done, run));
+      evicted.push_back(r);
+      active.erase(active.begin() + q);
+      auto it = std::find(sel.begin(), sel.end(), r);
+      if (it != sel.end()) sel.erase(it);
+      dirty = true;
+    }
+    return MSC_OK;
+  }
+
+  // Watchdog: queue copies of the running solves' flag words, or read the
+  // copies queued earlier once they have completed.
+  int watch_health() {
+    if (health_pending) {
+      hipError_t e = hipEventQuery(health_ev);
+      if (e == hipErrorNotReady) return MSC_OK;
+      if (e != hipSuccess) return fail(MSC_EHIP, "watchdog copy: %s", hipGetErrorString(e));
+      std::lock_guard<std::mutex> lk(mu);
+      for (size_t q = 0; q < health_ids.size(); q++) {
+        Req* r = find(health_ids[q]);
+        if (!health_host[q] || !r || r->state != MSC_REQ_RUNNING || r->cancel) continue;
+        r->cancel = true;
+        r->end_state = MSC_REQ_FAILED;
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "non-finite fields (NaN or infinity) reached the NTFF surface by step %d",
+                      r->t.load());
+        r->err = buf;
+      }
+      health_pending = false;
+      return MSC_OK;
+    }
+    if (iters % kHealthEvery != 0 || active.empty()) return MSC_OK;
+    health_ids.clear();
+    for (Req* r : active) {
+      if (health_ids.size() == kHealthSlots) break;
+      HIPCHK(hipMemcpyAsync(&health_host[health_ids.size()], r->s->health_word(), sizeof(int),
+                            hipMemcpyDeviceToHost, run));
+      health_ids.push_back(r->id);
+    }
+    HIPCHK(hipEventRecord(health_ev, run));
+    health_pending = true;
     return MSC_OK;
   }
 
@@ -591,6 +741,12 @@ struct Server : ServerBase {
         std::unique_lock<std::mutex> lk(mu);
         if (!ready.empty()) {
           for (Req* r : ready) {
+            if (r->cancel) {  // cancelled before it joined: its set-up has completed
+              r->s.reset();
+              end_unrun(r);
+              cv_work.notify_all();
+              continue;
+            }
             r->state = MSC_REQ_RUNNING;
             r->t_start = now();
             active.push_back(r);
@@ -598,7 +754,7 @@ struct Server : ServerBase {
           ready.clear();
           dirty = true;
         }
-        if (active.empty() && draining.empty()) {
+        if (active.empty() && draining.empty() && evicted.empty()) {
           // idle, or only preparation pending: give the worker a moment and
           // return so the caller can submit or poll meanwhile
           if (!queue.empty()) cv_ready.wait_for(lk, std::chrono::milliseconds(2));
@@ -606,13 +762,15 @@ struct Server : ServerBase {
           continue;
         }
       }
+      if ((rc = evict())) return rc;
       if (active.empty()) {
-        // wait for the draining solves' work
+        // wait for the draining and evicted solves' work
         HIPCHK(hipStreamSynchronize(run));
         continue;
       }
       if (dirty && (rc = rebuild())) return rc;
       if ((rc = launch_iteration())) return rc;
+      if ((rc = watch_health())) return rc;
       ++it;
     }
     return retire(ndone);
@@ -634,7 +792,7 @@ struct Server : ServerBase {
     out->t_submit = r->t_submit;
     out->t_start = r->t_start;
     out->t_done = r->t_done;
-    if (r->state == MSC_REQ_FAILED) fail(MSC_EINVAL, "%s", r->err.c_str());
+    if (r->state == MSC_REQ_FAILED || r->state == MSC_REQ_CANCELLED) fail(MSC_EINVAL, "%s", r->err.c_str());
     return MSC_OK;
   }
 
@@ -662,7 +820,7 @@ struct Server : ServerBase {
     std::lock_guard<std::mutex> g(mu);
     Req* r = find(id);
     if (!r) return fail(MSC_EINVAL, "unknown request %lld", (long long)id);
-    if (r->state != MSC_REQ_DONE && r->state != MSC_REQ_FAILED)
+    if (r->state != MSC_REQ_DONE && r->state != MSC_REQ_FAILED && r->state != MSC_REQ_CANCELLED)
       return fail(MSC_ESTATE, "request %lld still in flight", (long long)id);
     reqs.erase(id);
     return MSC_OK;
@@ -673,7 +831,7 @@ struct Server : ServerBase {
     out->queued = (int)queue.size();
     out->ready = (int)ready.size();
     out->running = (int)active.size();
-    out->draining = (int)draining.size();
+    out->draining = (int)(draining.size() + evicted.size());
     out->arena_bytes = arena.capacity();
     out->arena_used = arena.used();
     out->iterations = iters;
@@ -771,6 +929,16 @@ int msc_server_get_stats(msc_server* s, msc_server_stats* out) {
 }
 
 double msc_server_clock(const msc_server* s) { return (s && s->impl) ? s->impl->clock() : -1.0; }
+
+int msc_server_cancel(msc_server* s, int64_t id) {
+  SERVER_OK(s);
+  return s->impl->cancel(id);
+}
+
+int msc_server_set_time_limit(msc_server* s, double seconds) {
+  SERVER_OK(s);
+  return s->impl->set_time_limit(seconds);
+}
 
 int msc_server_configure(msc_server* s, int fill_tiles, int use_graphs) {
   SERVER_OK(s);

@@ -36,6 +36,8 @@ struct HostPlan {
 };
 
 int make_plan(const msc_params* p, HostPlan& h);
+// Every media value (wp, nu maps as msc_read_run_dir returns them) finite.
+int check_media(const msc_params* p, const double* wp, const double* nu);
 // The source DFT sums Eydft[2M], Ezdft[2M] through `steps` time steps.
 void source_dft(const HostPlan& h, int steps, double* ey, double* ez);
 
@@ -143,6 +145,10 @@ struct SolverBase {
   // block or the run is complete)
   virtual int flush_ntff() = 0;
   virtual int get_probes(double* out) = 0;
+  // The non-finite watchdog flag (1: a NaN or infinite value reached the
+  // NTFF surface), read after the work queued so far; and its device word.
+  virtual int nonfinite(int* flag) = 0;
+  virtual int* health_word() = 0;
   virtual int get_frame(int f, double* e, double* h, double* emax) = 0;
   int sync();
   size_t frame_doubles() const {
@@ -170,6 +176,7 @@ struct Solver : SolverBase {
   double* hframes = nullptr;
   unsigned long long* emax = nullptr;
   int* tdev = nullptr;
+  int* health = nullptr;  // non-finite watchdog flag (ntff_sample)
 
   size_t layout() override;
   int init(const double* wp, const double* nu) override;
@@ -180,6 +187,8 @@ struct Solver : SolverBase {
   int flush_ntff() override;
   int get_probes(double* out) override;
   int get_frame(int f, double* e, double* h, double* emax) override;
+  int nonfinite(int* flag) override;
+  int* health_word() override { return health; }
 
  private:
   // CPML-aligned tile starts per axis, computed by layout()

@@ -179,6 +179,17 @@ static void plan_incident(HostPlan& h) {
   }
 }
 
+// Every media value finite (the reference reads wp.dat/nu.dat unchecked,
+// meteor-fork.cpp:340-347; a NaN or infinity there would poison every
+// field it reaches).
+int check_media(const msc_params* p, const double* wp, const double* nu) {
+  const size_t n = (size_t)p->nx * p->ny * (p->media3d ? (size_t)p->nz : 1);
+  for (size_t c = 0; c < n; c++)
+    if (!std::isfinite(wp[c]) || !std::isfinite(nu[c]))
+      return fail(MSC_EINVAL, "media value %zu is not finite (wp %g, nu %g)", c, wp[c], nu[c]);
+  return MSC_OK;
+}
+
 // Source DFT (meteor-fork.cpp:1407-1414) through `steps` steps: the hard
 // source values of the incident line's first cell (:1308-1335) times the
 // phase table, summed in step order as the reference does.
@@ -611,6 +622,7 @@ size_t Solver<T>::carve(char* base) {
   take(sa.ae, 2 * L);
   take(sa.inc, (size_t)h.T * 4 * nx);
   take(tdev, 1);
+  take(health, 1);
   const size_t fsz = frame_doubles();
   if (h.p.savefields) {
     take(sa.eslice, fsz);
@@ -860,6 +872,7 @@ int Solver<T>::init(const double* wp, const double* nu) {
     for (int c = 0; c < 3; c++) { na.e[q][c] = sa.e[q][c]; na.h[q][c] = sa.h[q][c]; }
   na.acc = acc; na.tdev = tdev;
   na.acc32 = acc32 ? 1 : 0;
+  na.health = health;
 
   xa.nx = nx; xa.ny = ny; xa.nz = nz; xa.nprobes = h.p.nprobes; xa.tsteps = h.T;
   xa.cadence = h.cadence; xa.nframes = h.nframes;
@@ -1080,6 +1093,15 @@ int Solver<T>::flush_ntff() {
 }
 
 template <typename T>
+int Solver<T>::nonfinite(int* flag) {
+  int v = 0;
+  HIPCHK(hipMemcpyAsync(&v, health, sizeof v, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  *flag = v != 0;
+  return MSC_OK;
+}
+
+template <typename T>
 int Solver<T>::get_probes(double* out) {
   const size_t n = (size_t)6 * plan.T * plan.p.nprobes;
   if (!n) return MSC_OK;
@@ -1193,6 +1215,8 @@ int msc_solver_create(const msc_params* p, const double* wp, const double* nu, i
   if (!p || !wp || !nu || !out) return fail(MSC_EINVAL, "NULL argument");
   *out = nullptr;
   if (precision != 32 && precision != 64) return fail(MSC_EINVAL, "precision must be 32 or 64 (got %d)", precision);
+  int rc0 = msc_params_validate(p);
+  if (rc0 || (rc0 = check_media(p, wp, nu))) return rc0;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MSC_EHIP, "no HIP device visible");
   std::unique_ptr<SolverBase> s;
@@ -1227,7 +1251,7 @@ int msc_solver_create_tiled(const msc_params* p, const double* wp, const double*
   *out = nullptr;
   if (precision != 32 && precision != 64) return fail(MSC_EINVAL, "precision must be 32 or 64 (got %d)", precision);
   int rc = check_tiling(tiling);
-  if (rc) return rc;
+  if (rc || (rc = msc_params_validate(p)) || (rc = check_media(p, wp, nu))) return rc;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MSC_EHIP, "no HIP device visible");
   std::unique_ptr<SolverBase> s;
@@ -1328,6 +1352,12 @@ int msc_solver_get_ntff(msc_solver* s, double* out, size_t n) {
 int msc_solver_flush_ntff(msc_solver* s) {
   SOLVER_OK(s);
   return s->impl->flush_ntff();
+}
+
+int msc_solver_nonfinite(msc_solver* s, int* flag) {
+  SOLVER_OK(s);
+  if (!flag) return fail(MSC_EINVAL, "NULL argument");
+  return s->impl->nonfinite(flag);
 }
 
 int msc_solver_get_probes(msc_solver* s, double* out, size_t n) {

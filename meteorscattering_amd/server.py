@@ -88,7 +88,7 @@ class Server:
         _check(self.lib.msc_server_info(self._h, rid, C.byref(inf)), self.lib)
         d = {f: getattr(inf, f) for f, _ in _abi.RequestInfo._fields_}
         d["state_name"] = _abi.REQ_STATES[inf.state]
-        if inf.state == _abi.REQ_FAILED:
+        if inf.state in (_abi.REQ_FAILED, _abi.REQ_CANCELLED):
             d["error"] = self.lib.msc_last_error().decode()
         return d
 
@@ -103,6 +103,14 @@ class Server:
         out = np.empty(max(n, 1))
         _check(self.lib.msc_server_get_probes(self._h, rid, _abi.dptr(out), n), self.lib)
         return out[:n]
+
+    def cancel(self, rid: int) -> None:
+        """Stop a request that has steps left (msc_server_cancel)."""
+        _check(self.lib.msc_server_cancel(self._h, rid), self.lib)
+
+    def set_time_limit(self, seconds: float) -> None:
+        """Fail requests that run longer than `seconds` once in the gang (0: no limit)."""
+        _check(self.lib.msc_server_set_time_limit(self._h, float(seconds)), self.lib)
 
     def release(self, rid: int) -> None:
         _check(self.lib.msc_server_release(self._h, rid), self.lib)

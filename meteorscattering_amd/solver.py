@@ -80,6 +80,12 @@ class Solver:
         _check(self.lib.msc_solver_get_probes(self._h, _abi.dptr(out), n), self.lib)
         return out[:n]
 
+    def nonfinite(self) -> bool:
+        """True once a NaN or infinite value has reached the NTFF surface."""
+        v = C.c_int()
+        _check(self.lib.msc_solver_nonfinite(self._h, C.byref(v)), self.lib)
+        return bool(v.value)
+
     @property
     def nframes(self) -> int:
         return self.lib.msc_solver_nframes(self._h)

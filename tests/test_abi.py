@@ -216,3 +216,25 @@ def test_solve_footprint_is_host_only_and_exact_in_form(lib):
     assert 1.6 < f64 / f32 < 2.0  # the FP64 NTFF sums and tables are common to both
     assert lib.msc_solve_footprint(C.byref(_abi.params_to_struct(dict(p, nx=10))), 32) == 0
     assert lib.msc_solve_footprint(C.byref(_abi.params_to_struct(p)), 16) == 0
+
+
+def test_non_finite_media_are_refused_before_any_device_work():
+    """msc_solver_create checks the parameters and every media value before
+    it looks for a device: NaN or infinity in wp.dat/nu.dat is refused."""
+    import ctypes as C
+    import math
+
+    from meteorscattering_amd import configs
+
+    lib = _abi.load_library()
+    p = configs.base_params((44, 44, 44), 0.5, 40)
+    wp = np.ones((44, 44)) * 1e7
+    nu = np.ones((44, 44)) * 1e5
+    for arr, val in ((wp, math.nan), (nu, math.inf)):
+        bad = arr.copy()
+        bad[5, 7] = val
+        a, b = (bad, nu) if arr is wp else (wp, bad)
+        s = _abi.params_to_struct(p)
+        h = C.c_void_p()
+        rc = lib.msc_solver_create(C.byref(s), _abi.dptr(a), _abi.dptr(b), 32, C.byref(h))
+        assert rc == -1 and b"not finite" in lib.msc_last_error()

@@ -231,3 +231,88 @@ def test_compensated_fp32_ntff_sums_in_the_server_equal_alone(monkeypatch):
         for k in mix:
             np.testing.assert_array_equal(srv.ntff(ids[k]), alone[k][0], err_msg=k)
             np.testing.assert_array_equal(srv.probes(ids[k]), alone[k][1], err_msg=k)
+
+
+def _unstable(**kw):
+    """Plasma frequency 1e4 times the trail's: omega_p dt ~ 1700, far past the
+    explicit coupling's stability limit, so the FP32 fields overflow to
+    infinity and NaN within a few dozen steps of the pulse reaching it."""
+    p, wp, nu = _case("lk_ricker_ez", **kw)
+    return p, wp * 1e4, nu
+
+
+def test_cancel_mid_flight_leaves_co_resident_results_bit_identical(alone):
+    names = ["lk_ricker_ez", "di_modgauss_ey", "lk_nopml_lhcp"]
+    with Server(precision=32, budget_bytes=4 << 30, fill_tiles=0) as srv:
+        ids = {n: srv.submit(*_case(n)) for n in names}
+        while srv.info(ids["di_modgauss_ey"])["t"] < 6:  # set-up is asynchronous
+            srv.advance(1)
+        srv.cancel(ids["di_modgauss_ey"])
+        srv.run_until_done()
+        inf = srv.info(ids["di_modgauss_ey"])
+        assert inf["state_name"] == "cancelled" and 0 < inf["t"] < inf["tsteps"], inf
+        for n in ("lk_ricker_ez", "lk_nopml_lhcp"):
+            assert srv.info(ids[n])["state_name"] == "done"
+            np.testing.assert_array_equal(srv.ntff(ids[n]), alone[n][0], err_msg=n)
+            np.testing.assert_array_equal(srv.probes(ids[n]), alone[n][1], err_msg=n)
+        assert srv.stats()["arena_used"] == 0
+        with pytest.raises(RuntimeError, match="no steps left"):
+            srv.cancel(ids["lk_ricker_ez"])
+        with pytest.raises(RuntimeError, match="not done"):
+            srv.ntff(ids["di_modgauss_ey"])
+        srv.release(ids["di_modgauss_ey"])
+
+
+def test_cancel_a_queued_request(alone):
+    """With room for one solve at a time, a request cancelled while it waits
+    never runs; the one ahead of it is unaffected."""
+    sizes = [footprint(_case(n)[0]) for n in ("lk_ricker_ez", "lk_nopml_lhcp")]
+    with Server(precision=32, budget_bytes=max(sizes) + min(sizes) // 2) as srv:
+        a = srv.submit(*_case("lk_ricker_ez"))
+        b = srv.submit(*_case("lk_nopml_lhcp"))
+        srv.cancel(b)
+        srv.run_until_done()
+        assert srv.info(b)["state_name"] == "cancelled" and srv.info(b)["t"] == 0
+        np.testing.assert_array_equal(srv.ntff(a), alone["lk_ricker_ez"][0])
+
+
+def test_time_limit_fails_requests_that_run_too_long(alone):
+    with Server(precision=32, budget_bytes=4 << 30) as srv:
+        srv.set_time_limit(1e-6)
+        rid = srv.submit(*_case("lk_ricker_ez"))
+        srv.run_until_done()
+        inf = srv.info(rid)
+        assert inf["state_name"] == "failed" and "time limit" in inf["error"], inf
+        assert inf["t"] < inf["tsteps"]
+        srv.set_time_limit(0)
+        rid = srv.submit(*_case("lk_ricker_ez"))
+        srv.run_until_done()
+        np.testing.assert_array_equal(srv.ntff(rid), alone["lk_ricker_ez"][0])
+        assert srv.stats()["arena_used"] == 0
+        with pytest.raises(RuntimeError):
+            srv.set_time_limit(-1)
+
+
+def test_non_finite_fields_fail_the_request_not_its_neighbours(alone):
+    """An unstable solve's fields overflow: the watchdog fails it (mid-flight
+    or at completion) while a co-resident solve completes with the bits it
+    has alone; a single solve reports the same flag."""
+    with Solver(*_unstable(), precision=32) as g:
+        g.step(120)
+        assert g.nonfinite()
+    with Solver(*_case("lk_ricker_ez"), precision=32) as g:
+        g.step(120)
+        assert not g.nonfinite()
+    with Server(precision=32, budget_bytes=4 << 30, fill_tiles=0) as srv:
+        bad = srv.submit(*_unstable())
+        good = srv.submit(*_case("lk_ricker_ez"))
+        srv.run_until_done()
+        inf = srv.info(bad)
+        assert inf["state_name"] == "failed" and "non-finite" in inf["error"], inf
+        assert srv.info(good)["state_name"] == "done"
+        np.testing.assert_array_equal(srv.ntff(good), alone["lk_ricker_ez"][0])
+        p, wp, nu = _case("lk_ricker_ez")
+        wp = wp.copy()
+        wp[3, 4] = np.nan
+        with pytest.raises(RuntimeError, match="not finite"):
+            srv.submit(p, wp, nu)
