@@ -65,6 +65,27 @@ def cpu_baseline(p, wp, nu, steps: int) -> dict:
                       f"{dt:.1f} s"}
 
 
+def rcs_check() -> dict:
+    """RCS error of the FP32 path, in dB: the 96x96x192 trail (collocated
+    current, 16 frequencies 30-80 MHz) over 1200 steps in FP32 and in the
+    all-FP64 path (which the tests pin to the FP64 oracle at 1e-10)."""
+    from meteorscattering_amd import configs, rcs
+    from meteorscattering_amd.solver import Solver
+
+    p, wp, nu = configs.trail_96x96x192(tsteps=1200)
+    p.update(j_collocated=1, nntf=16, fntf=list(np.linspace(30e6, 80e6, 16)), savefields=0, numfiles=1)
+    pays = {}
+    for prec in (32, 64):
+        with Solver(p, wp, nu, precision=prec) as g:
+            g.step(p["tsteps"])
+            pays[prec] = g.ntff()
+    e = rcs.db_errors(pays[32], pays[64], p)
+    return {"bistatic_db_max": round(e["bistatic_db_max"], 5), "monostatic_db_max": round(e["monostatic_db_max"], 5),
+            "workload": "96x96x192 trail, 16 frequencies 30-80 MHz, 1200 steps; bistatic over two 1-degree "
+                        "principal cuts within 40 dB of each cut's peak",
+            "against": "all-FP64 GPU path (pinned to the FP64 oracle at 1e-10 by the tests)"}
+
+
 def traffic_from_profile(kernel: str, workload: str):
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.json")
     if not os.path.exists(path):
@@ -149,6 +170,7 @@ def main():
     ap.add_argument("--workload", default="400x400x800")
     ap.add_argument("--precision", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rcs", action="store_true", help="skip the FP32 RCS error record")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--rate", type=float, default=2.0, help="serve: Poisson arrival rate (requests/s)")
     ap.add_argument("--requests", type=int, default=48, help="batch/serve: number of trails")
@@ -275,6 +297,10 @@ def main():
                      "measured_stream_copy_gbps": round(copy_gbps, 1),
                      "frac_of_copy": round(achieved / copy_gbps, 4) if achieved else None},
     }
+    if not args.no_rcs:
+        t2 = time.perf_counter()
+        out["rcs_db_err"] = rcs_check()
+        log(f"rcs check: {out['rcs_db_err']} ({time.perf_counter() - t2:.1f} s)")
     if not args.no_cpu:
         t2 = time.perf_counter()
         out["cpu_baseline"] = cpu_baseline(p, wp, nu, args.cpu_steps)

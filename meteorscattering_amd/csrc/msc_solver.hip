@@ -574,9 +574,11 @@ size_t Solver<T>::layout() {
   // sample-buffer depth: the accumulators' read-modify-write per step
   // (2 x 2M x 8 B per sample point, once per K steps) stays near 2.5% of the
   // field step's 80 B per cell, and at least 32 steps share one contraction
-  // launch (a latency-bound pass: 80 us per call at 400x400x800)
+  // launch (a latency-bound pass: 80 us per call at 400x400x800); at most
+  // 256 steps (96x96x192 with 64 frequencies: K = 213, a 314 MB FP32
+  // sample buffer against 377 MB of FP64 sums)
   const double kk = std::ceil(16.0 * h.M * (double)uo / ((double)nx * ny * nz));
-  na.K = (int)std::min(64.0, std::max(32.0, kk));
+  na.K = (int)std::min(256.0, std::max(32.0, kk));
   na.T_steps = h.T;
   return carve(nullptr);
 }

@@ -568,3 +568,31 @@ def test_ntff_compensated_fp32_sums_match_oracle(msc, oracle_lib, monkeypatch, n
     print(f"compensated FP32 sums, M={nntf} FP{precision}: NTFF rel err {worst:.2e}")
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("pair", ["0", "1"])
+@pytest.mark.parametrize("precision,tol", [(64, 1e-10), (32, 2e-5)])
+def test_headline_extensions_on_a_small_grid_match_oracle(msc, oracle_lib, monkeypatch, pair, precision, tol):
+    """Every extension the 400x400x800 workload uses, on a grid the oracle
+    runs in seconds: 12-cell CPML (slab offsets, TF/SF faces at L + 10, NTFF
+    planes at L + 5, the two-step region from L + 6), 3-D media of a trail
+    tilted 30 deg, nu(z) with a 9 m scale height and the collocated
+    Lee-Kalluri current; one-step and two-step tiles."""
+    from meteorscattering_amd import configs
+
+    Solver, _ = msc
+    monkeypatch.setenv("MSC_PAIR", pair)
+    p = configs.base_params((64, 56, 220), 0.5, 130, pmllen=12)
+    p.update(nu_scale_height=9.0, B0=configs.b_field(60.0, tilt_deg=30.0), j_collocated=1, media3d=1,
+             numfiles=3, nprobes=2, probex=[32, 21], probey=[28, 30], probez=[110, 40])
+    wp, nu = configs.trail_3d(p, 3e13, 1.5, 2e6, 30.0)
+    g, o, rep = run_pair(Solver, oracle_lib, p, wp, nu, precision, 120, 20, tol,
+                         f"extensions pair={pair} FP{precision}", stride=10)
+    assert not rep, "\n".join(rep[:12])
+    assert relerr(g.ntff(), o.ntff()) < tol
+    assert relerr(g.probes(), o.probes()) < tol
+    if pair == "1":
+        g.set_timing(True)
+        g.step(10)
+        g.sync()
+        assert g.timing()["pair_launches"] > 0
