@@ -1,6 +1,6 @@
 """Benchmark of the fused FDTD step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 400x400x800|96x96x192] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 400x400x800|96x96x192|96x96x192_m64] [--no-cpu]
     python bench.py --workload batch [--requests 48]          # 48-trail sweep, server vs serial
     python bench.py --workload serve --rate R [--requests N]  # Poisson stream, server vs serial
 
@@ -206,6 +206,9 @@ def main():
     elif args.workload == "96x96x192":
         p, wp, nu = configs.trail_96x96x192()
         wname = "96x96x192_fp32_lk_cpml10"
+    elif args.workload == "96x96x192_m64":
+        p, wp, nu = configs.ntff_m64_96x96x192()
+        wname = "96x96x192_fp32_lk_cpml10_m64"
     else:
         raise SystemExit(f"unknown workload {args.workload}")
     if args.precision == 64:
@@ -274,8 +277,10 @@ def main():
         "dtype": "f32" if args.precision == 32 else "f64", "data": "synthetic",
         "config": {"workload": f"fused Yee+Lee-Kalluri plasma step, {wname}", "nx": p["nx"], "ny": p["ny"],
                    "nz": p["nz"], "pmllen": p["pmllen"], "ntff_freqs": p["nntf"],
-                   "media": "Gaussian trail tilted 30 deg, nu(z) 7 km scale height, seed 0",
-                   "j_collocated": p.get("j_collocated", 0)},
+                   "media": ("Gaussian trail tilted 30 deg, nu(z) 7 km scale height, seed 0"
+                             if args.workload == "400x400x800" else "Gaussian trail along z, n0 1e12, r0 1.5 m"),
+                   "j_collocated": p.get("j_collocated", 0),
+                   "ntff_sums": "compensated FP32" if os.environ.get("MSC_NTFF_ACC") == "32" else "FP64"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": traffic,
