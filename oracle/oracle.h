@@ -1,7 +1,7 @@
 /*
  * oracle.h - FP64 CPU restatement of the reference meteor FDTD solver.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * TEST INFRASTRUCTURE ONLY.  This library checks the HIP path's results:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it, and never as the thing measured or shipped.
  *

@@ -1,7 +1,7 @@
 """ctypes wrapper of oracle/liboracle.so (the FP64 CPU restatement).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, always as the checker or the CPU baseline,
+bench.py's cpu_baseline leg, always to check the HIP path's results or as the CPU baseline,
 never as the product path.
 """
 from __future__ import annotations

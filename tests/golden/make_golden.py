@@ -1,6 +1,7 @@
 """Write golden fixtures from the reference solver built from its own source.
 
-Run in the container that has /root/reference (never on the GPU box):
+Run where /root/reference exists (the fixtures it writes are data and travel
+with the tests; the reference itself does not):
 
     make -C oracle ref
     python tests/golden/make_golden.py

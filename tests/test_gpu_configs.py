@@ -1,4 +1,4 @@
-"""The BASELINE workloads, unchanged, on the HIP path against the FP64 oracle.
+"""The benchmark workloads, unchanged, on the HIP path against the FP64 oracle.
 
 Each workload comes from meteorscattering_amd.configs with its own grid,
 CPML thickness, media, current model and tiling rules; only the number of
@@ -92,30 +92,30 @@ def check_run(g, st, precision, label):
 
 # ---- 400x400x800 ----------------------------------------------------------
 
-CFG2_STEPS = 60
+STEPS_400 = 60
 
 
 @pytest.fixture(scope="module")
-def cfg2(oracle_lib):
+def trail_400(oracle_lib):
     p, wp, nu = configs.trail_400x400x800()
-    st = oracle_states(oracle_lib, p, wp, nu, [CFG2_STEPS])
+    st = oracle_states(oracle_lib, p, wp, nu, [STEPS_400])
     # the pulse and the trail's current have both arrived
     assert np.abs(st["fields"][-1][2]).max() > 1e-3 and np.abs(st["fields"][-1][8]).max() > 0
     return p, wp, nu, st
 
 
 @pytest.mark.parametrize("precision", [32, 64])
-def test_400x400x800_matches_oracle_with_two_step_region(cfg2, precision):
+def test_400x400x800_matches_oracle_with_two_step_region(trail_400, precision):
     """The headline workload as bench.py runs it (default tiling: the
     two-step region holds 4,836 fdtd_pair tiles) against the oracle."""
-    p, wp, nu, st = cfg2
+    p, wp, nu, st = trail_400
     with Solver(p, wp, nu, precision=precision) as g:
         g.step(1)  # t = 0 writes a frame: one-step everywhere
         g.set_timing(True)
-        g.step(CFG2_STEPS - 1)
+        g.step(STEPS_400 - 1)
         g.sync()
         tm = g.timing()
-        assert tm["pair_launches"] == (CFG2_STEPS - 2) // 2 and tm["pair_cells"] > 0.7 * 400 * 400 * 800, tm
+        assert tm["pair_launches"] == (STEPS_400 - 2) // 2 and tm["pair_cells"] > 0.7 * 400 * 400 * 800, tm
         assert tm["pair_j_cells"] > 0, tm
         check_run(g, st, precision, "400x400x800")
 
@@ -123,14 +123,14 @@ def test_400x400x800_matches_oracle_with_two_step_region(cfg2, precision):
 # ---- 96x96x192 -------------------------------------------------------------
 
 @pytest.fixture(scope="module")
-def cfg1(oracle_lib):
+def trail_96(oracle_lib):
     p, wp, nu = configs.trail_96x96x192()
     return p, wp, nu, oracle_states(oracle_lib, p, wp, nu, [40, 80, 120, 160, 200])
 
 
 @pytest.mark.parametrize("precision,pair", [(32, None), (64, None), (32, "1"), (64, "1")])
-def test_96x96x192_reference_form_matches_oracle(cfg1, monkeypatch, precision, pair):
-    p, wp, nu, st = cfg1
+def test_96x96x192_reference_form_matches_oracle(trail_96, monkeypatch, precision, pair):
+    p, wp, nu, st = trail_96
     if pair is None:
         monkeypatch.delenv("MSC_PAIR", raising=False)
     else:
@@ -145,16 +145,16 @@ def test_96x96x192_reference_form_matches_oracle(cfg1, monkeypatch, precision, p
 
 
 @pytest.fixture(scope="module")
-def cfg3(oracle_lib):
+def trail_96_m64(oracle_lib):
     p, wp, nu = configs.ntff_m64_96x96x192()
     return p, wp, nu, oracle_states(oracle_lib, p, wp, nu, [65, 130])
 
 
 @pytest.mark.parametrize("precision", [32, 64])
-def test_96x96x192_64_frequencies_matches_oracle(cfg3, precision):
+def test_96x96x192_64_frequencies_matches_oracle(trail_96_m64, precision):
     """M = 64: the contraction runs on the matrix cores (FP64 MFMA), two
     64-step blocks plus a flushed block of 2 steps."""
-    p, wp, nu, st = cfg3
+    p, wp, nu, st = trail_96_m64
     with Solver(p, wp, nu, precision=precision) as g:
         check_run(g, st, precision, "96x96x192 M=64")
 

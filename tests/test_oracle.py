@@ -1,4 +1,4 @@
-"""Pins the FP64 oracle (oracle/oracle.c) before it is trusted as the checker.
+"""Pins the FP64 oracle (oracle/oracle.c) before the GPU tests trust its results.
 
 1. Golden fixtures written by the reference solver compiled from its own
    source (tests/golden/make_golden.py): NTF face arrays, probes, slices,
