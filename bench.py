@@ -124,14 +124,25 @@ def batch_or_serve(args, copy_gbps) -> dict:
         t1 = time.perf_counter()
         table = autotune.tune_classes(trails, [4 << 20, 16 << 20], precision=args.precision)
         log(f"autotuned {len(table)} size classes in {time.perf_counter() - t1:.1f} s: {table}")
+    gang = None
+    if args.gang_tune:
+        from meteorscattering_amd import autotune
+        t1 = time.perf_counter()
+        gtable, g0, g1 = autotune.tune_gang(trails, [4 << 20, 16 << 20], precision=args.precision, log=log)
+        gang = {"table": [{"max_cells": int(m), **t} for m, t in gtable], "default_batch_s": round(g0, 3),
+                "tuned_batch_s": round(g1, 3), "tuning_s": round(time.perf_counter() - t1, 1),
+                "steps_per_trail": 400}
+        log(f"gang-tuned in {gang['tuning_s']} s: {gtable} ({g0:.3f} -> {g1:.3f} s per tuning batch)")
     ser = workloads.run_serial(trails, precision=args.precision, tilings=table)
     log(f"serial: {ser['wall']:.2f} s ({len(trails) / ser['wall'] * 3600:.0f} solves/hr)")
+    stable = [(g["max_cells"], {k: g[k] for k in ("ty", "ty_b", "chunk", "chunk2")}) for g in gang["table"]] \
+        if gang else table
     if args.workload == "batch":
-        res = workloads.run_batch(trails, precision=args.precision, tilings=table)
+        res = workloads.run_batch(trails, precision=args.precision, tilings=stable)
         arrivals = np.zeros(len(trails))
     else:
         arrivals = workloads.poisson_arrivals(args.rate, len(trails), args.seed)
-        res = workloads.run_serve(trails, arrivals, precision=args.precision, tilings=table)
+        res = workloads.run_serve(trails, arrivals, precision=args.precision, tilings=stable)
     same = all(np.array_equal(a, b, equal_nan=True) for a, b in zip(ser["ntff"], res["ntff"]))
     nonfinite = sum(int(not np.all(np.isfinite(a))) for a in res["ntff"])
     lat = res["latency"] if "latency" in res else np.array([i["t_done"] - i["t_submit"] for i in res["info"]])
@@ -157,6 +168,7 @@ def batch_or_serve(args, copy_gbps) -> dict:
         "server_setup_s": round(res["setup"], 3) if "setup" in res else None,
         "speedup_vs_serial": round(sph / sph_serial, 3),
         "results_identical_to_serial": bool(same),
+        "gang_tuning": gang,
         "solves_with_nonfinite_ntff": nonfinite,
         "measured_stream_copy_gbps": round(copy_gbps, 1),
     }
@@ -177,6 +189,9 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--autotune", action="store_true",
                     help="batch/serve: tune one tiling per size class (autotune.tune_classes) first")
+    ap.add_argument("--gang-tune", action="store_true",
+                    help="batch/serve: tune the server's size-class tilings inside the gang (autotune.tune_gang) "
+                         "first; the serial baseline keeps its own defaults")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the streaming-copy reference (counter calibration)")
     args = ap.parse_args()

@@ -2,7 +2,8 @@
 
     from meteorscattering_amd import autotune
     tiling, ms = autotune.tune(p, wp, nu)               # one solve
-    table = autotune.tune_classes(requests, bounds)     # server size classes
+    table = autotune.tune_classes(requests, bounds)     # server size classes, each on one solve
+    table = autotune.tune_gang(requests, bounds)        # server size classes, inside the gang
 
 The library times candidate tilings on the solve itself (a coordinate
 search from the defaults over chunk2, chunk, ty and ty_b; each candidate a
@@ -17,6 +18,7 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
+from .server import Server
 
 
 def default() -> dict:
@@ -60,3 +62,71 @@ def tune_classes(requests: list[tuple[dict, np.ndarray, np.ndarray]], bounds: li
         tiling, _ = tune(p, wp, nu, precision, steps)
         table.append((hi, tiling))
     return table
+
+
+def _class_of(cells: int, bounds: list[int]) -> int:
+    for c, b in enumerate(bounds):
+        if cells <= b:
+            return c
+    return len(bounds)
+
+
+def gang_time(srv: Server, requests, table, steps: int) -> float:
+    """Seconds on the server clock for every request's first `steps` steps,
+    all submitted at once to `srv` (which must hold no request) with size-class
+    table `table`."""
+    srv.set_tilings(table)
+    t0 = srv.clock
+    ids = [srv.submit(dict(p, tsteps=min(steps, p["tsteps"]), numfiles=1), wp, nu) for p, wp, nu in requests]
+    srv.run_until_done(4)
+    t1 = max(srv.info(i)["t_done"] for i in ids)
+    for i in ids:
+        if srv.info(i)["state_name"] != "done":
+            raise RuntimeError(f"request {i} during gang tuning: {srv.info(i)}")
+        srv.release(i)
+    return t1 - t0
+
+
+def tune_gang(requests: list[tuple[dict, np.ndarray, np.ndarray]], bounds: list[int], precision: int = 32,
+              steps: int = 400, chunk2s=(32, 48, 64), chunks=(8, 12, 16), tys=(7, 15),
+              budget_bytes: int = 0, log=None) -> tuple[list[tuple[int, dict]], float, float]:
+    """Tilings by size class tuned inside the gang: a coordinate search per
+    class (over chunk2, chunk, ty, ty_b, from the defaults), each candidate
+    table timed on one batch of every request's first `steps` steps in one
+    server (fill 0), keeping a candidate only when the batch is 0.5% faster.
+    Every candidate tile is at least as large as the default's (two-step
+    chunks of 32 planes or more, one-step chunks of 8 or more, 7 or 15
+    rows), so no gang holds more workgroups than the default tiling's.
+    Returns the table (for Server.set_tilings), the default table's batch
+    time and the tuned one's (seconds).  Tilings change speed only, never
+    results."""
+    if min(chunk2s) < 32 or min(chunks) < 8 or min(tys) < 7:
+        raise ValueError("gang tuning keeps tiles at least as large as the defaults (chunk2 >= 32, chunk >= 8, "
+                         "rows >= 7)")
+    cells = [p["nx"] * p["ny"] * p["nz"] for p, _, _ in requests]
+    edges = list(bounds) + [max(cells)]
+    present = sorted({_class_of(c, bounds) for c in cells})
+    cur = {c: default() for c in present}
+
+    def table_of(t):
+        return [(edges[c], dict(t[c])) for c in present]
+
+    with Server(precision=precision, budget_bytes=budget_bytes, fill_tiles=0) as srv:
+        gang_time(srv, requests, table_of(cur), min(steps, 40))  # warm the server and the kernels
+        base = best = gang_time(srv, requests, table_of(cur), steps)
+        if log:
+            log(f"gang tuning: defaults {best:.3f} s")
+        for c in present:
+            for field, values in (("chunk2", chunk2s), ("chunk", chunks), ("ty", tys), ("ty_b", tys)):
+                start = cur[c][field]
+                for v in values:
+                    if v == start:
+                        continue
+                    cand = {k: dict(t) for k, t in cur.items()}
+                    cand[c][field] = v
+                    sec = gang_time(srv, requests, table_of(cand), steps)
+                    if log:
+                        log(f"gang tuning: class <= {edges[c]} {field}={v}: {sec:.3f} s")
+                    if sec < best * 0.995:
+                        cur, best = cand, sec
+    return table_of(cur), base, best

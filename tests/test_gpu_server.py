@@ -316,3 +316,25 @@ def test_non_finite_fields_fail_the_request_not_its_neighbours(alone):
         wp[3, 4] = np.nan
         with pytest.raises(RuntimeError, match="not finite"):
             srv.submit(p, wp, nu)
+
+
+def test_gang_tuning_returns_a_table_that_keeps_the_bits(alone):
+    """autotune.tune_gang times candidate size-class tables inside one
+    server's gang (short batches of every request) and returns a table no
+    smaller than the defaults; solves under it equal their solo bits."""
+    from meteorscattering_amd import autotune
+
+    names = ["lk_ricker_ez", "di_modgauss_ey", "lk_nopml_lhcp"]
+    reqs = [_case(n) for n in names]
+    table, t0, t1 = autotune.tune_gang(reqs, [80_000], steps=24, chunk2s=(32, 48), chunks=(8, 16), tys=(7, 15),
+                                       budget_bytes=4 << 30)
+    assert table and all(t["chunk"] >= 8 and t["chunk2"] >= 32 and t["ty"] >= 7 for _, t in table)
+    assert 0 < t1 <= t0
+    with pytest.raises(ValueError):
+        autotune.tune_gang(reqs, [80_000], chunks=(4, 8))
+    with Server(precision=32, budget_bytes=4 << 30, fill_tiles=0) as srv:
+        srv.set_tilings(table)
+        ids = {n: srv.submit(*_case(n)) for n in names}
+        srv.run_until_done()
+        for n in names:
+            np.testing.assert_array_equal(srv.ntff(ids[n]), alone[n][0], err_msg=n)
