@@ -604,11 +604,20 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   const bool e1ok = h1ok && r >= 1 && lane >= 1;                // E^(t+1): needs the -1 H neighbours
   const bool h2ok = e1ok && r < R - 2 && lane < PL - 2;     // H^(t+3/2)
   const bool own = inside && r >= 2 && r < R - 2 && j < j1 && lane >= 2 && lane < PL - 2 && k < k1;
-  // owned cells whose step-t values the one-step launches or the probes read
-  bool rimjk = j < a.y2lo + 2 || j >= a.y2hi - 2 || k < a.z2lo + 2 || k >= a.z2hi - 2;
-  unsigned long long pmask = 0;  // probes whose (j, k) box holds this cell
+  // owned cells whose step-t values the one-step launches or the probes
+  // read: the two outer layers of the region and the 3x3x3 box of each probe
+  // (the x planes of up to two probes whose (j, k) box holds the cell are
+  // kept; a cell in the boxes of more writes back on every plane, which
+  // stores only exact step-t+1 values)
+  const bool rimjk = j < a.y2lo + 2 || j >= a.y2hi - 2 || k < a.z2lo + 2 || k >= a.z2hi - 2;
+  int pxa = -8, pxb = -8;
+  bool pall = false;
   for (int p = 0; p < a.nprobes; ++p)
-    if (abs(j - a.py[p]) <= 1 && abs(k - a.pz[p]) <= 1) pmask |= 1ull << p;
+    if (abs(j - a.py[p]) <= 1 && abs(k - a.pz[p]) <= 1) {
+      if (pxa == -8) pxa = a.px[p];
+      else if (pxb == -8) pxb = a.px[p];
+      else pall = true;
+    }
 
   const bool jin = j >= a.by1 && j < a.by2, kin = k >= a.bz1 && k < a.bz2;
   const T sHx0 = kin ? (T)((int)(j == a.by1 - 1) - (int)(j == a.by2 - 1)) : (T)0;
@@ -628,8 +637,11 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   const T* __restrict__ Jxo = a.j[s2o][0];  // J^(t-1)
   const T* __restrict__ Jyo = a.j[s2o][1];
   const T* __restrict__ Jzo = a.j[s2o][2];
-  const T* __restrict__ inc1 = a.inc + (size_t)t * 4 * nx;        // step t
-  const T* __restrict__ inc2 = a.inc + (size_t)(t + 1) * 4 * nx;  // step t+1
+  // incident terms (uniform: scalar loads, prefetched one plane ahead and
+  // waited for at the next barrier)
+  using CT = __attribute__((address_space(4))) const T;
+  CT* inc1 = (CT*)(a.inc + (size_t)t * 4 * nx);        // step t
+  CT* inc2 = (CT*)(a.inc + (size_t)(t + 1) * 4 * nx);  // step t+1
   const T c2h = a.c2h, c2e0 = a.c2e0, inv = a.inv_d;
   const T tfEzB1 = inc1[a.bx1], tfEzB2 = inc1[a.bx2 - 1];
   const T tfEyB1 = inc1[nx + a.bx1], tfEyB2 = inc1[nx + a.bx2 - 1];
@@ -690,12 +702,8 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
       mt0 = inc2[i]; mt1 = inc2[nx + i]; mt2 = inc2[2 * nx + i]; mt3 = inc2[3 * nx + i];
     }
     const bool rim = own && i >= ib && i < ie &&
-                     (rimjk || i < a.x2lo + 2 || i >= a.x2hi - 2 ||
-                      (pmask && [&] {
-                        for (int p = 0; p < a.nprobes; ++p)
-                          if (((pmask >> p) & 1) && abs(i - a.px[p]) <= 1) return true;
-                        return false;
-                      }()));
+                     (rimjk || i < a.x2lo + 2 || i >= a.x2hi - 2 || pall || abs(i - pxa) <= 1 ||
+                      abs(i - pxb) <= 1);
     lds_barrier();
 
     // ---- B1: H^(t+1/2) at plane i   (:797-806, :854-865, :913-924 and TF/SF)

@@ -219,6 +219,26 @@ def test_two_step_tiles_are_bit_identical(msc, monkeypatch, name, jcol, chunk2, 
     assert not bad, "\n".join(bad)
 
 
+@pytest.mark.parametrize("precision", [32, 64])
+def test_crowded_probes_give_the_same_bits(msc, monkeypatch, precision):
+    """Probes whose 3x3x3 boxes overlap inside the two-step region (three in
+    one (y, z) column, a fourth beside them) need step-t+1 write-back from
+    cells in up to four boxes; two-step mode gives the one-step path's
+    fields, probes and NTFF payload bit for bit."""
+    Solver, _ = msc
+    p, wp, nu = pair_case("lk_ricker_ez", 0, 60)
+    p.update(nprobes=4, probex=[24, 26, 28, 27], probey=[25, 25, 25, 26], probez=[100, 100, 100, 101])
+    outs = []
+    for pair in (0, 1):
+        monkeypatch.setenv("MSC_PAIR", str(pair))
+        with Solver(p, wp, nu, precision=precision) as g:
+            for c in (1, 2, 5, 13, 39):
+                g.step(c)
+            outs.append(snapshot(g))
+    bad = [i for i, (a, b) in enumerate(zip(outs[1], outs[0])) if not np.array_equal(a, b)]
+    assert not bad, bad
+
+
 @pytest.mark.parametrize("name", ["lk_ricker_ez", "lk_nopml_lhcp", "plate_ricker_45"])
 @pytest.mark.parametrize("pair,precision", [(1, 32), (0, 32), (1, 64), (0, 64)])
 def test_lane_widths_give_the_same_bits(msc, monkeypatch, name, pair, precision):
