@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "msc_kernels.h"
 
@@ -637,11 +638,16 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   const T* __restrict__ Jxo = a.j[s2o][0];  // J^(t-1)
   const T* __restrict__ Jyo = a.j[s2o][1];
   const T* __restrict__ Jzo = a.j[s2o][2];
-  // incident terms (uniform: scalar loads, prefetched one plane ahead and
-  // waited for at the next barrier)
-  using CT = __attribute__((address_space(4))) const T;
-  CT* inc1 = (CT*)(a.inc + (size_t)t * 4 * nx);        // step t
-  CT* inc2 = (CT*)(a.inc + (size_t)(t + 1) * 4 * nx);  // step t+1
+  // incident terms, prefetched one plane ahead.  FP64 and the closed-form
+  // current read them with scalar loads (they are uniform; waited for at the
+  // next barrier), which frees VGPRs there (FP64 Lee-Kalluri: 31, and its
+  // launch goes from 5.61 to 4.98 ms at 400x400x800; the FP32 closed form
+  // stops spilling); the other FP32 variants keep vector loads, 0.7% faster
+  // for them.
+  using IncT = std::conditional_t<sizeof(T) == 8 || CUR == kCurLKClosed, __attribute__((address_space(4))) const T,
+                                  const T>;
+  IncT* inc1 = (IncT*)(a.inc + (size_t)t * 4 * nx);        // step t
+  IncT* inc2 = (IncT*)(a.inc + (size_t)(t + 1) * 4 * nx);  // step t+1
   const T c2h = a.c2h, c2e0 = a.c2e0, inv = a.inv_d;
   const T tfEzB1 = inc1[a.bx1], tfEzB2 = inc1[a.bx2 - 1];
   const T tfEyB1 = inc1[nx + a.bx1], tfEyB2 = inc1[nx + a.bx2 - 1];
