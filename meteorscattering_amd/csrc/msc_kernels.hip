@@ -161,6 +161,16 @@ __device__ __forceinline__ void lds_barrier() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// Whether positions [lo, hi) include f - 1 or f for a TF/SF face f = f1 or
+// f2: the only positions where a lane's face signs can be nonzero.  A tile
+// whose rows and lanes hold none of them weights the per-plane incident
+// terms by zero on every lane, so it skips their loads and corrections (the
+// field is then unchanged, where fma(0, term, field) could turn a field of
+// exactly -0 into +0).
+__device__ __forceinline__ bool holds_face(int lo, int hi, int f1, int f2) {
+  return (f1 >= lo && f1 <= hi) || (f2 >= lo && f2 <= hi);
+}
+
 template <typename A>
 __device__ __forceinline__ bool in_sphere(const A& a, int i, int j, int k) {
   // meteor-fork.cpp:203-209, coordinates as :137-141 (one loop per axis)
@@ -215,6 +225,7 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   const T sEx2 = jin ? (T)((int)(k == a.bz1) - (int)(k == a.bz2)) : (T)0;
   const T sEx3 = kin ? (T)((int)(j == a.by2) - (int)(j == a.by1)) : (T)0;
   const T mJK = (jin && kin) ? (T)1 : (T)0;
+  const bool tfl = holds_face(j0 - 1, j0 - 1 + R, a.by1, a.by2) || holds_face(k0 - 1, k0 - 1 + ZL, a.bz1, a.bz2);
 
   const int t = *a.tdev;
   const int par = t & 1;  // psi_H ping-pong
@@ -283,7 +294,7 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
     }
     // (no J or media for plane istart: it is either the warm-up plane ib-1,
     // which only recomputes H, or plane 0, outside the current's domain)
-    nt0 = inc[istart]; nt1 = inc[nx + istart]; nt2 = inc[2 * nx + istart]; nt3 = inc[3 * nx + istart];
+    if (tfl) { nt0 = inc[istart]; nt1 = inc[nx + istart]; nt2 = inc[2 * nx + istart]; nt3 = inc[3 * nx + istart]; }
   }
   T exprev = 0, hyprev = 0, hzprev = 0;
   int buf = 0;
@@ -313,7 +324,7 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         nqx = ldo(Jxo, cn); nqy = ldo(Jyo, cn); nqz = ldo(Jzo, cn);
         nqw = ldo(a.wpe2, cn); nqn = ldo(a.nue, cn);
       }
-      nt0 = inc[i + 1]; nt1 = inc[nx + i + 1]; nt2 = inc[2 * nx + i + 1]; nt3 = inc[3 * nx + i + 1];
+      if (tfl) { nt0 = inc[i + 1]; nt1 = inc[nx + i + 1]; nt2 = inc[2 * nx + i + 1]; nt3 = inc[3 * nx + i + 1]; }
     }
     lds_barrier();
 
@@ -332,7 +343,7 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         const T dEz = (ezjp - ecz) * inv;
         const T dEy = (eykp - ecy) * inv;
         hx = hx - c2h * (dEz - dEy);
-        if (i >= a.bx1 && i < a.bx2) {  // TF/SF (:808-822)
+        if (tfl && i >= a.bx1 && i < a.bx2) {  // TF/SF (:808-822)
           hx = fma(sHx0, tf0, hx);
           hx = fma(sHx1, tf1, hx);
         }
@@ -453,7 +464,7 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         const T dHz = (hz - sH[2][r - 1][lane]) * inv;
         const T dHy = (hy - sH[1][r][lane - 1]) * inv;
         T ex = c1 * ecx + c2 * (dHz - dHy) - c2 * jx;
-        if (i >= a.bx1 && i < a.bx2 - 1) {
+        if (tfl && i >= a.bx1 && i < a.bx2 - 1) {
           ex = fma(sEx2, tf2, ex);
           ex = fma(sEx3, tf3, ex);
         }
@@ -626,6 +637,7 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   const T sEx2 = jin ? (T)((int)(k == a.bz1) - (int)(k == a.bz2)) : (T)0;
   const T sEx3 = kin ? (T)((int)(j == a.by2) - (int)(j == a.by1)) : (T)0;
   const T mJK = (jin && kin) ? (T)1 : (T)0;
+  const bool tfl = holds_face(j0 - 2, j0 - 2 + R, a.by1, a.by2) || holds_face(k0 - 2, k0 - 2 + PL, a.bz1, a.bz2);
 
   const int t = *a.tdev;
   const int s0i = t % 3, s1o = (t + 1) % 3, s2o = (t + 2) % 3;  // step t, t+1, t+2 field sets
@@ -638,7 +650,8 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   const T* __restrict__ Jxo = a.j[s2o][0];  // J^(t-1)
   const T* __restrict__ Jyo = a.j[s2o][1];
   const T* __restrict__ Jzo = a.j[s2o][2];
-  // incident terms, prefetched one plane ahead.  FP64 and the closed-form
+  // incident terms, prefetched one plane ahead on the tiles that hold a TF/SF
+  // face (tfl; at 400x400x800 one tile in fifteen).  FP64 and the closed-form
   // current read them with scalar loads (they are uniform; waited for at the
   // next barrier), which frees VGPRs there (FP64 Lee-Kalluri: 31, and its
   // launch goes from 5.61 to 4.98 ms at 400x400x800; the FP32 closed form
@@ -676,7 +689,7 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
       px = ldo(Exo, c + sxb); py = ldo(Eyo, c + sxb); pz = ldo(Ezo, c + sxb);
     }
     s0[i0 & 1][0][r][lane] = ex; s0[i0 & 1][1][r][lane] = ey; s0[i0 & 1][2][r][lane] = ez;
-    nt0 = inc1[i0]; nt1 = inc1[nx + i0]; nt2 = inc1[2 * nx + i0]; nt3 = inc1[3 * nx + i0];
+    if (tfl) { nt0 = inc1[i0]; nt1 = inc1[nx + i0]; nt2 = inc1[2 * nx + i0]; nt3 = inc1[3 * nx + i0]; }
   }
   // carried along x: step-t H and E, step-t+1 E, J^t and media of plane i-1,
   // step-t+1 E x of plane i-2, step-t+1 H of plane i-2
@@ -704,8 +717,10 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         nqx = ldo(Jxo, cn); nqy = ldo(Jyo, cn); nqz = ldo(Jzo, cn);
         nqw = ldo(a.wpe2, cn); nqn = ldo(a.nue, cn);
       }
-      nt0 = inc1[i + 1]; nt1 = inc1[nx + i + 1]; nt2 = inc1[2 * nx + i + 1]; nt3 = inc1[3 * nx + i + 1];
-      mt0 = inc2[i]; mt1 = inc2[nx + i]; mt2 = inc2[2 * nx + i]; mt3 = inc2[3 * nx + i];
+      if (tfl) {
+        nt0 = inc1[i + 1]; nt1 = inc1[nx + i + 1]; nt2 = inc1[2 * nx + i + 1]; nt3 = inc1[3 * nx + i + 1];
+        mt0 = inc2[i]; mt1 = inc2[nx + i]; mt2 = inc2[2 * nx + i]; mt3 = inc2[3 * nx + i];
+      }
     }
     const bool rim = own && i >= ib && i < ie &&
                      (rimjk || i < a.x2lo + 2 || i >= a.x2hi - 2 || pall || abs(i - pxa) <= 1 ||
@@ -725,7 +740,7 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         const T dEz = (ezjp - ecz) * inv;
         const T dEy = (eykp - ecy) * inv;
         h1x = h1x - c2h * (dEz - dEy);
-        if (i >= a.bx1 && i < a.bx2) {
+        if (tfl && i >= a.bx1 && i < a.bx2) {
           h1x = fma(sHx0, tf0, h1x);
           h1x = fma(sHx1, tf1, h1x);
         }
@@ -781,7 +796,7 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         const T dHz = (h1z - s1h[2][r - 1][lane]) * inv;
         const T dHy = (h1y - s1h[1][r][lane - 1]) * inv;
         e1x = c1 * ecx + c2 * (dHz - dHy) - c2 * j1x;
-        if (i >= a.bx1 && i < a.bx2 - 1) {
+        if (tfl && i >= a.bx1 && i < a.bx2 - 1) {
           e1x = fma(sEx2, tf2, e1x);
           e1x = fma(sEx3, tf3, e1x);
         }
@@ -828,7 +843,7 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         const T dEz = (ezjp - e1pz) * inv;
         const T dEy = (eykp - e1py) * inv;
         h2x = h2x - c2h * (dEz - dEy);
-        if (ip >= a.bx1 && ip < a.bx2) {
+        if (tfl && ip >= a.bx1 && ip < a.bx2) {
           h2x = fma(sHx0, uf0, h2x);
           h2x = fma(sHx1, uf1, h2x);
         }
@@ -887,7 +902,7 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
         const T dHz = (h2z - s2h[2][r - 1][lane]) * inv;
         const T dHy = (h2y - s2h[1][r][lane - 1]) * inv;
         T ex = c1 * e1px + c2 * (dHz - dHy) - c2 * j2x;
-        if (ip >= a.bx1 && ip < a.bx2 - 1) {
+        if (tfl && ip >= a.bx1 && ip < a.bx2 - 1) {
           ex = fma(sEx2, uf2, ex);
           ex = fma(sEx3, uf3, ex);
         }
