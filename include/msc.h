@@ -337,6 +337,24 @@ typedef struct msc_server_stats {
 /* Device bytes one solve occupies in a server (0 for invalid parameters). */
 size_t msc_solve_footprint(const msc_params* p, int precision);
 
+/* Host-only check of the tile layout a server gang would launch for one
+ * solve (two-step mode, one-step tiles of `zlanes` z lanes, 32 or 64, the
+ * given tiling or the defaults when NULL).  Every tile descriptor is decoded
+ * as the kernels decode it: its indices must lie inside the cut tables, its
+ * owned range inside the grid and inside what its workgroup covers, its
+ * CPML variant must carry every slab axis its cells meet, and the one-step
+ * tiling, and the outer tiles with the two-step tiles, must each own every
+ * cell exactly once.  No device call.  No counterpart in the reference. */
+typedef struct msc_layout_report {
+  long long cells;           /* nx ny nz */
+  long long one_step_tiles;  /* workgroups of one one-step step over the grid */
+  long long outer_tiles;     /* one-step workgroups outside the two-step region */
+  long long pair_tiles;      /* fdtd_pair workgroups (0: no two-step region) */
+  long long region_cells;    /* cells of the two-step region */
+  long long max_list_tiles;  /* largest single one-step list */
+} msc_layout_report;
+int msc_layout_check(const msc_params* p, int precision, int zlanes, const msc_tiling* tiling, msc_layout_report* rep);
+
 int msc_server_create(int precision, size_t budget_bytes, msc_server** out);
 int msc_server_submit(msc_server* s, const msc_params* p, const double* wp, const double* nu, int64_t* id);
 int msc_server_advance(msc_server* s, int max_iterations, int* ndone);

@@ -183,6 +183,19 @@ class ServerStats(C.Structure):
     ]
 
 
+class LayoutReport(C.Structure):
+    """``msc_layout_report``."""
+
+    _fields_ = [
+        ("cells", C.c_longlong),
+        ("one_step_tiles", C.c_longlong),
+        ("outer_tiles", C.c_longlong),
+        ("pair_tiles", C.c_longlong),
+        ("region_cells", C.c_longlong),
+        ("max_list_tiles", C.c_longlong),
+    ]
+
+
 def _declare(lib: C.CDLL) -> None:
     P = C.POINTER
     vp = C.c_void_p
@@ -221,6 +234,7 @@ def _declare(lib: C.CDLL) -> None:
         "msc_run_dir": (C.c_int, [C.c_char_p, C.c_int]),
         "msc_stream_copy": (C.c_int, [C.c_size_t, C.c_int, dp]),
         "msc_solve_footprint": (C.c_size_t, [P(MscParams), C.c_int]),
+        "msc_layout_check": (C.c_int, [P(MscParams), C.c_int, C.c_int, P(Tiling), P(LayoutReport)]),
         "msc_server_create": (C.c_int, [C.c_int, C.c_size_t, P(vp)]),
         "msc_server_submit": (C.c_int, [vp, P(MscParams), dp, dp, P(C.c_int64)]),
         "msc_server_advance": (C.c_int, [vp, C.c_int, P(C.c_int)]),

@@ -41,8 +41,9 @@ int check_media(const msc_params* p, const double* wp, const double* nu);
 // The source DFT sums Eydft[2M], Ezdft[2M] through `steps` time steps.
 void source_dft(const HostPlan& h, int steps, double* ey, double* ez);
 
-// z lanes of the one-step tiles of the server's gang launches
-constexpr int kServerZLanes = 64;
+// z lanes of the one-step tiles of the server's gang launches: 32, so the
+// z bands around a solve's two-step region are 31 planes deep, not 63
+constexpr int kServerZLanes = 32;
 
 // Device bytes of one solve (its single block) with the server's settings,
 // without allocating.
@@ -128,6 +129,8 @@ struct SolverBase {
   // Plan the device block: builds the tile and NTFF lists and returns the
   // block's size (no device call).
   virtual size_t layout() = 0;
+  // Decode every tile of the layout and check it (msc_layout_check).
+  virtual int check_layout(msc_layout_report* rep) const = 0;
   // Allocate (arena or hipMalloc), clear and fill the block on `stream`.
   virtual int init(const double* wp, const double* nu) = 0;
   virtual int step(int n) = 0;
@@ -179,6 +182,7 @@ struct Solver : SolverBase {
   int* health = nullptr;  // non-finite watchdog flag (ntff_sample)
 
   size_t layout() override;
+  int check_layout(msc_layout_report* rep) const override;
   int init(const double* wp, const double* nu) override;
   int step(int n) override;
   int get_field(int comp, double* out) override;

@@ -424,6 +424,111 @@ int SolverBase::sync() {
   return MSC_OK;
 }
 
+// The layout's tiles, decoded as the kernels decode them (fdtd_step:
+// tile & 0xff -> zstart, bits 8-19 -> ystart (ystart_b for the CPML
+// variants), bits 20-30 -> xstart; a workgroup owns at most its rows and
+// zlanes - 1 lanes; fdtd_pair the same fields over xstart2 / ystart2 /
+// zstart2, at most kPairRows - 4 rows and kPairOwnZ lanes), checked for
+// bounds, for the CPML variant each tile needs, and for owning every cell
+// exactly once.
+template <typename T>
+int Solver<T>::check_layout(msc_layout_report* rep) const {
+  const HostPlan& h = plan;
+  const int nx = h.nx, ny = h.ny, nz = h.nz, L = h.L;
+  const bool pml = h.p.dopml != 0;
+  std::vector<uint8_t> own((size_t)nx * ny * nz);
+  auto zone = [&](int lo, int hi, int n) { return pml && (lo <= L || hi - 1 >= n - L - 1); };
+  // mark [x0,x1) x [y0,y1) x [z0,z1); false if a cell is owned twice
+  auto mark = [&](int x0, int x1, int y0, int y1, int z0, int z1) {
+    for (int x = x0; x < x1; x++)
+      for (int y = y0; y < y1; y++) {
+        uint8_t* row = own.data() + ((size_t)x * ny + y) * nz;
+        for (int z = z0; z < z1; z++) {
+          if (row[z]) return false;
+          row[z] = 1;
+        }
+      }
+    return true;
+  };
+  auto all_owned = [&](const char* what) {
+    for (size_t c = 0; c < own.size(); c++)
+      if (!own[c])
+        return fail(MSC_EINVAL, "layout: %s leaves cell (%zu, %zu, %zu) unowned", what, c / ((size_t)ny * nz),
+                    c / nz % ny, c % nz);
+    return MSC_OK;
+  };
+  // one one-step tile of list l; inregion: whether it may meet the region
+  auto one_step = [&](uint32_t w, int l, bool outer) {
+    const int tz = w & 0xff, ty_ = (w >> 8) & 0xfff, ch = (w >> 20) & 0x7ff;
+    const int pm = list_pm(l);
+    const std::vector<int>& yt = pm ? ysb_ : ys_;
+    const int rows = l == kListInterior ? ty : ty_b;
+    if (tz + 1 >= (int)zs_.size() || ty_ + 1 >= (int)yt.size() || ch + 1 >= (int)xs_.size())
+      return fail(MSC_EINVAL, "layout: list %d tile %08x indexes past its tables (%zu, %zu, %zu)", l, w, zs_.size(),
+                  yt.size(), xs_.size());
+    const int x0 = xs_[ch], x1 = xs_[ch + 1], y0 = yt[ty_], y1 = yt[ty_ + 1], z0 = zs_[tz], z1 = zs_[tz + 1];
+    if (!(0 <= x0 && x0 < x1 && x1 <= nx && 0 <= y0 && y0 < y1 && y1 <= ny && 0 <= z0 && z0 < z1 && z1 <= nz))
+      return fail(MSC_EINVAL, "layout: list %d tile [%d,%d)x[%d,%d)x[%d,%d) is empty or leaves the grid", l, x0, x1,
+                  y0, y1, z0, z1);
+    if (y1 - y0 > rows || z1 - z0 > zlanes - 1)
+      return fail(MSC_EINVAL, "layout: list %d tile owns %d rows x %d lanes, its workgroup %d x %d", l, y1 - y0,
+                  z1 - z0, rows, zlanes - 1);
+    const int need = (zone(x0, x1, nx) ? kPmlX : 0) | (zone(y0, y1, ny) ? kPmlY : 0) | (zone(z0, z1, nz) ? kPmlZ : 0);
+    if ((need & ~pm) != 0)
+      return fail(MSC_EINVAL, "layout: list %d tile [%d,%d)x[%d,%d)x[%d,%d) meets CPML axes %d, its variant %d", l, x0,
+                  x1, y0, y1, z0, z1, need, pm);
+    if (outer && x0 < x2hi && x1 > x2lo && y0 < y2hi && y1 > y2lo && z0 < z2hi && z1 > z2lo)
+      return fail(MSC_EINVAL, "layout: outer tile [%d,%d)x[%d,%d)x[%d,%d) meets the two-step region", x0, x1, y0, y1,
+                  z0, z1);
+    if (!mark(x0, x1, y0, y1, z0, z1))
+      return fail(MSC_EINVAL, "layout: list %d tile [%d,%d)x[%d,%d)x[%d,%d) owns a cell another tile owns", l, x0, x1,
+                  y0, y1, z0, z1);
+    return MSC_OK;
+  };
+  msc_layout_report r{};
+  r.cells = (long long)nx * ny * nz;
+  int rc;
+  for (int l = 0; l < kNumLists; l++) {
+    r.max_list_tiles = std::max<long long>(r.max_list_tiles, list_off[l + 1] - list_off[l]);
+    for (int q = list_off[l]; q < list_off[l + 1]; q++)
+      if ((rc = one_step(tiles_host[q], l, false))) return rc;
+  }
+  r.one_step_tiles = (long long)tiles_host.size();
+  if ((rc = all_owned("the one-step tiling"))) return rc;
+  if (!pair_host.empty()) {
+    if (x2lo < L + 6 || y2lo < L + 6 || z2lo < L + 6 || x2hi > nx - L - 6 || y2hi > ny - L - 6 || z2hi > nz - L - 6)
+      return fail(MSC_EINVAL, "layout: two-step region [%d,%d)x[%d,%d)x[%d,%d) is not 6 cells clear of the faces",
+                  x2lo, x2hi, y2lo, y2hi, z2lo, z2hi);
+    std::fill(own.begin(), own.end(), 0);
+    for (int l = 0; l < kNumLists; l++)
+      for (int q = outer_off[l]; q < outer_off[l + 1]; q++)
+        if ((rc = one_step(pair_host[q], l, true))) return rc;
+    for (size_t q = outer_off[kNumLists]; q < pair_host.size(); q++) {
+      const uint32_t w = pair_host[q];
+      const int tz = w & 0xff, ty_ = (w >> 8) & 0xfff, ch = (w >> 20) & 0x7ff;
+      if (tz + 1 >= (int)zs2_.size() || ty_ + 1 >= (int)ys2_.size() || ch + 1 >= (int)xs2_.size())
+        return fail(MSC_EINVAL, "layout: two-step tile %08x indexes past its tables", w);
+      const int x0 = xs2_[ch], x1 = xs2_[ch + 1], y0 = ys2_[ty_], y1 = ys2_[ty_ + 1], z0 = zs2_[tz], z1 = zs2_[tz + 1];
+      if (!(x2lo <= x0 && x0 < x1 && x1 <= x2hi && y2lo <= y0 && y0 < y1 && y1 <= y2hi && z2lo <= z0 && z0 < z1 &&
+            z1 <= z2hi))
+        return fail(MSC_EINVAL, "layout: two-step tile [%d,%d)x[%d,%d)x[%d,%d) is empty or leaves the region", x0, x1,
+                    y0, y1, z0, z1);
+      if (y1 - y0 > kPairRows - 4 || z1 - z0 > kPairOwnZ<T>)
+        return fail(MSC_EINVAL, "layout: two-step tile owns %d rows x %d lanes, its workgroup %d x %d", y1 - y0,
+                    z1 - z0, kPairRows - 4, kPairOwnZ<T>);
+      if (!mark(x0, x1, y0, y1, z0, z1))
+        return fail(MSC_EINVAL, "layout: two-step tile [%d,%d)x[%d,%d)x[%d,%d) owns a cell another tile owns", x0,
+                    x1, y0, y1, z0, z1);
+    }
+    if ((rc = all_owned("the outer tiles with the two-step tiles"))) return rc;
+    r.outer_tiles = outer_off[kNumLists];
+    r.pair_tiles = (long long)pair_host.size() - outer_off[kNumLists];
+    r.region_cells = pair_cells;
+  }
+  if (rep) *rep = r;
+  return MSC_OK;
+}
+
 size_t SolverBase::ntff_doubles() const {
   const size_t M2 = 2 * (size_t)plan.M;
   return 2 * M2 + 8 * M2 * ((size_t)plan.ny * plan.nz + (size_t)plan.nx * plan.nz + (size_t)plan.nx * plan.ny);
@@ -1141,7 +1246,7 @@ size_t solve_footprint(const msc_params* p, int precision) {
   if (precision == 32) s.reset(new Solver<float>());
   else if (precision == 64) s.reset(new Solver<double>());
   else return 0;
-  // the server's settings: its tiling, two-step mode, 64-lane one-step tiles
+  // the server's settings: its tiling, two-step mode, its one-step tile width
   s->tiling_from_env = false;
   s->pair = true;
   s->zlanes = kServerZLanes;
@@ -1236,6 +1341,35 @@ int msc_solver_create(const msc_params* p, const double* wp, const double* nu, i
   h->impl = std::move(s);
   *out = h;
   return MSC_OK;
+}
+
+int msc_layout_check(const msc_params* p, int precision, int zlanes, const msc_tiling* tiling, msc_layout_report* rep) {
+  if (!p) return fail(MSC_EINVAL, "NULL parameters");
+  if (precision != 32 && precision != 64) return fail(MSC_EINVAL, "precision must be 32 or 64 (got %d)", precision);
+  if (zlanes != 32 && zlanes != 64) return fail(MSC_EINVAL, "zlanes must be 32 or 64 (got %d)", zlanes);
+  int rc = msc_params_validate(p);
+  if (rc) return rc;
+  msc_tiling t;
+  msc_tiling_default(&t);
+  if (tiling) {
+    if ((rc = check_tiling(tiling))) return rc;
+    t = *tiling;
+  }
+  std::unique_ptr<SolverBase> s;
+  if (precision == 32) s.reset(new Solver<float>());
+  else s.reset(new Solver<double>());
+  // the server's settings: its tiling, two-step mode whatever the region's size
+  s->tiling_from_env = false;
+  s->pair = true;
+  s->zlanes = zlanes;
+  s->ty = t.ty;
+  s->ty_b = t.ty_b;
+  s->chunk = t.chunk;
+  s->chunk2 = t.chunk2;
+  if ((rc = make_plan(p, s->plan))) return rc;
+  s->layout();
+  if (s->tiles_host.empty()) return fail(MSC_EINVAL, "layout: no one-step tiles (a descriptor field would overflow)");
+  return s->check_layout(rep);
 }
 
 void msc_tiling_default(msc_tiling* t) {

@@ -238,3 +238,39 @@ def test_non_finite_media_are_refused_before_any_device_work():
         h = C.c_void_p()
         rc = lib.msc_solver_create(C.byref(s), _abi.dptr(a), _abi.dptr(b), 32, C.byref(h))
         assert rc == -1 and b"not finite" in lib.msc_last_error()
+
+
+def test_gang_layouts_own_every_cell_once(lib):
+    """msc_layout_check over every trail of the 48-trail sweep and of the
+    48-request stream as a server gang lays them out: one-step tiles of 32
+    and 64 lanes, FP32 and FP64 two-step tiles, the default tiling and a
+    short-chunk table (4-plane one-step chunks, 3-row CPML tiles, 16-plane
+    two-step chunks).  Each descriptor must decode inside its tables and its
+    workgroup, carry the CPML axes its cells meet, and each step's tiles must
+    own every cell once.  The gang sizes (workgroups of one half-iteration
+    with every trail resident) are printed for the record."""
+    from meteorscattering_amd import configs
+
+    short = _abi.Tiling(7, 3, 4, 16)
+    trails = [p for p, _, _ in configs.trail_sweep(count=48, maps=False)]
+    trails += [p for p, _, _ in configs.request_stream(count=48, maps=False)]
+    cases = [(32, 32, None), (32, 64, None), (64, 32, None), (32, 32, short), (32, 64, short)]
+    for prec, zl, til in cases:
+        one = outer = pair = 0
+        for n, p in enumerate(trails):
+            if prec == 64 and n % 4:
+                continue  # FP64 changes only the two-step tiles: every fourth trail
+            rep = _abi.LayoutReport()
+            rc = lib.msc_layout_check(C.byref(_abi.params_to_struct(p)), prec, zl,
+                                      C.byref(til) if til is not None else None, C.byref(rep))
+            assert rc == 0, (n, prec, zl, lib.msc_last_error().decode())
+            assert rep.cells == p["nx"] * p["ny"] * p["nz"]
+            one += rep.one_step_tiles
+            outer += rep.outer_tiles if rep.pair_tiles else rep.one_step_tiles
+            pair += rep.pair_tiles
+        print(f"fp{prec} {zl} lanes {'short' if til else 'default'}: one-step tiling {one} workgroups; "
+              f"two-step gang: {outer} one-step + {pair} fdtd_pair workgroups")
+    # what the check refuses
+    p = trails[0]
+    assert lib.msc_layout_check(C.byref(_abi.params_to_struct(p)), 32, 48, None, None) != 0
+    assert lib.msc_layout_check(C.byref(_abi.params_to_struct(p)), 16, 32, None, None) != 0

@@ -129,9 +129,9 @@ def test_scheduling_settings_do_not_change_results(alone, fill, graphs):
 
 
 def _two_step_case(name, tsteps, **kw):
-    """60x50x205 grid: in the server (64-lane one-step tiles) a two-step
-    region [16,44) x [16,34) x [63,189) with a probe in its rim; alone (32
-    lanes) the region is [16,44) x [16,34) x [31,186), and the bits match."""
+    """60x50x205 grid: in the server (32-lane one-step tiles) a two-step
+    region [16,44) x [16,34) x [31,186) with a probe in its rim; alone the
+    grid runs one-step, and the bits match."""
     p, _, _ = cases.case(name)
     p.update(nx=60, ny=50, nz=205, savefields=0, tsteps=tsteps, nprobes=2, probex=[30, 17], probey=[25, 33],
              probez=[100, 64], **kw)
@@ -148,13 +148,17 @@ TWO_STEP_MIX = {
 
 
 @pytest.mark.parametrize("fill,graphs", [(0, True), (1, False), (5000, True)])
-def test_two_step_gangs_equal_alone(fill, graphs):
+def test_two_step_gangs_equal_alone(fill, graphs, monkeypatch):
     """Solves with a two-step region advance two steps per gang iteration
     (the last one alone when their step count is odd) beside one-step
     solves; fill 1 steps one solve at a time, 5000 a few at a time.  The
-    NTFF payloads and probes equal the single-solve bits."""
+    NTFF payloads and probes equal the bits of single solves run one-step
+    (these regions are too small for a single solve's two-step rule) with
+    64-lane tiles."""
     mix = {k: f() for k, f in TWO_STEP_MIX.items()}
+    monkeypatch.setenv("MSC_ZLANES", "64")
     alone = {k: _alone(*v) for k, v in mix.items()}
+    monkeypatch.delenv("MSC_ZLANES")
     with Server(precision=32, budget_bytes=4 << 30, fill_tiles=fill, graphs=graphs) as srv:
         ids = {k: srv.submit(*v) for k, v in mix.items()}
         srv.run_until_done()
