@@ -1011,63 +1011,164 @@ __global__ void __launch_bounds__(256) field_slab(const StepArgs<T>* __restrict_
 }
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
 // The same contraction on the matrix cores, for 2M >= 16 frequency rows:
 // per sample array q it is the product acc[q] (2M x n) += Phi (2M x kn) S
 // (kn x n), with S[k][p] the buffered samples and Phi[f][k] =
 // sincos[t0+k][f] (f = 2m + c).  One wavefront per 16 points of one array
-// runs FP64 16x16x4 MFMA tiles over all 2M rows, 8 tiles (128 rows) at a
-// time, so each sample is read once per 128 rows instead of once per 8.
-// Lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16] and holds
-// D[4 v + l / 16][l % 16] in element v (checked on gfx950).  FP64 products
-// and sums, in a different order than ntff_contract.
+// runs FP64 16x16x4 MFMA tiles over the 2M rows, up to 8 tiles (128 rows)
+// per pass, so each sample is read once per 128 rows instead of once per 8.
+// The slots go in blocks of 16: the workgroup stages a block of Phi (16
+// slots x 128 rows, shared by its four wavefronts) in LDS while it runs the
+// previous block's MFMAs, and each wavefront loads the next block's samples
+// meanwhile, so the loads overlap the matrix work (one barrier per block).
+// Loads use clamped indices and are masked where used, so nothing waits on
+// a load before its value is needed.  Lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16] and holds D[4 v + l / 16][l % 16] in element v (checked
+// on gfx950).  FP64 products and sums, in a different order than
+// ntff_contract.
+constexpr int kPhiBlock = 16;  // slots per staged block of Phi
+
+// Neumaier's compensated sum of (s, c) and x: the new pair.
+__device__ __forceinline__ float2 compensated_sum(float2 e, float x) {
+  const float t = e.x + x;
+  const float c = e.y + (fabsf(e.x) >= fabsf(x) ? (e.x - t) + x : (x - t) + e.x);
+  return make_float2(t, c);
+}
+
+// One pass of the matrix-core contraction over rows [f0, f0 + nf), nf <=
+// 16 NT.  C32 = false: FP64 Phi, samples and 16x16x4 FP64 tiles, added to
+// the FP64 sums (lane l holds D[4 v + l / 16][l % 16] in element v).  C32 =
+// true: FP32 Phi, samples and 16x16x4 FP32 tiles summed over the slots in
+// FP32, then added to the compensated (sum, compensation) pairs (lane l
+// holds D[4 (l / 16) + v][l % 16] in element v).  Both layouts were checked
+// on gfx950.
+template <int NT, bool C32, typename T>
+__device__ __forceinline__ void mfma_pass(std::conditional_t<C32, float, double> (*stab)[kPhiBlock][128],
+                                          const std::conditional_t<C32, float, double>* __restrict__ tab,
+                                          const T* __restrict__ sb, double* acc, long long U, long long np, int f0,
+                                          int nf, int M2, int kn, int t0, bool wv, bool pv, int col, int kr) {
+  using P = std::conditional_t<C32, float, double>;
+  using V = std::conditional_t<C32, v4f, v4d>;
+  constexpr int KB = kPhiBlock, PER = KB * 128 / 256;
+  const int nblk = (kn + KB - 1) / KB;
+  P rphi[PER];
+  T bc[KB / 4], bn[KB / 4];
+  // this thread's share of a Phi block: elements e = threadIdx.x + 256 m of
+  // the 16 x 128 block (slot e / 128, row e % 128), clamped into the table
+  auto load_phi = [&](int k0) {
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      const int e = threadIdx.x + 256 * m;
+      const int k = min(k0 + (e >> 7), kn - 1), fe = min(e & 127, nf - 1);
+      rphi[m] = ldo(tab, (uint32_t)((t0 + k) * M2 + f0 + fe) * (uint32_t)sizeof(P));
+    }
+  };
+  auto store_phi = [&](int buf, int k0) {
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      const int e = threadIdx.x + 256 * m;
+      stab[buf][e >> 7][e & 127] = (k0 + (e >> 7) < kn && (e & 127) < nf) ? rphi[m] : (P)0;
+    }
+  };
+  // this lane's samples of a block: slots k0 + 4 s + kr (clamped; masked at use)
+  auto load_s = [&](int k0, T* bs) {
+#pragma unroll
+    for (int s4 = 0; s4 < KB / 4; ++s4)
+      bs[s4] = ldo(sb, (uint32_t)(min(k0 + 4 * s4 + kr, kn - 1) * U) * (uint32_t)sizeof(T));
+  };
+  load_phi(0);
+  load_s(0, bc);
+  store_phi(0, 0);
+  lds_barrier();
+  V d[NT];
+#pragma unroll
+  for (int tt = 0; tt < NT; ++tt) d[tt] = V{0, 0, 0, 0};
+  for (int bk = 0; bk < nblk; ++bk) {
+    const int buf = bk & 1, k0 = bk * KB;
+    const bool more = bk + 1 < nblk;  // uniform
+    if (more) {
+      load_phi(k0 + KB);
+      load_s(k0 + KB, bn);
+    }
+    if (wv) {
+#pragma unroll
+      for (int s4 = 0; s4 < KB / 4; ++s4) {
+        const P b = (pv && k0 + 4 * s4 + kr < kn) ? (P)bc[s4] : (P)0;
+#pragma unroll
+        for (int tt = 0; tt < NT; ++tt) {
+          if constexpr (C32)
+            d[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(stab[buf][4 * s4 + kr][tt * 16 + col], b, d[tt], 0, 0, 0);
+          else
+            d[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(stab[buf][4 * s4 + kr][tt * 16 + col], b, d[tt], 0, 0, 0);
+        }
+      }
+    }
+    if (more) {
+      store_phi(buf ^ 1, k0 + KB);
+#pragma unroll
+      for (int s4 = 0; s4 < KB / 4; ++s4) bc[s4] = bn[s4];
+    }
+    lds_barrier();
+  }
+  // the running sums, four tiles at a time: their rows read first, then
+  // written (as one read-modify-write per row, each load would wait for the
+  // store before it)
+  if (pv) {
+    constexpr int G = NT < 4 ? NT : 4;
+    auto row = [&](int tt, int v) { return C32 ? tt * 16 + 4 * kr + v : tt * 16 + 4 * v + kr; };
+#pragma unroll
+    for (int g0 = 0; g0 < NT; g0 += G) {
+      double old[G][4];
+#pragma unroll
+      for (int tt = 0; tt < G; ++tt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) old[tt][v] = acc[(size_t)(f0 + min(row(g0 + tt, v), nf - 1)) * np];
+#pragma unroll
+      for (int tt = 0; tt < G; ++tt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int f = row(g0 + tt, v);
+          if (f >= nf) continue;
+          if constexpr (C32) {
+            const float2 e = compensated_sum(__builtin_bit_cast(float2, old[tt][v]), d[g0 + tt][v]);
+            acc[(size_t)(f0 + f) * np] = __builtin_bit_cast(double, e);
+          } else {
+            acc[(size_t)(f0 + f) * np] = old[tt][v] + d[g0 + tt][v];
+          }
+        }
+    }
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) ntff_contract_mfma(const NtffArgs<T>* __restrict__ args, int flush) {
+  __shared__ double stab[2][kPhiBlock][128];  // Phi blocks, double-buffered
   const ConstArgs<NtffArgs<T>>& a = as_const_space(args);
   const int t = *a.tdev - 1;
-  if (t < 0 || (!flush && (t + 1) % a.K != 0 && t + 1 != a.T_steps)) return;
+  if (t < 0 || (!flush && (t + 1) % a.K != 0 && t + 1 != a.T_steps)) return;  // uniform
   const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kLanes;
-  if (w >= a.wofs[24]) return;  // uniform per wavefront
+  // wavefronts past the last one stage Phi and meet the barriers, nothing else
+  const bool wv = w < a.wofs[24];  // uniform per wavefront
   int q = 0;
-  while (q < 23 && w >= a.wofs[q + 1]) ++q;
+  if (wv)
+    while (q < 23 && w >= a.wofs[q + 1]) ++q;
   const int lane = threadIdx.x & (kLanes - 1);
   const int col = lane & 15, kr = lane >> 4;
   const long long np = a.uoff[q + 1] - a.uoff[q];
-  const long long p = (w - a.wofs[q]) * 16 + col;
-  const bool pv = p < np;
+  const long long p = wv ? (w - a.wofs[q]) * 16 + col : 0;
+  const bool pv = wv && p < np;
   const long long U = a.uoff[24];
   const int kn = t % a.K + 1, t0 = t + 1 - kn, M2 = 2 * a.M;
   const T* __restrict__ sb = a.sbuf + a.uoff[q] + (pv ? p : 0);
-  const double* __restrict__ tab = a.sincos;
   double* acc = a.acc + a.off[q] + (pv ? p : 0);
   for (int f0 = 0; f0 < M2; f0 += 128) {
-    const int ntile = min(8, (M2 - f0 + 15) / 16);
-    v4d d[8];
-#pragma unroll
-    for (int tt = 0; tt < 8; ++tt) d[tt] = v4d{0, 0, 0, 0};
-    for (int k0 = 0; k0 < kn; k0 += 4) {
-      const int k = k0 + kr;
-      const double b = (pv && k < kn) ? (double)sb[(size_t)k * U] : 0.0;
-      const double* row = tab + (size_t)(t0 + (k < kn ? k : 0)) * M2;
-#pragma unroll
-      for (int tt = 0; tt < 8; ++tt) {
-        if (tt < ntile) {
-          const int f = f0 + tt * 16 + col;
-          const double av = (k < kn && f < M2) ? row[f] : 0.0;
-          d[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b, d[tt], 0, 0, 0);
-        }
-      }
-    }
-#pragma unroll
-    for (int tt = 0; tt < 8; ++tt) {
-      if (tt < ntile && pv) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int f = f0 + tt * 16 + 4 * v + kr;
-          if (f < M2) acc[(size_t)f * np] += d[tt][v];
-        }
-      }
-    }
+    const int nf = min(128, M2 - f0);  // uniform
+    if (nf > 64) mfma_pass<8, false>(stab, a.sincos, sb, acc, U, np, f0, nf, M2, kn, t0, wv, pv, col, kr);
+    else if (nf > 32) mfma_pass<4, false>(stab, a.sincos, sb, acc, U, np, f0, nf, M2, kn, t0, wv, pv, col, kr);
+    else if (nf > 16) mfma_pass<2, false>(stab, a.sincos, sb, acc, U, np, f0, nf, M2, kn, t0, wv, pv, col, kr);
+    else mfma_pass<1, false>(stab, a.sincos, sb, acc, U, np, f0, nf, M2, kn, t0, wv, pv, col, kr);
   }
   // a flush consumes the block's samples so far: zeroed, they add nothing
   // when the block's own contraction sums all of its slots
@@ -1077,71 +1178,41 @@ __global__ void __launch_bounds__(256) ntff_contract_mfma(const NtffArgs<T>* __r
   }
 }
 
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-// Neumaier's compensated addition of x to the pair (s, c) held in `slot`.
-__device__ __forceinline__ void add_compensated(float2* slot, float x) {
-  const float2 e = *slot;
-  const float t = e.x + x;
-  const float c = e.y + (fabsf(e.x) >= fabsf(x) ? (e.x - t) + x : (x - t) + e.x);
-  *slot = make_float2(t, c);
-}
 
 // The contraction with FP32 inputs and FP32 accumulation (compensated
 // FP32 sums): FP32 samples and the FP32 phase table on FP32 16x16x4 matrix
 // tiles, summed over the block in FP32; each block sum is added to the
-// running sum of its entry with Neumaier's compensation.  Inputs as in
-// ntff_contract_mfma (lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]),
-// but the FP32 instruction's result layout differs from the FP64 one: lane l
-// holds D[4 (l / 16) + v][l % 16] in element v (checked on gfx950).  Every 2M
-// (from 2 rows up) takes this path in the compensated mode.
+// running sum of its entry with Neumaier's compensation, the pair (sum,
+// compensation) held as two floats in the entry's 8 bytes.  The structure
+// is ntff_contract_mfma's (mfma_pass).  Every 2M (from 2 rows up) takes this
+// path in the compensated mode.
 template <typename T>
 __global__ void __launch_bounds__(256) ntff_contract_mfma32(const NtffArgs<T>* __restrict__ args, int flush) {
+  __shared__ float stab[2][kPhiBlock][128];  // Phi blocks, double-buffered
   const ConstArgs<NtffArgs<T>>& a = as_const_space(args);
   const int t = *a.tdev - 1;
-  if (t < 0 || (!flush && (t + 1) % a.K != 0 && t + 1 != a.T_steps)) return;
+  if (t < 0 || (!flush && (t + 1) % a.K != 0 && t + 1 != a.T_steps)) return;  // uniform
   const long long w = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / kLanes;
-  if (w >= a.wofs[24]) return;  // uniform per wavefront
+  // wavefronts past the last one stage Phi and meet the barriers, nothing else
+  const bool wv = w < a.wofs[24];  // uniform per wavefront
   int q = 0;
-  while (q < 23 && w >= a.wofs[q + 1]) ++q;
+  if (wv)
+    while (q < 23 && w >= a.wofs[q + 1]) ++q;
   const int lane = threadIdx.x & (kLanes - 1);
   const int col = lane & 15, kr = lane >> 4;
   const long long np = a.uoff[q + 1] - a.uoff[q];
-  const long long p = (w - a.wofs[q]) * 16 + col;
-  const bool pv = p < np;
+  const long long p = wv ? (w - a.wofs[q]) * 16 + col : 0;
+  const bool pv = wv && p < np;
   const long long U = a.uoff[24];
   const int kn = t % a.K + 1, t0 = t + 1 - kn, M2 = 2 * a.M;
   const T* __restrict__ sb = a.sbuf + a.uoff[q] + (pv ? p : 0);
-  const float* __restrict__ tab = a.sincos32;
-  float2* acc = reinterpret_cast<float2*>(a.acc + a.off[q] + (pv ? p : 0));
+  double* acc = a.acc + a.off[q] + (pv ? p : 0);
   for (int f0 = 0; f0 < M2; f0 += 128) {
-    const int ntile = min(8, (M2 - f0 + 15) / 16);
-    v4f d[8];
-#pragma unroll
-    for (int tt = 0; tt < 8; ++tt) d[tt] = v4f{0, 0, 0, 0};
-    for (int k0 = 0; k0 < kn; k0 += 4) {
-      const int k = k0 + kr;
-      const float b = (pv && k < kn) ? (float)sb[(size_t)k * U] : 0.0f;
-      const float* row = tab + (size_t)(t0 + (k < kn ? k : 0)) * M2;
-#pragma unroll
-      for (int tt = 0; tt < 8; ++tt) {
-        if (tt < ntile) {
-          const int f = f0 + tt * 16 + col;
-          const float av = (k < kn && f < M2) ? row[f] : 0.0f;
-          d[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b, d[tt], 0, 0, 0);
-        }
-      }
-    }
-#pragma unroll
-    for (int tt = 0; tt < 8; ++tt) {
-      if (tt < ntile && pv) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-          const int f = f0 + tt * 16 + 4 * kr + v;
-          if (f < M2) add_compensated(acc + (size_t)f * np, d[tt][v]);
-        }
-      }
-    }
+    const int nf = min(128, M2 - f0);  // uniform
+    if (nf > 64) mfma_pass<8, true>(stab, a.sincos32, sb, acc, U, np, f0, nf, M2, kn, t0, wv, pv, col, kr);
+    else if (nf > 32) mfma_pass<4, true>(stab, a.sincos32, sb, acc, U, np, f0, nf, M2, kn, t0, wv, pv, col, kr);
+    else if (nf > 16) mfma_pass<2, true>(stab, a.sincos32, sb, acc, U, np, f0, nf, M2, kn, t0, wv, pv, col, kr);
+    else mfma_pass<1, true>(stab, a.sincos32, sb, acc, U, np, f0, nf, M2, kn, t0, wv, pv, col, kr);
   }
   if (flush && pv) {
     T* __restrict__ sw = a.sbuf + a.uoff[q] + p;

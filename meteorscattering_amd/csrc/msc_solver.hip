@@ -684,6 +684,9 @@ size_t Solver<T>::layout() {
   // sample buffer against 377 MB of FP64 sums)
   const double kk = std::ceil(16.0 * h.M * (double)uo / ((double)nx * ny * nz));
   na.K = (int)std::min(256.0, std::max(32.0, kk));
+  // the contraction addresses the sample buffer with 32-bit byte offsets
+  const double kmax = std::floor(4294967295.0 / ((double)uo * sizeof(T)));
+  na.K = (int)std::max(1.0, std::min((double)na.K, kmax));
   na.T_steps = h.T;
   return carve(nullptr);
 }
@@ -795,6 +798,9 @@ int Solver<T>::init(const double* wp, const double* nu) {
     return fail(MSC_EINVAL, "64-lane one-step tiles carry no slice capture: savefields needs 32 lanes");
   if ((double)N * sizeof(T) >= 4294967296.0)
     return fail(MSC_EINVAL, "grid of %zu cells exceeds the 4 GiB-per-array limit of the 32-bit offset kernel", N);
+  if ((double)plan.T * 2.0 * plan.M * sizeof(double) >= 4294967296.0)
+    return fail(MSC_EINVAL, "%d steps x %d frequencies: the sin/cos table exceeds the 4 GiB the contraction addresses",
+                plan.T, plan.M);
   const size_t bytes = layout();
   if ((int)zs_.size() - 1 > 255 || (int)ys_.size() - 1 > 4095 || (int)ysb_.size() - 1 > 4095 ||
       (int)xs_.size() - 1 > kMaxChunks)
