@@ -250,6 +250,9 @@ def main():
     el = time.perf_counter() - t1
     kms = s.kernel_ms
     tm = s.timing()
+    # fdtd_pair's memory traffic alone (same tiles, loads and stores, no
+    # arithmetic; the solve's state is unchanged), after the timed region
+    pattern_ms = s.pattern_probe(10) if tm["pair_launches"] > 0 else None
     s.close()
     ms_per_step = el * 1e3 / args.steps
     value = ncell * args.steps / el / 1e9
@@ -282,6 +285,20 @@ def main():
                "frac_of_copy": round(hbm_gbs / copy_gbps, 4),
                "basis": "HBM bytes the launch actually moves (rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, "
                         "calibrated on a copy of known size; profiles/) over this run's launch duration"}
+    pattern = None
+    if pattern_ms:
+        # the launch's minimum bytes: each cell's E, H (and J, media where the
+        # tile carries J) read once and written once
+        minb = noj * tm["pair_cells"] + jb * tm["pair_j_cells"]
+        pattern = {"ms_per_launch": round(pattern_ms, 4), "min_bytes_per_launch": minb,
+                   "achieved_min_bytes": round(minb / (pattern_ms * 1e-3) / 1e9, 1),
+                   "fdtd_pair_min_bytes": round(minb / (kernel_ms * 1e-3) / 1e9, 1),
+                   "fdtd_pair_frac_of_pattern": round(pattern_ms / kernel_ms, 4),
+                   "basis": "pair_pattern: fdtd_pair's tiles, loads (tile and 2-cell halo, one plane ahead) and "
+                            "owned stores with no arithmetic, timed after the timed region; bandwidth on the "
+                            "launch's minimum bytes (each value read and written once)"}
+        log(f"fdtd_pair memory pattern alone: {pattern_ms:.3f} ms/launch ({pattern['achieved_min_bytes']:.0f} GB/s "
+            f"on minimum bytes); fdtd_pair {kernel_ms:.3f} ms ({pattern['fdtd_pair_frac_of_pattern']:.2f} of it)")
     log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, GPU span {span:.3f} ms/step, "
         f"{value:.2f} Gcell/s; {kname} {kernel_ms:.3f} ms/launch, {achieved:.0f} GB/s algorithmic"
         + (f", {hbm['achieved']:.0f} GB/s of HBM traffic" if hbm else ""))
@@ -315,7 +332,8 @@ def main():
                                        f"per step over the elapsed GPU time of all the step's kernels (HIP "
                                        f"events, first to last)"},
                      "measured_stream_copy_gbps": round(copy_gbps, 1),
-                     "frac_of_copy": round(achieved / copy_gbps, 4) if achieved else None},
+                     "frac_of_copy": round(achieved / copy_gbps, 4) if achieved else None,
+                     "pattern": pattern},
     }
     if not args.no_rcs:
         t2 = time.perf_counter()

@@ -276,6 +276,16 @@ int msc_solver_get_timing(const msc_solver* s, msc_timing* out);
 
 void msc_solver_destroy(msc_solver* s);
 
+/* The memory traffic of the two-step kernel over this solve's region with no
+ * arithmetic: the same tiles, the same loads (each tile and its 2-cell halo,
+ * one x plane ahead) and one store per owned value, of the value just read,
+ * to where it was read, so the solve's state is unchanged.  Waits for the
+ * solver's queued work, then times `reps` launches; *ms_per_launch = their
+ * mean.  No counterpart in the reference: it measures the bandwidth the
+ * two-step kernel's access pattern reaches on the device (MSC_EINVAL when
+ * the solve has no two-step region). */
+int msc_solver_pattern_probe(msc_solver* s, int reps, double* ms_per_launch);
+
 /* Streaming copy of `bytes` between two fresh device buffers, `iters` times
  * at each of several grid sizes; *gbps = bytes read + written per second of
  * the fastest.  No counterpart in the

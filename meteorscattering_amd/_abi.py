@@ -233,6 +233,7 @@ def _declare(lib: C.CDLL) -> None:
         "msc_solver_destroy": (None, [vp]),
         "msc_run_dir": (C.c_int, [C.c_char_p, C.c_int]),
         "msc_stream_copy": (C.c_int, [C.c_size_t, C.c_int, dp]),
+        "msc_solver_pattern_probe": (C.c_int, [vp, C.c_int, dp]),
         "msc_solve_footprint": (C.c_size_t, [P(MscParams), C.c_int]),
         "msc_layout_check": (C.c_int, [P(MscParams), C.c_int, C.c_int, P(Tiling), P(LayoutReport)]),
         "msc_server_create": (C.c_int, [C.c_int, C.c_size_t, P(vp)]),

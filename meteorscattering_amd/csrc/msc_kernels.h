@@ -160,6 +160,8 @@ template <typename T>
 hipError_t launch_pair(const StepArgs<T>* a, const uint16_t* sids, int cur, const uint32_t* tiles, int n,
                        hipStream_t s);
 template <typename T>
+hipError_t launch_pair_pattern(const StepArgs<T>* a, const uint32_t* tiles, int n, hipStream_t s);
+template <typename T>
 hipError_t launch_ntff(const NtffArgs<T>* a, const uint2* blocks, int nblocks, hipStream_t s);
 template <typename T>
 hipError_t launch_ntff_contract(const NtffArgs<T>* a, long long U, int M, long long nwaves, hipStream_t s,

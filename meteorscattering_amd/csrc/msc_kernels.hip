@@ -940,6 +940,58 @@ fdtd_pair(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   }
 }
 
+// fdtd_pair's memory traffic with no arithmetic: the same tiles in the same
+// order, the same loads one plane ahead (E and H of the tile and its 2-cell
+// halo; J and the media on tiles that carry J, halo included) and one store
+// per owned value and array, of the value just read, to where it was read,
+// so the solve's state is unchanged.  Timed beside fdtd_pair it measures
+// what this access pattern reaches on the device: an upper bound for
+// fdtd_pair, which moves the same bytes and also computes.
+template <typename T, int PR, int PL>
+__global__ void __launch_bounds__(PR * PL) pair_pattern(const StepArgs<T>* __restrict__ args,
+                                                         const uint32_t* __restrict__ tiles) {
+  constexpr uint32_t W = sizeof(T);
+  const int lane = threadIdx.x & (PL - 1);
+  const int r = threadIdx.x / PL;
+  const uint32_t tile = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+  const ConstArgs<StepArgs<T>>& a = as_const_space(args);
+  const int k0 = a.zstart2[tile & 0xff], k1 = a.zstart2[(tile & 0xff) + 1];
+  const int j0 = a.ystart2[(tile >> 8) & 0xfff], j1 = a.ystart2[((tile >> 8) & 0xfff) + 1];
+  const int ib = a.xstart2[(tile >> 20) & 0x7ff], ie = a.xstart2[((tile >> 20) & 0x7ff) + 1];
+  const bool cj = (tile & kTileNoCurrent) == 0;
+  const int j = j0 - 2 + r, k = k0 - 2 + lane;
+  const bool valid = j >= 0 && j < a.ny && k >= 0 && k < a.nz;
+  const bool own = valid && r >= 2 && r < PR - 2 && j < j1 && lane >= 2 && lane < PL - 2 && k < k1;
+  const bool jl = cj && valid && r >= 1 && r < PR - 1 && lane >= 1 && lane < PL - 1;  // fdtd_pair's J lanes
+  const int t = *a.tdev;
+  const int s0i = t % 3, sj = (t + 2) % 3;
+  T* const ex = a.e[s0i][0]; T* const ey = a.e[s0i][1]; T* const ez = a.e[s0i][2];
+  T* const hx = a.h[s0i][0]; T* const hy = a.h[s0i][1]; T* const hz = a.h[s0i][2];
+  T* const jx = a.j[sj][0]; T* const jy = a.j[sj][1]; T* const jz = a.j[sj][2];
+  const uint32_t sxb = (uint32_t)a.sx * W;
+  const uint32_t ob = valid ? ((uint32_t)j * (uint32_t)a.sy + (uint32_t)k) * W : 0u;
+  T v[11] = {};
+  auto load = [&](int i) {
+    const uint32_t c = (uint32_t)i * sxb + ob;
+    if (valid) { v[0] = ldo(ex, c); v[1] = ldo(ey, c); v[2] = ldo(ez, c); v[3] = ldo(hx, c); v[4] = ldo(hy, c); v[5] = ldo(hz, c); }
+    if (jl) { v[6] = ldo(jx, c); v[7] = ldo(jy, c); v[8] = ldo(jz, c); v[9] = ldo(a.wpe2, c); v[10] = ldo(a.nue, c); }
+  };
+  load(ib - 2);
+  for (int i = ib - 2; i <= ie; ++i) {
+    T c[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) c[q] = v[q];
+    if (i + 1 <= ie) load(i + 1);
+    lds_barrier();  // paces the waves as fdtd_pair's barriers do
+    if (own && i >= ib && i < ie) {
+      const uint32_t cc = (uint32_t)i * sxb + ob;
+      sto(ex, cc, c[0]); sto(ey, cc, c[1]); sto(ez, cc, c[2]);
+      sto(hx, cc, c[3]); sto(hy, cc, c[4]); sto(hz, cc, c[5]);
+      if (cj) { sto(jx, cc, c[6]); sto(jy, cc, c[7]); sto(jz, cc, c[8]); }
+    }
+  }
+}
+
 // J^(t+1) from J^t and E^(t+1) into `out` (what the reference holds after
 // the same number of steps); used only to read J back.
 template <typename T, int CUR>
@@ -1533,6 +1585,14 @@ hipError_t launch_pair(const StepArgs<T>* a, const uint16_t* sids, int cur, cons
 }
 
 template <typename T>
+hipError_t launch_pair_pattern(const StepArgs<T>* a, const uint32_t* tiles, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  constexpr int PL = kPairLanes<T>;
+  hipLaunchKernelGGL((pair_pattern<T, kPairRows, PL>), dim3(n), dim3(kPairRows * PL), 0, s, a, tiles);
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_ntff(const NtffArgs<T>* a, const uint2* blocks, int nblocks, hipStream_t s) {
   if (nblocks <= 0) return hipSuccess;
   hipLaunchKernelGGL((ntff_sample<T>), dim3(nblocks), dim3(kNtffBlock), 0, s, a, blocks);
@@ -1597,6 +1657,7 @@ hipError_t launch_field_slab(const StepArgs<T>* a, int cur, int comp, long long 
                                      hipStream_t);                                                                 \
   template hipError_t launch_pair<T>(const StepArgs<T>*, const uint16_t*, int, const uint32_t*, int, hipStream_t); \
   template hipError_t launch_ntff<T>(const NtffArgs<T>*, const uint2*, int, hipStream_t);                          \
+  template hipError_t launch_pair_pattern<T>(const StepArgs<T>*, const uint32_t*, int, hipStream_t);           \
   template hipError_t launch_ntff_contract<T>(const NtffArgs<T>*, long long, int, long long, hipStream_t, int, int); \
   template hipError_t launch_frames<T>(const AuxArgs<T>*, hipStream_t);                                            \
   template hipError_t launch_probes<T>(const AuxArgs<T>*, int, hipStream_t);                                       \

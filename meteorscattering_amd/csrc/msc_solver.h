@@ -131,6 +131,8 @@ struct SolverBase {
   virtual size_t layout() = 0;
   // Decode every tile of the layout and check it (msc_layout_check).
   virtual int check_layout(msc_layout_report* rep) const = 0;
+  // Mean duration of `reps` launches of pair_pattern over the region's tiles
+  virtual int pattern_probe(int reps, double* ms) = 0;
   // Allocate (arena or hipMalloc), clear and fill the block on `stream`.
   virtual int init(const double* wp, const double* nu) = 0;
   virtual int step(int n) = 0;
@@ -183,6 +185,7 @@ struct Solver : SolverBase {
 
   size_t layout() override;
   int check_layout(msc_layout_report* rep) const override;
+  int pattern_probe(int reps, double* ms) override;
   int init(const double* wp, const double* nu) override;
   int step(int n) override;
   int get_field(int comp, double* out) override;

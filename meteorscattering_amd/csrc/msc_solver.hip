@@ -529,6 +529,29 @@ int Solver<T>::check_layout(msc_layout_report* rep) const {
   return MSC_OK;
 }
 
+template <typename T>
+int Solver<T>::pattern_probe(int reps, double* ms) {
+  const int npair = pair_host.empty() ? 0 : (int)pair_host.size() - outer_off[kNumLists];
+  if (npair <= 0) return fail(MSC_EINVAL, "the solve has no two-step region");
+  if (reps < 1) return fail(MSC_EINVAL, "reps must be >= 1");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  hipError_t e = launch_pair_pattern<T>(sa_dev, tiles2 + outer_off[kNumLists], npair, stream);  // warm
+  if (e == hipSuccess) e = hipEventRecord(e0, stream);
+  for (int q = 0; q < reps && e == hipSuccess; ++q)
+    e = launch_pair_pattern<T>(sa_dev, tiles2 + outer_off[kNumLists], npair, stream);
+  if (e == hipSuccess) e = hipEventRecord(e1, stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (e != hipSuccess) return fail(MSC_EHIP, "pattern probe: %s", hipGetErrorString(e));
+  *ms = t / reps;
+  return MSC_OK;
+}
+
 size_t SolverBase::ntff_doubles() const {
   const size_t M2 = 2 * (size_t)plan.M;
   return 2 * M2 + 8 * M2 * ((size_t)plan.ny * plan.nz + (size_t)plan.nx * plan.nz + (size_t)plan.nx * plan.ny);
@@ -1449,6 +1472,14 @@ int msc_autotune(const msc_params* p, const double* wp, const double* nu, int pr
   *best = cur;
   if (best_ms) *best_ms = cur_ms;
   return MSC_OK;
+}
+
+int msc_solver_pattern_probe(msc_solver* s, int reps, double* ms_per_launch) {
+  SOLVER_OK(s);
+  if (!ms_per_launch) return fail(MSC_EINVAL, "NULL argument");
+  int rc = s->impl->sync();
+  if (rc) return rc;
+  return s->impl->pattern_probe(reps, ms_per_launch);
 }
 
 int msc_solver_step(msc_solver* s, int nsteps) {

@@ -117,6 +117,14 @@ class Solver:
     def kernel_ms(self) -> float:
         return self.lib.msc_solver_kernel_ms(self._h)
 
+    def pattern_probe(self, reps: int = 10) -> float:
+        """Mean ms of `reps` launches of the two-step kernel's memory traffic
+        with no arithmetic over this solve's region (msc_solver_pattern_probe);
+        the solve's state is unchanged."""
+        v = C.c_double()
+        _check(self.lib.msc_solver_pattern_probe(self._h, int(reps), C.byref(v)), self.lib)
+        return v.value
+
     def timing(self) -> dict:
         """Timing of the last step() call by launch kind (``msc_timing``)."""
         t = _abi.Timing()

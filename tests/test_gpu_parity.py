@@ -242,9 +242,9 @@ def test_crowded_probes_give_the_same_bits(msc, monkeypatch, precision):
 @pytest.mark.parametrize("name", ["lk_ricker_ez", "lk_nopml_lhcp", "plate_ricker_45"])
 @pytest.mark.parametrize("pair,precision", [(1, 32), (0, 32), (1, 64), (0, 64)])
 def test_lane_widths_give_the_same_bits(msc, monkeypatch, name, pair, precision):
-    """One-step tiles of 64 z lanes (63 owned z cells; the server's width)
-    and of 32 lanes (two rows per wave, 31 cells; the single-solve default,
-    whose two-step region then starts and ends on multiples of 31) give the
+    """One-step tiles of 64 z lanes (63 owned z cells) and of 32 lanes (two
+    rows per wave, 31 cells; the default of single solves and server gangs,
+    whose two-step regions then start and end on multiples of 31) give the
     same fields, NTFF payload and probes bit for bit."""
     Solver, _ = msc
     p, wp, nu = pair_case(name, 0, 61)
@@ -616,3 +616,27 @@ def test_headline_extensions_on_a_small_grid_match_oracle(msc, oracle_lib, monke
         g.step(10)
         g.sync()
         assert g.timing()["pair_launches"] > 0
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_pattern_probe_leaves_the_solve_unchanged(msc, monkeypatch, precision):
+    """msc_solver_pattern_probe (fdtd_pair's loads and owned stores with no
+    arithmetic) stores every value it reads back where it read it: the
+    fields right after it, and every step after that, are bit-identical to
+    a solve that never ran it."""
+    Solver, _ = msc
+    p, wp, nu = pair_case("lk_ricker_ez", 0, 61)
+    p.update(savefields=0)
+    monkeypatch.setenv("MSC_PAIR", "1")
+    outs = []
+    for probe in (False, True):
+        with Solver(p, wp, nu, precision=precision) as g:
+            g.step(17)
+            if probe:
+                assert g.pattern_probe(3) > 0
+            g.sync()
+            a = snapshot(g)
+            g.step(21)
+            outs.append((a, snapshot(g)))
+    for x, y in zip(outs[0], outs[1]):
+        assert all(np.array_equal(u, v) for u, v in zip(x, y))
