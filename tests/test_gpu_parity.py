@@ -500,12 +500,14 @@ def test_default_two_step_mode_follows_the_region_size(msc, monkeypatch, precisi
         assert (tm["pair_launches"] > 0) == expect, (nx, ny, nz, tiles, cus, tm["pair_launches"])
 
 
-@pytest.mark.parametrize("nntf", [16, 100])
+@pytest.mark.parametrize("nntf", [8, 16, 24, 100])
 @pytest.mark.parametrize("precision,tol", [(64, 1e-10), (32, 2e-5)])
 def test_ntff_matrix_core_contraction_matches_oracle(msc, oracle_lib, nntf, precision, tol):
     """With 2M >= 16 frequency rows the NTFF contraction runs on the matrix
-    cores (FP64 MFMA tiles; 100 frequencies = 200 rows, two row groups and a
-    partial tile).  75 steps close K-step blocks mid-run and at the end."""
+    cores (FP64 MFMA tiles): 8, 16 and 24 frequencies take passes of 1, 2
+    and 4 tiles; 100 frequencies = 200 rows, a pass of 8 tiles and one of 5
+    with a partial tile.  75 steps close K-step blocks (16-slot Phi blocks,
+    the last one partial) mid-run and at the end."""
     Solver, _ = msc
     p, wp, nu = cases.case("lk_ricker_ez")
     p.update(tsteps=75, nntf=nntf, fntf=list(np.linspace(30e6, 80e6, nntf)))
