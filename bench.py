@@ -134,7 +134,7 @@ def batch_or_serve(args, copy_gbps) -> dict:
                 "steps_per_trail": 400}
         log(f"gang-tuned in {gang['tuning_s']} s: {gtable} ({g0:.3f} -> {g1:.3f} s per tuning batch)")
     ser = workloads.run_serial(trails, precision=args.precision, tilings=table)
-    log(f"serial: {ser['wall']:.2f} s ({len(trails) / ser['wall'] * 3600:.0f} solves/hr)")
+    log(f"serial: {ser['wall']:.2f} s busy")
     stable = [(g["max_cells"], {k: g[k] for k in ("ty", "ty_b", "chunk", "chunk2")}) for g in gang["table"]] \
         if gang else table
     if args.workload == "batch":
@@ -148,7 +148,12 @@ def batch_or_serve(args, copy_gbps) -> dict:
     lat = res["latency"] if "latency" in res else np.array([i["t_done"] - i["t_submit"] for i in res["info"]])
     slat = workloads.fifo_latencies(arrivals, ser["times"])
     sph = len(trails) / res["wall"] * 3600
-    sph_serial = len(trails) / ser["wall"] * 3600
+    # the serial baseline under the same arrivals: one solve at a time, first
+    # come first served, so its last result comes at the last arrival's
+    # finish (its busy time when every request arrives at once, as in batch)
+    serial_span = max(float(ser["wall"]), float(np.max(arrivals + slat)))
+    sph_serial = len(trails) / serial_span * 3600
+    log(f"serial under the same arrivals: {serial_span:.2f} s ({sph_serial:.0f} solves/hr)")
     log(f"{args.workload}: {res['wall']:.2f} s ({sph:.0f} solves/hr), x{sph / sph_serial:.2f} vs serial; "
         f"identical results: {same}")
     q = lambda a, f: round(float(np.percentile(a, f)), 3)
@@ -162,8 +167,9 @@ def batch_or_serve(args, copy_gbps) -> dict:
                    "tilings": [{"max_cells": int(m), **t} for m, t in table] if table else "defaults"},
         "gcell_updates_per_s": round(updates / res["wall"] / 1e9, 3),
         "latency_s": {"p50": q(lat, 50), "p99": q(lat, 99), "max": q(lat, 100)},
-        "serial_gpu": {"solves_per_hr": round(sph_serial, 1), "wall_s": round(ser["wall"], 3),
-                       "gcell_updates_per_s": round(updates / ser["wall"] / 1e9, 3),
+        "serial_gpu": {"solves_per_hr": round(sph_serial, 1), "wall_s": round(serial_span, 3),
+                       "busy_s": round(ser["wall"], 3),
+                       "gcell_updates_per_s": round(updates / serial_span / 1e9, 3),
                        "latency_s": {"p50": q(slat, 50), "p99": q(slat, 99), "max": q(slat, 100)}},
         "server_setup_s": round(res["setup"], 3) if "setup" in res else None,
         "speedup_vs_serial": round(sph / sph_serial, 3),
