@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -85,32 +86,29 @@ static void plan_sources(HostPlan& h) {
   }
 }
 
-// CPML tables, meteor-fork.cpp:444-561 (kappa 1, alpha 0; one table per grid
-// since dx == dy == dz).
+// CPML tables (meteor-fork.cpp:444-561, whose stretching kappa is 1 and
+// frequency shift alpha 0, so its recursion coefficients reduce exactly to
+// b = exp(-sigma dt / eps0) and a = b - 1).  Slab cell n of the high side
+// (table entry L + n) and its mirror on the low side (entry L - 1 - n) take
+// the polynomially graded conductivity sigma_max ((n + 1/2) / L)^4 on the
+// E grid and sigma_max ((n + 1) / L)^4 on the H grid, sigma_max = 1.5 (4 + 1)
+// / (150 pi dx).  One table per grid since dx == dy == dz.
 static void plan_cpml(HostPlan& h) {
   const int L = h.L;
   h.be.assign(2 * L, 0); h.ae.assign(2 * L, 0); h.bh.assign(2 * L, 0); h.ah.assign(2 * L, 0);
   if (!h.p.dopml) return;
-  const double pmlm = 4;
-  const double simax = 1.5 * (double)(pmlm + 1) / (150 * kPi * h.dx);
-  const double kamax = 1, almax = 0;
+  const double grade = 4;
+  const double sigma_max = 1.5 * (grade + 1) / (150 * kPi * h.dx);
   for (int n = 0; n < L; n++) {
-    const double mf = (double)n;
-    const double s = simax * std::pow(((mf + 0.5) / L), pmlm);
-    const double sm = simax * std::pow(((mf + 1) / L), pmlm);
-    const double k = 1 + (kamax - 1) * std::pow(((mf + 0.5) / L), pmlm);
-    const double km = 1 + (kamax - 1) * std::pow(((mf + 1) / L), pmlm);
-    const double a = almax * std::pow(((L - mf - 0.5) / L), pmlm);
-    const double am = almax * std::pow(((L - mf - 1) / L), pmlm);
-    const int m = n + L;
-    h.be[m] = std::exp(-((s / k) + a) * h.dt / kE0);
-    h.ae[m] = s / (s * k + std::pow(k, 2) * a) * (h.be[m] - 1);
-    h.bh[m] = std::exp(-((sm / km) + am) * h.dt / kE0);
-    h.ah[m] = sm / (sm * km + std::pow(km, 2) * am) * (h.bh[m] - 1);
-  }
-  for (int m = 0; m < L; m++) {
-    h.be[m] = h.be[2 * L - 1 - m]; h.ae[m] = h.ae[2 * L - 1 - m];
-    h.bh[m] = h.bh[2 * L - 1 - m]; h.ah[m] = h.ah[2 * L - 1 - m];
+    const double sig_e = sigma_max * std::pow((n + 0.5) / L, grade);
+    const double sig_h = sigma_max * std::pow((n + 1.0) / L, grade);
+    const double be = std::exp(-sig_e * h.dt / kE0), bh = std::exp(-sig_h * h.dt / kE0);
+    for (const int m : {L + n, L - 1 - n}) {
+      h.be[m] = be;
+      h.ae[m] = be - 1;
+      h.bh[m] = bh;
+      h.ah[m] = bh - 1;
+    }
   }
 }
 
