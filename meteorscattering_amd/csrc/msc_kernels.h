@@ -159,6 +159,12 @@ hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int c
 template <typename T>
 hipError_t launch_pair(const StepArgs<T>* a, const uint16_t* sids, int cur, const uint32_t* tiles, int n,
                        hipStream_t s);
+// The kNumLists one-step lists of one solve (list l = tiles[off[l],
+// off[l+1]), all of `ty` rows) in one launch, each tile running its list's
+// CPML variant; same results as one launch_step per list.
+template <typename T>
+hipError_t launch_steps(const StepArgs<T>* a, int ty, int cur, bool slice, int zl, const uint32_t* tiles,
+                        const int* off, hipStream_t s);
 template <typename T>
 hipError_t launch_pair_pattern(const StepArgs<T>* a, const uint32_t* tiles, int n, hipStream_t s);
 template <typename T>

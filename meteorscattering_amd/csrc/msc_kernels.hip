@@ -180,22 +180,28 @@ __device__ __forceinline__ bool in_sphere(const A& a, int i, int j, int k) {
   return sqrt(__dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z))) < a.rsph;
 }
 
+// LDS of a one-step workgroup.  sE[b]: one x-plane of E^n with a +1 row
+// and +1 column halo.  Three buffers rotate: plane i (read), plane i+1
+// (staged at the start of plane i), and plane i-1, which slower threads may
+// still read in phase C of plane i-1 while plane i+1 is being staged.
+template <typename T, int TY, int ZL>
+struct StepLds {
+  T sE[3][3][TY + 2][ZL + 1];
+  T sH[3][TY + 1][ZL];
+};
+
+// One one-step tile (tile `blk` of the list `tiles`) with the CPML code of
+// slab axes PM.  The body of fdtd_step, and of fdtd_steps for each variant.
 template <typename T, int TY, int CUR, int PM, bool SLICE, int ZL>
-__global__ void __launch_bounds__((TY + 1) * ZL)
-fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ tiles,
-          const uint16_t* __restrict__ sids) {
+__device__ __forceinline__ void step_tile(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ tiles,
+                                          const uint16_t* __restrict__ sids, int blk, StepLds<T, TY, ZL>& lds) {
   constexpr int R = TY + 1;  // rows including the j0-1 halo row
   constexpr uint32_t W = sizeof(T);
-  // sE[b]: one x-plane of E^n with a +1 row and +1 column halo.  Three
-  // buffers rotate: plane i (read), plane i+1 (staged at the start of plane
-  // i), and plane i-1, which slower threads may still read in phase C of
-  // plane i-1 while plane i+1 is being staged.
-  __shared__ T sE[3][3][R + 1][ZL + 1];
-  __shared__ T sH[3][R][ZL];
+  auto& sE = lds.sE;
+  auto& sH = lds.sH;
 
   const int lane = threadIdx.x & (ZL - 1);
   const int r = threadIdx.x / ZL;
-  const int blk = xcd_remap(blockIdx.x, gridDim.x);
   const uint32_t tile = tiles[blk];
   const ConstArgs<StepArgs<T>>& a = as_const_space(args + (sids ? sids[blk] : 0));
   const int tz = tile & 0xff;
@@ -569,6 +575,34 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
     hyprev = hy;
     hzprev = hz;
   }
+}
+
+template <typename T, int TY, int CUR, int PM, bool SLICE, int ZL>
+__global__ void __launch_bounds__((TY + 1) * ZL)
+fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ tiles,
+          const uint16_t* __restrict__ sids) {
+  __shared__ StepLds<T, TY, ZL> lds;
+  step_tile<T, TY, CUR, PM, SLICE, ZL>(args, tiles, sids, xcd_remap(blockIdx.x, gridDim.x), lds);
+}
+
+// The five one-step lists of a step in one launch: tiles [o[l], o[l+1]) of
+// `tiles` belong to list l (interior, x, y, z and all slab axes, whose
+// variants step_tile carries), so a small grid's lists fill the device
+// together instead of one stream launch each.  Every tile runs its list's
+// code; interior and CPML tiles must have the same rows (TY).
+struct ListOffsets {
+  int o[kNumLists + 1];
+};
+template <typename T, int TY, int CUR, bool SLICE, int ZL>
+__global__ void __launch_bounds__((TY + 1) * ZL)
+fdtd_steps(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ tiles, ListOffsets off) {
+  __shared__ StepLds<T, TY, ZL> lds;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  if (blk < off.o[kListX]) step_tile<T, TY, CUR, 0, SLICE, ZL>(args, tiles, nullptr, blk, lds);
+  else if (blk < off.o[kListY]) step_tile<T, TY, CUR, kPmlX, SLICE, ZL>(args, tiles, nullptr, blk, lds);
+  else if (blk < off.o[kListZ]) step_tile<T, TY, CUR, kPmlY, SLICE, ZL>(args, tiles, nullptr, blk, lds);
+  else if (blk < off.o[kListAll]) step_tile<T, TY, CUR, kPmlZ, SLICE, ZL>(args, tiles, nullptr, blk, lds);
+  else step_tile<T, TY, CUR, kPmlAll, SLICE, ZL>(args, tiles, nullptr, blk, lds);
 }
 
 // Two time steps per launch on the tiles of the two-step region (no CPML
@@ -1567,6 +1601,53 @@ hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int c
   return hipErrorInvalidValue;
 }
 
+template <typename T, int TY, int CUR>
+static hipError_t launch_fc(const StepArgs<T>* a, bool slice, int zl, const uint32_t* tiles, ListOffsets o, int n,
+                            hipStream_t s) {
+  if (zl == 64) {
+    if (slice) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((fdtd_steps<T, TY, CUR, false, 64>), dim3(n), dim3((TY + 1) * 64), 0, s, a, tiles, o);
+  } else if (zl == 32) {
+    if (slice)
+      hipLaunchKernelGGL((fdtd_steps<T, TY, CUR, true, 32>), dim3(n), dim3((TY + 1) * 32), 0, s, a, tiles, o);
+    else
+      hipLaunchKernelGGL((fdtd_steps<T, TY, CUR, false, 32>), dim3(n), dim3((TY + 1) * 32), 0, s, a, tiles, o);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <typename T, int TY>
+static hipError_t launch_fty(const StepArgs<T>* a, int cur, bool slice, int zl, const uint32_t* tiles, ListOffsets o,
+                             int n, hipStream_t s) {
+  switch (cur) {
+    case kCurDI: return launch_fc<T, TY, kCurDI>(a, slice, zl, tiles, o, n, s);
+    case kCurLKPoly: return launch_fc<T, TY, kCurLKPoly>(a, slice, zl, tiles, o, n, s);
+    case kCurLKClosed: return launch_fc<T, TY, kCurLKClosed>(a, slice, zl, tiles, o, n, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+// The kNumLists one-step lists of one solve in one launch (fdtd_steps):
+// list l is tiles[off[l], off[l+1]), the lists one after another; every
+// list has tiles of `ty` rows.
+template <typename T>
+hipError_t launch_steps(const StepArgs<T>* a, int ty, int cur, bool slice, int zl, const uint32_t* tiles,
+                        const int* off, hipStream_t s) {
+  ListOffsets o;
+  for (int l = 0; l <= kNumLists; l++) o.o[l] = off[l] - off[0];
+  const int n = o.o[kNumLists];
+  if (n <= 0) return hipSuccess;
+  tiles += off[0];
+  switch (ty) {
+    case 3: return launch_fty<T, 3>(a, cur, slice, zl, tiles, o, n, s);
+    case 7: return launch_fty<T, 7>(a, cur, slice, zl, tiles, o, n, s);
+    case 15: return launch_fty<T, 15>(a, cur, slice, zl, tiles, o, n, s);
+  }
+  return hipErrorInvalidValue;
+}
+
 // Two steps of the tiles of the two-step region (fdtd_pair): workgroups of
 // kPairRows rows of kPairLanes<T> z lanes (64 in FP32; 32 in FP64, whose
 // 512-thread workgroups leave room for its registers).
@@ -1655,6 +1736,7 @@ hipError_t launch_field_slab(const StepArgs<T>* a, int cur, int comp, long long 
 #define MSC_INST(T)                                                                                                \
   template hipError_t launch_step<T>(const StepArgs<T>*, const uint16_t*, int, int, int, bool, int, const uint32_t*, int, \
                                      hipStream_t);                                                                 \
+  template hipError_t launch_steps<T>(const StepArgs<T>*, int, int, bool, int, const uint32_t*, const int*, hipStream_t); \
   template hipError_t launch_pair<T>(const StepArgs<T>*, const uint16_t*, int, const uint32_t*, int, hipStream_t); \
   template hipError_t launch_ntff<T>(const NtffArgs<T>*, const uint2*, int, hipStream_t);                          \
   template hipError_t launch_pair_pattern<T>(const StepArgs<T>*, const uint32_t*, int, hipStream_t);           \

@@ -114,6 +114,10 @@ struct SolverBase {
   // every other launch ordered on `stream`).
   static constexpr int kMaxAux = 3;
   int list_streams = 2;             // auxiliary streams used (0: every list on `stream`)
+  // 1: the one-step lists of a step run as one launch_steps on `stream`
+  // (when interior and CPML tiles have the same rows) instead of one launch
+  // per list
+  int fuse_lists = 0;
   hipStream_t aux[kMaxAux] = {};
   hipEvent_t fork_ev = nullptr, join_ev[kMaxAux] = {};
   bool timing = false;

@@ -264,12 +264,14 @@ def test_lane_widths_give_the_same_bits(msc, monkeypatch, name, pair, precision)
 
 @pytest.mark.parametrize("pair", [1, 0])
 def test_concurrent_lists_give_the_same_bits(msc, monkeypatch, pair):
-    """The one-step lists of a step on 0 (one after another), 1 and 3
-    auxiliary streams give the same fields, NTFF payload, probes and frames
-    bit for bit: they write disjoint cells and read only the current set."""
+    """The one-step lists of a step launched one per list (MSC_FUSE_LISTS=0)
+    on 0 (one after another), 1 and 3 auxiliary streams give the same
+    fields, NTFF payload, probes and frames bit for bit: they write disjoint
+    cells and read only the current set."""
     Solver, _ = msc
     p, wp, nu = pair_case("lk_ricker_ez", 0, 61)
     monkeypatch.setenv("MSC_PAIR", str(pair))
+    monkeypatch.setenv("MSC_FUSE_LISTS", "0")
     outs = []
     for ns in (0, 1, 3):
         monkeypatch.setenv("MSC_LIST_STREAMS", str(ns))
@@ -280,6 +282,41 @@ def test_concurrent_lists_give_the_same_bits(msc, monkeypatch, pair):
     for o in outs[1:]:
         bad = [i for i, (a, b) in enumerate(zip(o, outs[0])) if not np.array_equal(a, b)]
         assert not bad, bad
+
+
+@pytest.mark.parametrize("name,ty,lanes,closed", [("lk_ricker_ez", 7, 32, 0), ("lk_ricker_ez", 3, 32, 0),
+                                                   ("lk_ricker_ez", 15, 64, 0), ("lk_ricker_ez", 7, 64, 1),
+                                                   ("di_modgauss_ey", 7, 64, 0), ("plate_ricker_45", 7, 32, 0),
+                                                   ("lk_nopml_lhcp", 7, 32, 0)])
+@pytest.mark.parametrize("pair", [0, 1])
+@pytest.mark.parametrize("precision", [32, 64])
+def test_fused_lists_give_the_same_bits(msc, monkeypatch, name, ty, lanes, closed, pair, precision):
+    """MSC_FUSE_LISTS=1 (the five one-step lists of a step in one launch of
+    fdtd_steps, each tile running its list's CPML variant, the default below
+    4 M cells) gives the fields, NTFF payload, probes and frames of
+    MSC_FUSE_LISTS=0 (one launch per list) bit for bit, for
+    tiles of 3, 7 and 15 rows, 32 lanes (with slice steps) and 64, the three
+    current models, and with the two-step region on and off."""
+    Solver, _ = msc
+    p, wp, nu = pair_case(name, 0, 61)
+    if lanes == 64:
+        p.update(savefields=0)
+    monkeypatch.setenv("MSC_PAIR", str(pair))
+    monkeypatch.setenv("MSC_ZLANES", str(lanes))
+    monkeypatch.setenv("MSC_TY", str(ty))
+    monkeypatch.setenv("MSC_TY_B", str(ty))
+    if closed:
+        monkeypatch.setenv("MSC_LK_CLOSED", "1")
+    outs = []
+    for fuse in (0, 1):
+        monkeypatch.setenv("MSC_FUSE_LISTS", str(fuse))
+        with Solver(p, wp, nu, precision=precision) as g:
+            for c in (1, 6, 23, 31):
+                g.step(c)
+            outs.append(snapshot(g))
+    labels = list(_abi.COMPONENTS) + ["ntff", "probes"] + [f"frame {f}" for f in range(len(outs[0]) - 11)]
+    bad = [f"{lab}: {worst(a, b)}" for lab, a, b in zip(labels, outs[1], outs[0]) if not np.array_equal(a, b)]
+    assert not bad, "\n".join(bad)
 
 
 def test_lane_width_is_checked(msc, monkeypatch):
