@@ -831,12 +831,13 @@ int Solver<T>::init(const double* wp, const double* nu) {
     own_stream = true;
   }
   if (const char* s = std::getenv("MSC_LIST_STREAMS")) list_streams = std::max(0, std::min(kMaxAux, std::atoi(s)));
-  // one launch for the one-step lists below 4 M cells, where launch and
-  // fork/join gaps dominate a step (2.1x faster at 0.5 M cells, 1.45x at
-  // 1.8 M, 1.17x at 2.8 M); above, the per-list launches keep the interior
-  // tiles at their own occupancy (7 waves per SIMD against the fused
-  // kernel's 5) and are 2-9% faster
-  fuse_lists = N < 4000000;
+  // one launch for the one-step lists below 16 MB per field array (4 M
+  // cells in FP32, 2 M in FP64), where launch and fork/join gaps dominate a
+  // step (FP32: 2.1x faster at 0.5 M cells, 1.45x at 1.8 M, 1.17x at 2.8 M;
+  // FP64: 1.49x at 0.5 M, 1.08x at 1.8 M); above, the per-list launches
+  // keep the interior tiles at their own occupancy (7 waves per SIMD
+  // against the fused kernel's 5 in FP32) and are 2-9% faster
+  fuse_lists = (double)N * sizeof(T) < 16e6;
   if (const char* s = std::getenv("MSC_FUSE_LISTS")) fuse_lists = std::atoi(s) != 0;
 
   if (arena) {

@@ -293,7 +293,7 @@ def test_concurrent_lists_give_the_same_bits(msc, monkeypatch, pair):
 def test_fused_lists_give_the_same_bits(msc, monkeypatch, name, ty, lanes, closed, pair, precision):
     """MSC_FUSE_LISTS=1 (the five one-step lists of a step in one launch of
     fdtd_steps, each tile running its list's CPML variant, the default below
-    4 M cells) gives the fields, NTFF payload, probes and frames of
+    16 MB per field array) gives the fields, NTFF payload, probes and frames of
     MSC_FUSE_LISTS=0 (one launch per list) bit for bit, for
     tiles of 3, 7 and 15 rows, 32 lanes (with slice steps) and 64, the three
     current models, and with the two-step region on and off."""
