@@ -801,6 +801,11 @@ int Solver<T>::init(const double* wp, const double* nu) {
     // few and their marches latency-bound, and shorter marches pay (3-27%
     // faster on 8 grids of 1.3-22 M cells, 3% slower at 33 M and 128 M)
     if (N < 24000000 && nx / 4 + 8 <= kMaxChunks) chunk = 4;
+    // shorter still on the smallest grids, whose few tiles run as one
+    // launch (FP32 ms per step, chunks of 2 / 3 / 4 planes: 48x48x96
+    // 0.027 / 0.031 / 0.035, 64x64x128 0.039 / 0.035 / 0.039, 96x96x128
+    // 0.059 / 0.057 / 0.063, 96x96x192 0.075 / 0.072 / 0.071)
+    if (N < 1500000) chunk = N < 350000 ? 2 : 3;
     if (const char* s = std::getenv("MSC_CHUNK")) chunk = std::max(1, std::atoi(s));
     if (const char* s = std::getenv("MSC_PAIR")) pair = std::atoi(s) != 0;
     if (const char* s = std::getenv("MSC_CHUNK2")) chunk2 = std::max(1, std::atoi(s));
