@@ -159,12 +159,13 @@ hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int c
 template <typename T>
 hipError_t launch_pair(const StepArgs<T>* a, const uint16_t* sids, int cur, const uint32_t* tiles, int n,
                        hipStream_t s);
-// The kNumLists one-step lists of one solve (list l = tiles[off[l],
-// off[l+1]), all of `ty` rows) in one launch, each tile running its list's
-// CPML variant; same results as one launch_step per list.
+// The kNumLists one-step lists (list l = tiles[first[l], first[l] +
+// count[l]), all of `ty` rows; tile b of solve sids[b], or solve 0 when sids
+// is null) in one launch, each tile running its list's CPML variant; the
+// same results as one launch_step per list.
 template <typename T>
-hipError_t launch_steps(const StepArgs<T>* a, int ty, int cur, bool slice, int zl, const uint32_t* tiles,
-                        const int* off, hipStream_t s);
+hipError_t launch_steps(const StepArgs<T>* a, const uint16_t* sids, int ty, int cur, bool slice, int zl,
+                        const uint32_t* tiles, const int* first, const int* count, hipStream_t s);
 template <typename T>
 hipError_t launch_pair_pattern(const StepArgs<T>* a, const uint32_t* tiles, int n, hipStream_t s);
 template <typename T>

@@ -585,24 +585,32 @@ fdtd_step(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ til
   step_tile<T, TY, CUR, PM, SLICE, ZL>(args, tiles, sids, xcd_remap(blockIdx.x, gridDim.x), lds);
 }
 
-// The five one-step lists of a step in one launch: tiles [o[l], o[l+1]) of
-// `tiles` belong to list l (interior, x, y, z and all slab axes, whose
-// variants step_tile carries), so a small grid's lists fill the device
-// together instead of one stream launch each.  Every tile runs its list's
-// code; interior and CPML tiles must have the same rows (TY).
-struct ListOffsets {
-  int o[kNumLists + 1];
+// The five one-step lists of a step in one launch, so a small grid's lists
+// (or a small gang's) fill the device together instead of one launch each.
+// List l (interior, x, y, z and all slab axes, whose variants step_tile
+// carries) is tiles[first[l] .. first[l] + n_l), and its workgroups are
+// [end[l-1], end[l]) (end[-1] = 0).  Every tile runs its list's code;
+// interior and CPML tiles must have the same rows (TY).
+struct ListSpans {
+  int first[kNumLists];
+  int end[kNumLists];
 };
 template <typename T, int TY, int CUR, bool SLICE, int ZL>
 __global__ void __launch_bounds__((TY + 1) * ZL)
-fdtd_steps(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ tiles, ListOffsets off) {
+fdtd_steps(const StepArgs<T>* __restrict__ args, const uint32_t* __restrict__ tiles,
+           const uint16_t* __restrict__ sids, ListSpans sp) {
   __shared__ StepLds<T, TY, ZL> lds;
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
-  if (blk < off.o[kListX]) step_tile<T, TY, CUR, 0, SLICE, ZL>(args, tiles, nullptr, blk, lds);
-  else if (blk < off.o[kListY]) step_tile<T, TY, CUR, kPmlX, SLICE, ZL>(args, tiles, nullptr, blk, lds);
-  else if (blk < off.o[kListZ]) step_tile<T, TY, CUR, kPmlY, SLICE, ZL>(args, tiles, nullptr, blk, lds);
-  else if (blk < off.o[kListAll]) step_tile<T, TY, CUR, kPmlZ, SLICE, ZL>(args, tiles, nullptr, blk, lds);
-  else step_tile<T, TY, CUR, kPmlAll, SLICE, ZL>(args, tiles, nullptr, blk, lds);
+  if (blk < sp.end[kListInterior])
+    step_tile<T, TY, CUR, 0, SLICE, ZL>(args, tiles, sids, sp.first[kListInterior] + blk, lds);
+  else if (blk < sp.end[kListX])
+    step_tile<T, TY, CUR, kPmlX, SLICE, ZL>(args, tiles, sids, sp.first[kListX] + blk - sp.end[kListInterior], lds);
+  else if (blk < sp.end[kListY])
+    step_tile<T, TY, CUR, kPmlY, SLICE, ZL>(args, tiles, sids, sp.first[kListY] + blk - sp.end[kListX], lds);
+  else if (blk < sp.end[kListZ])
+    step_tile<T, TY, CUR, kPmlZ, SLICE, ZL>(args, tiles, sids, sp.first[kListZ] + blk - sp.end[kListY], lds);
+  else
+    step_tile<T, TY, CUR, kPmlAll, SLICE, ZL>(args, tiles, sids, sp.first[kListAll] + blk - sp.end[kListZ], lds);
 }
 
 // Two time steps per launch on the tiles of the two-step region (no CPML
@@ -1602,16 +1610,16 @@ hipError_t launch_step(const StepArgs<T>* a, const uint16_t* sids, int ty, int c
 }
 
 template <typename T, int TY, int CUR>
-static hipError_t launch_fc(const StepArgs<T>* a, bool slice, int zl, const uint32_t* tiles, ListOffsets o, int n,
-                            hipStream_t s) {
+static hipError_t launch_fc(const StepArgs<T>* a, const uint16_t* sids, bool slice, int zl, const uint32_t* tiles,
+                            const ListSpans& sp, int n, hipStream_t s) {
   if (zl == 64) {
     if (slice) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((fdtd_steps<T, TY, CUR, false, 64>), dim3(n), dim3((TY + 1) * 64), 0, s, a, tiles, o);
+    hipLaunchKernelGGL((fdtd_steps<T, TY, CUR, false, 64>), dim3(n), dim3((TY + 1) * 64), 0, s, a, tiles, sids, sp);
   } else if (zl == 32) {
     if (slice)
-      hipLaunchKernelGGL((fdtd_steps<T, TY, CUR, true, 32>), dim3(n), dim3((TY + 1) * 32), 0, s, a, tiles, o);
+      hipLaunchKernelGGL((fdtd_steps<T, TY, CUR, true, 32>), dim3(n), dim3((TY + 1) * 32), 0, s, a, tiles, sids, sp);
     else
-      hipLaunchKernelGGL((fdtd_steps<T, TY, CUR, false, 32>), dim3(n), dim3((TY + 1) * 32), 0, s, a, tiles, o);
+      hipLaunchKernelGGL((fdtd_steps<T, TY, CUR, false, 32>), dim3(n), dim3((TY + 1) * 32), 0, s, a, tiles, sids, sp);
   } else {
     return hipErrorInvalidValue;
   }
@@ -1619,31 +1627,35 @@ static hipError_t launch_fc(const StepArgs<T>* a, bool slice, int zl, const uint
 }
 
 template <typename T, int TY>
-static hipError_t launch_fty(const StepArgs<T>* a, int cur, bool slice, int zl, const uint32_t* tiles, ListOffsets o,
-                             int n, hipStream_t s) {
+static hipError_t launch_fty(const StepArgs<T>* a, const uint16_t* sids, int cur, bool slice, int zl,
+                             const uint32_t* tiles, const ListSpans& sp, int n, hipStream_t s) {
   switch (cur) {
-    case kCurDI: return launch_fc<T, TY, kCurDI>(a, slice, zl, tiles, o, n, s);
-    case kCurLKPoly: return launch_fc<T, TY, kCurLKPoly>(a, slice, zl, tiles, o, n, s);
-    case kCurLKClosed: return launch_fc<T, TY, kCurLKClosed>(a, slice, zl, tiles, o, n, s);
+    case kCurDI: return launch_fc<T, TY, kCurDI>(a, sids, slice, zl, tiles, sp, n, s);
+    case kCurLKPoly: return launch_fc<T, TY, kCurLKPoly>(a, sids, slice, zl, tiles, sp, n, s);
+    case kCurLKClosed: return launch_fc<T, TY, kCurLKClosed>(a, sids, slice, zl, tiles, sp, n, s);
   }
   return hipErrorInvalidValue;
 }
 
-// The kNumLists one-step lists of one solve in one launch (fdtd_steps):
-// list l is tiles[off[l], off[l+1]), the lists one after another; every
-// list has tiles of `ty` rows.
+// The kNumLists one-step lists in one launch (fdtd_steps): list l is
+// tiles[first[l], first[l] + count[l]) (solve ids sids[...] alike, or every
+// tile solve 0 when sids is null); every list has tiles of `ty` rows.
 template <typename T>
-hipError_t launch_steps(const StepArgs<T>* a, int ty, int cur, bool slice, int zl, const uint32_t* tiles,
-                        const int* off, hipStream_t s) {
-  ListOffsets o;
-  for (int l = 0; l <= kNumLists; l++) o.o[l] = off[l] - off[0];
-  const int n = o.o[kNumLists];
+hipError_t launch_steps(const StepArgs<T>* a, const uint16_t* sids, int ty, int cur, bool slice, int zl,
+                        const uint32_t* tiles, const int* first, const int* count, hipStream_t s) {
+  ListSpans sp;
+  int n = 0;
+  for (int l = 0; l < kNumLists; l++) {
+    if (count[l] < 0) return hipErrorInvalidValue;
+    sp.first[l] = first[l];
+    n += count[l];
+    sp.end[l] = n;
+  }
   if (n <= 0) return hipSuccess;
-  tiles += off[0];
   switch (ty) {
-    case 3: return launch_fty<T, 3>(a, cur, slice, zl, tiles, o, n, s);
-    case 7: return launch_fty<T, 7>(a, cur, slice, zl, tiles, o, n, s);
-    case 15: return launch_fty<T, 15>(a, cur, slice, zl, tiles, o, n, s);
+    case 3: return launch_fty<T, 3>(a, sids, cur, slice, zl, tiles, sp, n, s);
+    case 7: return launch_fty<T, 7>(a, sids, cur, slice, zl, tiles, sp, n, s);
+    case 15: return launch_fty<T, 15>(a, sids, cur, slice, zl, tiles, sp, n, s);
   }
   return hipErrorInvalidValue;
 }
@@ -1736,7 +1748,8 @@ hipError_t launch_field_slab(const StepArgs<T>* a, int cur, int comp, long long 
 #define MSC_INST(T)                                                                                                \
   template hipError_t launch_step<T>(const StepArgs<T>*, const uint16_t*, int, int, int, bool, int, const uint32_t*, int, \
                                      hipStream_t);                                                                 \
-  template hipError_t launch_steps<T>(const StepArgs<T>*, int, int, bool, int, const uint32_t*, const int*, hipStream_t); \
+  template hipError_t launch_steps<T>(const StepArgs<T>*, const uint16_t*, int, int, bool, int, const uint32_t*,     \
+                                      const int*, const int*, hipStream_t);                                      \
   template hipError_t launch_pair<T>(const StepArgs<T>*, const uint16_t*, int, const uint32_t*, int, hipStream_t); \
   template hipError_t launch_ntff<T>(const NtffArgs<T>*, const uint2*, int, hipStream_t);                          \
   template hipError_t launch_pair_pattern<T>(const StepArgs<T>*, const uint32_t*, int, hipStream_t);           \

@@ -151,6 +151,23 @@ struct Server : ServerBase {
   size_t ptile_off[3] = {};
   int n_nb = 0, n_nb_b = 0;
   size_t n_pair = 0;
+  // The gang's one-step lists run as one launch per current model and tile
+  // rows (fdtd_steps) when its one-step cells fill less than 64 MB of one
+  // field array (16 M cells in FP32).  Gang launches run one after another,
+  // so the lists gain more from one launch than a single solve's, whose
+  // lists overlap on three streams: gangs of 0.5-33 M cells ran 1.1-2.3x
+  // faster, 65-134 M within 1%, 180 M 4% and 422 M (the 48-trail sweep) 9%
+  // slower.
+  // MSC_SERVER_FUSE=0/1 decides instead.
+  int fuse_env = -1;
+  bool fuse_gang = false;
+  // The gang runs two-step tiles only when its solves' regions give at
+  // least pair_min fdtd_pair workgroups together (1.5 per CU, the single-
+  // solve rule): a gang of a few small solves otherwise spends each
+  // iteration in a handful of long two-step marches.  MSC_SERVER_PAIR_MIN
+  // sets the count (0: a gang pairs whenever a solve has a region).
+  long long pair_min = 0;
+  bool gang_pairs = true;
   // run-ahead limit: at most kDepth iterations queued on the run stream
   static constexpr int kDepth = 4;
   hipEvent_t iter_ev[kDepth] = {};
@@ -201,6 +218,13 @@ struct Server : ServerBase {
     if (rc) return rc;
     if (const char* f = std::getenv("MSC_FILL_TILES")) fill_tiles = std::atoi(f);
     if (const char* f = std::getenv("MSC_GRAPHS")) use_graphs = std::atoi(f) != 0;
+    if (const char* f = std::getenv("MSC_SERVER_FUSE")) fuse_env = std::atoi(f) != 0;
+    {
+      int cus = 0;
+      HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+      pair_min = 3LL * cus / 2;
+    }
+    if (const char* f = std::getenv("MSC_SERVER_PAIR_MIN")) pair_min = std::max(0LL, std::atoll(f));
     worker = std::thread([this] { work(); });
     return MSC_OK;
   }
@@ -439,18 +463,27 @@ struct Server : ServerBase {
     const int gi = cur_gang ^ 1;  // the set the queued launches are not reading
     Gang& g = gang[gi];
     if (g.used) HIPCHK(hipEventSynchronize(g.last_use));
+    gang_pairs = true;
     select();
+    long long region_tiles = 0;
+    for (const Req* r : sel)
+      if (has_pair(r)) region_tiles += (long long)r->s->pair_host.size() - r->s->outer_off[kNumLists];
+    gang_pairs = region_tiles >= pair_min;
     // two-step solves (a region, two or more steps left) first
-    std::stable_partition(sel.begin(), sel.end(), [](const Req* r) { return pairs(r); });
-    n_pair = (size_t)std::count_if(sel.begin(), sel.end(), [](const Req* r) { return pairs(r); });
+    std::stable_partition(sel.begin(), sel.end(), [this](const Req* r) { return pairs(r); });
+    n_pair = (size_t)std::count_if(sel.begin(), sel.end(), [this](const Req* r) { return pairs(r); });
     const std::vector<Req*>& active = sel;
     const size_t ns = active.size();
     if (ns > 65535) return fail(MSC_EINVAL, "more than 65535 resident solves");
     size_t nt = 0, nnb = 0;
+    double cells1 = 0;  // cells the one-step lists of half 0 update
     for (Req* r : active) {
       nt += pairs(r) ? r->s->pair_host.size() : r->s->tiles_host.size();
       nnb += r->s->ntff_host.size();
+      const HostPlan& hp = r->s->plan;
+      cells1 += (double)hp.nx * hp.ny * hp.nz - (pairs(r) ? (double)r->s->pair_cells : 0.0);
     }
+    fuse_gang = fuse_env >= 0 ? fuse_env != 0 : cells1 * sizeof(T) < 64e6;
     int rc = ensure(g, std::max<size_t>(ns, 1), std::max<size_t>(nt, 1), std::max<size_t>(nnb, 1));
     if (rc) return rc;
     // host image: args, then tiles/sids grouped by (current model, variant), then NTFF blocks
@@ -520,7 +553,8 @@ struct Server : ServerBase {
 
   // A solve steps two at a time when it has a two-step region and at least
   // two steps left (the server's solves write no frames or slices).
-  static bool pairs(const Req* r) { return !r->s->pair_host.empty() && r->t + 2 <= r->s->plan.T; }
+  static bool has_pair(const Req* r) { return !r->s->pair_host.empty() && r->t + 2 <= r->s->plan.T; }
+  bool pairs(const Req* r) const { return gang_pairs && has_pair(r); }
 
   // The fixed part of an iteration, in two halves.  Half 0: step t of every
   // solve (the one-step launches per current model and variant, the
@@ -532,13 +566,24 @@ struct Server : ServerBase {
   hipError_t launch_fixed(const Gang& g, int half) {
     hipError_t e = hipSuccess;
     for (int c = 0; c < 3 && e == hipSuccess; c++) {
-      for (int v = 0; v < kNumLists && e == hipSuccess; v++)
+      if (fuse_gang) {
         for (int y = 0; y < 3 && e == hipSuccess; y++) {
-          const int n = half ? n_tiles_b[c][v][y] : n_tiles[c][v][y];
-          if (!n) continue;
-          const size_t o = tile_off[c][v][y];
-          e = launch_step<T>(g.sa, g.sids + o, kTys[y], c, list_pm(v), false, kServerZLanes, g.tiles + o, n, run);
+          int first[kNumLists], count[kNumLists];
+          for (int v = 0; v < kNumLists; v++) {
+            first[v] = (int)tile_off[c][v][y];
+            count[v] = half ? n_tiles_b[c][v][y] : n_tiles[c][v][y];
+          }
+          e = launch_steps<T>(g.sa, g.sids, kTys[y], c, false, kServerZLanes, g.tiles, first, count, run);
         }
+      } else {
+        for (int v = 0; v < kNumLists && e == hipSuccess; v++)
+          for (int y = 0; y < 3 && e == hipSuccess; y++) {
+            const int n = half ? n_tiles_b[c][v][y] : n_tiles[c][v][y];
+            if (!n) continue;
+            const size_t o = tile_off[c][v][y];
+            e = launch_step<T>(g.sa, g.sids + o, kTys[y], c, list_pm(v), false, kServerZLanes, g.tiles + o, n, run);
+          }
+      }
       if (!half && n_ptiles[c] && e == hipSuccess)
         e = launch_pair<T>(g.sa, g.sids + ptile_off[c], c, g.tiles + ptile_off[c], n_ptiles[c], run);
     }

@@ -1092,7 +1092,9 @@ int Solver<T>::step(int n) {
   while (naux < std::min(list_streams, kMaxAux) && aux[naux]) naux++;
   auto lists = [&](const uint32_t* base, const int* off, bool slice) -> int {
     if (fuse_lists && ty == ty_b) {
-      HIPCHK(launch_steps<T>(sa_dev, ty, cur, slice, zlanes, base, off, stream));
+      int cnt[kNumLists];
+      for (int l = 0; l < kNumLists; l++) cnt[l] = off[l + 1] - off[l];
+      HIPCHK(launch_steps<T>(sa_dev, nullptr, ty, cur, slice, zlanes, base, off, cnt, stream));
       return MSC_OK;
     }
     if (naux == 0) {

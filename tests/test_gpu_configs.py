@@ -184,11 +184,13 @@ def test_sweep_and_stream_trails_match_oracle(oracle_lib, mix, which, precision)
         check_run(g, st, precision, f"trail {p['nx']}x{p['ny']}x{p['nz']} M={p['nntf']}")
 
 
-def test_sweep_and_stream_trails_in_the_server_equal_alone(mix):
+def test_sweep_and_stream_trails_in_the_server_equal_alone(mix, monkeypatch):
     """Whole solves (1,500-4,000 steps) of six workload trails, one of them
-    mid-size with a two-step region, all resident in one server: every NTFF
-    payload and probe history equals the same solve alone, bit for bit, and
-    is finite."""
+    mid-size with a two-step region (paired whatever its size:
+    MSC_SERVER_PAIR_MIN=0), all resident in one server: every NTFF payload
+    and probe history equals the same solve alone, bit for bit, and is
+    finite."""
+    monkeypatch.setenv("MSC_SERVER_PAIR_MIN", "0")
     alone = []
     for p, wp, nu in mix:
         with Solver(p, wp, nu, precision=32) as g:

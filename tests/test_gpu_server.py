@@ -151,7 +151,8 @@ TWO_STEP_MIX = {
 def test_two_step_gangs_equal_alone(fill, graphs, monkeypatch):
     """Solves with a two-step region advance two steps per gang iteration
     (the last one alone when their step count is odd) beside one-step
-    solves; fill 1 steps one solve at a time, 5000 a few at a time.  The
+    solves, with MSC_SERVER_PAIR_MIN=0 so that gangs of these small regions
+    pair; fill 1 steps one solve at a time, 5000 a few at a time.  The
     NTFF payloads and probes equal the bits of single solves run one-step
     (these regions are too small for a single solve's two-step rule) with
     64-lane tiles."""
@@ -159,6 +160,7 @@ def test_two_step_gangs_equal_alone(fill, graphs, monkeypatch):
     monkeypatch.setenv("MSC_ZLANES", "64")
     alone = {k: _alone(*v) for k, v in mix.items()}
     monkeypatch.delenv("MSC_ZLANES")
+    monkeypatch.setenv("MSC_SERVER_PAIR_MIN", "0")  # these small regions pair in any gang
     with Server(precision=32, budget_bytes=4 << 30, fill_tiles=fill, graphs=graphs) as srv:
         ids = {k: srv.submit(*v) for k, v in mix.items()}
         srv.run_until_done()
@@ -173,12 +175,55 @@ def test_two_step_gangs_equal_alone(fill, graphs, monkeypatch):
             assert st["iterations"] == expect
 
 
-def test_fp64_two_step_gangs_equal_alone():
+@pytest.mark.parametrize("fuse,graphs,precision", [(1, True, 32), (1, False, 32), (0, True, 32), (1, True, 64)])
+def test_one_launch_gang_lists_equal_alone(fuse, graphs, precision, monkeypatch):
+    """MSC_SERVER_FUSE=1 runs a gang's one-step lists as one launch per
+    current model and tile rows (fdtd_steps, the default for gangs of few
+    one-step cells), 0 as one launch per list and variant; with two-step
+    and one-step solves of two current models in one gang, captured or
+    launched directly, both give the single-solve bits."""
+    mix = {k: f() for k, f in TWO_STEP_MIX.items()}
+    mix["di_small"] = _case("di_modgauss_ey", tsteps=40)
+    alone = {k: _alone(*v, precision=precision) for k, v in mix.items()}
+    monkeypatch.setenv("MSC_SERVER_FUSE", str(fuse))
+    monkeypatch.setenv("MSC_SERVER_PAIR_MIN", "0")  # both halves of an iteration
+    with Server(precision=precision, budget_bytes=4 << 30, fill_tiles=0, graphs=graphs) as srv:
+        ids = {k: srv.submit(*v) for k, v in mix.items()}
+        srv.run_until_done()
+        for k in mix:
+            np.testing.assert_array_equal(srv.ntff(ids[k]), alone[k][0], err_msg=k)
+            np.testing.assert_array_equal(srv.probes(ids[k]), alone[k][1], err_msg=k)
+
+
+@pytest.mark.parametrize("fuse", [1, 0])
+def test_gangs_of_small_regions_run_one_step(fuse, monkeypatch):
+    """By default a gang pairs only when its solves' regions give at least
+    1.5 two-step workgroups per CU together.  With fill 1 each gang is one
+    of these small solves, whose region does not, so every iteration
+    advances it by one step (iterations = solve steps, where
+    test_two_step_gangs_equal_alone's paired gangs take about half), and
+    the results are the single-solve bits."""
+    mix = {k: f() for k, f in TWO_STEP_MIX.items()}
+    alone = {k: _alone(*v) for k, v in mix.items()}
+    monkeypatch.delenv("MSC_SERVER_PAIR_MIN", raising=False)
+    monkeypatch.setenv("MSC_SERVER_FUSE", str(fuse))
+    with Server(precision=32, budget_bytes=4 << 30, fill_tiles=1) as srv:
+        ids = {k: srv.submit(*v) for k, v in mix.items()}
+        srv.run_until_done()
+        for k in mix:
+            np.testing.assert_array_equal(srv.ntff(ids[k]), alone[k][0], err_msg=k)
+            np.testing.assert_array_equal(srv.probes(ids[k]), alone[k][1], err_msg=k)
+        st = srv.stats()
+        assert st["iterations"] == st["solve_steps"] == sum(v[0]["tsteps"] for v in mix.values())
+
+
+def test_fp64_two_step_gangs_equal_alone(monkeypatch):
     """FP64 gangs with two-step tiles (16 rows x 32 lanes, 12 x 28 owned) on
-    the 60x50x205 grids equal FP64 single solves (which run these small
-    regions one-step) bit for bit."""
+    the 60x50x205 grids (paired with MSC_SERVER_PAIR_MIN=0) equal FP64
+    single solves (which run these small regions one-step) bit for bit."""
     mix = {k: f() for k, f in TWO_STEP_MIX.items()}
     alone = {k: _alone(*v, precision=64) for k, v in mix.items()}
+    monkeypatch.setenv("MSC_SERVER_PAIR_MIN", "0")
     with Server(precision=64, budget_bytes=4 << 30, fill_tiles=1) as srv:
         ids = {k: srv.submit(*v) for k, v in mix.items()}
         srv.run_until_done()
@@ -203,13 +248,15 @@ def test_server_refuses_auxiliary_list_streams(monkeypatch):
         assert srv.stats()["arena_bytes"] >= 1 << 30
 
 
-def test_size_class_tilings_equal_alone():
+def test_size_class_tilings_equal_alone(monkeypatch):
     """Tilings by size class (small grids 3/15-row tiles and 5/7-plane
     chunks, large ones 15/3 rows and 11/13 planes): the gang launches carry
-    several tile heights at once; results equal the single-solve bits.  The
-    table can change only while the server holds no request."""
+    several tile heights at once (two-step gangs: MSC_SERVER_PAIR_MIN=0);
+    results equal the single-solve bits.  The table can change only while
+    the server holds no request."""
     mix = {k: f() for k, f in TWO_STEP_MIX.items()}
     alone = {k: _alone(*v) for k, v in mix.items()}
+    monkeypatch.setenv("MSC_SERVER_PAIR_MIN", "0")
     small = {"ty": 3, "ty_b": 15, "chunk": 5, "chunk2": 7}
     large = {"ty": 15, "ty_b": 3, "chunk": 11, "chunk2": 13}
     with Server(precision=32, budget_bytes=4 << 30, fill_tiles=0) as srv:
@@ -225,8 +272,10 @@ def test_size_class_tilings_equal_alone():
 
 def test_compensated_fp32_ntff_sums_in_the_server_equal_alone(monkeypatch):
     """With MSC_NTFF_ACC=32 the gang's contractions run the compensated FP32
-    kernel on each solve's own sums: payloads equal the single solves'."""
+    kernel on each solve's own sums (two-step gangs: MSC_SERVER_PAIR_MIN=0):
+    payloads equal the single solves'."""
     monkeypatch.setenv("MSC_NTFF_ACC", "32")
+    monkeypatch.setenv("MSC_SERVER_PAIR_MIN", "0")
     mix = {k: f() for k, f in TWO_STEP_MIX.items()}
     alone = {k: _alone(*v) for k, v in mix.items()}
     with Server(precision=32, budget_bytes=4 << 30, fill_tiles=0) as srv:
