@@ -483,7 +483,7 @@ struct Server : ServerBase {
       const HostPlan& hp = r->s->plan;
       cells1 += (double)hp.nx * hp.ny * hp.nz - (pairs(r) ? (double)r->s->pair_cells : 0.0);
     }
-    fuse_gang = fuse_env >= 0 ? fuse_env != 0 : cells1 * sizeof(T) < 64e6;
+    fuse_gang = (fuse_env >= 0 ? fuse_env != 0 : cells1 * sizeof(T) < 64e6) && nt < (size_t)INT32_MAX;
     int rc = ensure(g, std::max<size_t>(ns, 1), std::max<size_t>(nt, 1), std::max<size_t>(nnb, 1));
     if (rc) return rc;
     // host image: args, then tiles/sids grouped by (current model, variant), then NTFF blocks
