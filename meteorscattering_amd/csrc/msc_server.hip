@@ -168,6 +168,16 @@ struct Server : ServerBase {
   // sets the count (0: a gang pairs whenever a solve has a region).
   long long pair_min = 0;
   bool gang_pairs = true;
+  // MSC_SERVER_TIMING=1: launches go out directly (no graphs) with HIP
+  // events between the parts of each half, every iteration waits for its
+  // events, and the destructor prints where the GPU time went (to stderr).
+  // Serialises host and device: for diagnosis, not for serving.
+  bool timing = false;
+  hipEvent_t tev[2][5] = {};
+  double tsum[2][4] = {};  // [half][one-step lists, two-step launch, NTFF samples + probes, contractions]
+  double tcells[2][2] = {};  // [half][cell-updates of the one-step lists, of the two-step launch]
+  long long thalves[2] = {};
+  double gang_cells[3] = {};  // the current gang's one-step cells of half 0, of half 1, and region cells
   // run-ahead limit: at most kDepth iterations queued on the run stream
   static constexpr int kDepth = 4;
   hipEvent_t iter_ev[kDepth] = {};
@@ -180,7 +190,7 @@ struct Server : ServerBase {
   double clock() const override { return now(); }
   int configure(int fill, int graphs) override {
     fill_tiles = fill;
-    use_graphs = graphs != 0;
+    use_graphs = graphs != 0 && !timing;  // timed launches go out directly
     dirty = true;
     return MSC_OK;
   }
@@ -225,11 +235,37 @@ struct Server : ServerBase {
       pair_min = 3LL * cus / 2;
     }
     if (const char* f = std::getenv("MSC_SERVER_PAIR_MIN")) pair_min = std::max(0LL, std::atoll(f));
+    if (const char* f = std::getenv("MSC_SERVER_TIMING")) timing = std::atoi(f) != 0;
+    if (timing) {
+      use_graphs = false;
+      for (auto& hv : tev)
+        for (auto& e : hv) HIPCHK(hipEventCreate(&e));
+    }
     worker = std::thread([this] { work(); });
     return MSC_OK;
   }
 
   ~Server() override {
+    if (timing && thalves[0] > 0) {
+      static const char* kPart[4] = {"one-step lists", "two-step launch", "NTFF samples + probes", "contractions"};
+      double tot = 0;
+      for (auto& hv : tsum)
+        for (double v : hv) tot += v;
+      std::fprintf(stderr, "server timing: %lld iterations (%lld with a second half), %.1f ms of GPU time\n",
+                   thalves[0], thalves[1], tot);
+      for (int h = 0; h < 2; h++)
+        for (int k = 0; k < 4; k++)
+          if (tsum[h][k] > 0) {
+            std::fprintf(stderr, "  half %d %-22s %10.1f ms  %5.1f%%  %8.1f us per half", h, kPart[k], tsum[h][k],
+                         100 * tsum[h][k] / tot, 1e3 * tsum[h][k] / thalves[h]);
+            if (k < 2 && tcells[h][k] > 0)
+              std::fprintf(stderr, "  %6.1f Gcell-updates/s", tcells[h][k] / (tsum[h][k] * 1e6));
+            std::fprintf(stderr, "\n");
+          }
+    }
+    for (auto& hv : tev)
+      for (auto& e : hv)
+        if (e) hipEventDestroy(e);
     {
       std::lock_guard<std::mutex> g(mu);
       stop = true;
@@ -477,12 +513,19 @@ struct Server : ServerBase {
     if (ns > 65535) return fail(MSC_EINVAL, "more than 65535 resident solves");
     size_t nt = 0, nnb = 0;
     double cells1 = 0;  // cells the one-step lists of half 0 update
+    gang_cells[1] = gang_cells[2] = 0;
     for (Req* r : active) {
       nt += pairs(r) ? r->s->pair_host.size() : r->s->tiles_host.size();
       nnb += r->s->ntff_host.size();
       const HostPlan& hp = r->s->plan;
-      cells1 += (double)hp.nx * hp.ny * hp.nz - (pairs(r) ? (double)r->s->pair_cells : 0.0);
+      const double n = (double)hp.nx * hp.ny * hp.nz, nr = pairs(r) ? (double)r->s->pair_cells : 0.0;
+      cells1 += n - nr;
+      if (pairs(r)) {
+        gang_cells[1] += n - nr;
+        gang_cells[2] += nr;
+      }
     }
+    gang_cells[0] = cells1;
     fuse_gang = (fuse_env >= 0 ? fuse_env != 0 : cells1 * sizeof(T) < 64e6) && nt < (size_t)INT32_MAX;
     int rc = ensure(g, std::max<size_t>(ns, 1), std::max<size_t>(nt, 1), std::max<size_t>(nnb, 1));
     if (rc) return rc;
@@ -565,6 +608,8 @@ struct Server : ServerBase {
   // serve every iteration until the gang changes.
   hipError_t launch_fixed(const Gang& g, int half) {
     hipError_t e = hipSuccess;
+    hipEvent_t* te = timing ? tev[half] : nullptr;
+    if (te) e = hipEventRecord(te[0], run);
     for (int c = 0; c < 3 && e == hipSuccess; c++) {
       if (fuse_gang) {
         for (int y = 0; y < 3 && e == hipSuccess; y++) {
@@ -584,11 +629,15 @@ struct Server : ServerBase {
             e = launch_step<T>(g.sa, g.sids + o, kTys[y], c, list_pm(v), false, kServerZLanes, g.tiles + o, n, run);
           }
       }
-      if (!half && n_ptiles[c] && e == hipSuccess)
-        e = launch_pair<T>(g.sa, g.sids + ptile_off[c], c, g.tiles + ptile_off[c], n_ptiles[c], run);
     }
+    if (te && e == hipSuccess) e = hipEventRecord(te[1], run);
+    for (int c = 0; c < 3 && e == hipSuccess; c++)
+      if (!half && n_ptiles[c])
+        e = launch_pair<T>(g.sa, g.sids + ptile_off[c], c, g.tiles + ptile_off[c], n_ptiles[c], run);
+    if (te && e == hipSuccess) e = hipEventRecord(te[2], run);
     if (e == hipSuccess) e = launch_ntff<T>(g.na, g.nb, half ? n_nb_b : n_nb, run);
     if (e == hipSuccess) e = launch_probes<T>(g.xa, half ? (int)n_pair : (int)sel.size(), run);
+    if (te && e == hipSuccess) e = hipEventRecord(te[3], run);
     return e;
   }
 
@@ -745,6 +794,19 @@ struct Server : ServerBase {
         const int t = sel[q]->t + h;
         if ((t + 1) % s.na.K == 0 || t + 1 == s.plan.T)
           HIPCHK(launch_ntff_contract<T>(g.na + q, s.na.uoff[24], s.na.M, s.na.wofs[24], run, 0, s.na.acc32));
+      }
+      if (timing) {
+        hipEvent_t* te = tev[h];
+        HIPCHK(hipEventRecord(te[4], run));
+        HIPCHK(hipEventSynchronize(te[4]));
+        for (int k = 0; k < 4; k++) {
+          float ms = 0;
+          HIPCHK(hipEventElapsedTime(&ms, te[k], te[k + 1]));
+          tsum[h][k] += ms;
+        }
+        tcells[h][0] += gang_cells[h];
+        if (h == 0) tcells[h][1] += 2 * gang_cells[2];
+        ++thalves[h];
       }
     }
     HIPCHK(hipEventRecord(iter_ev[slot], run));
