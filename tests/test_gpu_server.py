@@ -217,6 +217,27 @@ def test_gangs_of_small_regions_run_one_step(fuse, monkeypatch):
         assert st["iterations"] == st["solve_steps"] == sum(v[0]["tsteps"] for v in mix.values())
 
 
+def test_timing_mode_keeps_the_bits_and_reports(capfd, monkeypatch):
+    """MSC_SERVER_TIMING=1 (direct launches with events between the parts of
+    each half) gives the single-solve bits and, when the server closes,
+    prints the GPU time of the one-step lists, the two-step launch, the
+    NTFF samples and probes, and the contractions."""
+    mix = {k: f() for k, f in TWO_STEP_MIX.items()}
+    alone = {k: _alone(*v) for k, v in mix.items()}
+    monkeypatch.setenv("MSC_SERVER_TIMING", "1")
+    monkeypatch.setenv("MSC_SERVER_PAIR_MIN", "0")
+    with Server(precision=32, budget_bytes=4 << 30, fill_tiles=0) as srv:
+        ids = {k: srv.submit(*v) for k, v in mix.items()}
+        srv.run_until_done()
+        for k in mix:
+            np.testing.assert_array_equal(srv.ntff(ids[k]), alone[k][0], err_msg=k)
+            np.testing.assert_array_equal(srv.probes(ids[k]), alone[k][1], err_msg=k)
+    err = capfd.readouterr().err
+    assert "server timing:" in err
+    for part in ("one-step lists", "two-step launch", "NTFF samples + probes"):
+        assert part in err, err
+
+
 def test_fp64_two_step_gangs_equal_alone(monkeypatch):
     """FP64 gangs with two-step tiles (16 rows x 32 lanes, 12 x 28 owned) on
     the 60x50x205 grids (paired with MSC_SERVER_PAIR_MIN=0) equal FP64
