@@ -103,14 +103,15 @@ def source_phasors(ntf: dict):
     return ey, ez
 
 
-def bistatic(ntf, p: dict, theta, phi):
+def bistatic(ntf, p: dict, theta, phi, log=None):
     """sigma (m^2) at directions (theta, phi) [rad] for every NTF frequency.
 
     Returns arrays shaped [n_directions, M]: 'sig' (total, Taflove 8.37),
     'e_theta2' = |L_phi + eta0 N_theta|^2, 'e_phi2' = |L_theta - eta0 N_phi|^2.
     `ntf` may also be a list of payloads of one grid and frequency set (e.g.
     the same solve in several precisions): they share the phase factors, and
-    a list of results comes back.
+    a list of results comes back.  `log`, if given, is called with a line
+    after each surface array.
     """
     many = isinstance(ntf, (list, tuple))
     ntfs = list(ntf) if many else [ntf]
@@ -140,6 +141,8 @@ def bistatic(ntf, p: dict, theta, phi):
         else:
             for m in range(M):
                 acc[:, :, m, comp] += (np.exp(-1j * k[m] * proj) @ zw[m]).T
+        if log:
+            log(f"rcs: {kind}{'xyz'[comp]} surface array of {pos.shape[0]} samples done")
     out = []
     for q, n in enumerate(ntfs):
         Lth = np.einsum("dmc,dc->dm", L[q], thhat)
@@ -209,7 +212,7 @@ def payload_to_ntf(payload: np.ndarray, p: dict) -> dict:
     return runio.parse_ntf(runio.encode_ntf(p["nntf"], b, p["fntf"], payload), p["nx"], p["ny"], p["nz"])
 
 
-def db_errors(payload, reference: np.ndarray, p: dict, floor_db: float = 40.0, step_deg: float = 1.0):
+def db_errors(payload, reference: np.ndarray, p: dict, floor_db: float = 40.0, step_deg: float = 1.0, log=None):
     """RCS of `payload` against that of `reference` (same solve), in dB.
 
     Per frequency: the largest |dB difference| of the bistatic RCS over the
@@ -222,7 +225,7 @@ def db_errors(payload, reference: np.ndarray, p: dict, floor_db: float = 40.0, s
     pays = list(payload) if many else [payload]
     ntfs = [payload_to_ntf(a, p) for a in pays] + [payload_to_ntf(reference, p)]
     th, ph, cuts = principal_cuts(step_deg)
-    dbs = [db(r["sig"]) for r in bistatic(ntfs, p, th, ph)]
+    dbs = [db(r["sig"]) for r in bistatic(ntfs, p, th, ph, log=log)]
     dr = dbs[-1]
     mr = db(monostatic(ntfs[-1], p)["sig"])
     out = []
